@@ -1,0 +1,136 @@
+"""Host-side driver of the CPU oracle (test infrastructure only).
+
+Restates the reference's preparation and back-transformation around the C
+kernel (pyscfad/cc/ccsd_t.py:82-192) in NumPy, and calls the C restatement
+``ccsd_t_oracle.c`` through ctypes, the same way the reference calls its
+``ccsd_t_energy_vjp`` (ccsd_t.py:107-133).
+
+Inputs use the reference's tensor conventions (pyscfad/cc/ccsd.py:470-498):
+    t1 (o,v), t2 (o,o,v,v), mo_energy (n,), fock (n,n), ovoo (o,v,o,o),
+    ovov (o,v,o,v), ovvv packed (o,v,v(v+1)/2) or unpacked (o,v,v,v).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import build as _build
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        path = _build.LIB
+        if not os.path.exists(path):
+            _build.build()
+        lib = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.oracle_ccsd_t_energy_range.restype = ctypes.c_double
+        lib.oracle_ccsd_t_energy_range.argtypes = [ctypes.c_int, ctypes.c_int] + [dp] * 6 + [
+            ctypes.c_long, ctypes.c_long]
+        lib.oracle_ccsd_t_energy_vjp_range.restype = None
+        lib.oracle_ccsd_t_energy_vjp_range.argtypes = (
+            [ctypes.c_int, ctypes.c_int] + [dp] * 6 + [ctypes.c_double] + [dp] * 6
+            + [ctypes.c_long, ctypes.c_long])
+        lib.oracle_njobs.restype = ctypes.c_long
+        lib.oracle_njobs.argtypes = [ctypes.c_int]
+        lib.oracle_num_threads.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def unpack_ovvv(ovvv, nvir):
+    """(o,v,v(v+1)/2) lower-triangle packed -> (o,v,v,v), symmetric in the last
+    two indices (pyscf lib.unpack_tril, used by eris.get_ovvv at ccsd_t.py:103).
+    The pair (f,b) with f>=b is stored at f*(f+1)/2 + b."""
+    ovvv = np.asarray(ovvv)
+    if ovvv.ndim == 4:
+        return ovvv
+    idx_f, idx_b = np.tril_indices(nvir)
+    full = np.empty(ovvv.shape[:2] + (nvir, nvir), dtype=np.float64)
+    full[:, :, idx_f, idx_b] = ovvv
+    full[:, :, idx_b, idx_f] = ovvv
+    return full
+
+
+def pack_ovvv_bar(ovvv_bar_full):
+    """Adjoint of unpack_ovvv: off-diagonal pairs summed, diagonal kept
+    (ccsd_t.py:186-191; equals the unpack_tril VJP, _numpy_helper_opt.py:32-55)."""
+    nvir = ovvv_bar_full.shape[-1]
+    x = ovvv_bar_full + ovvv_bar_full.transpose(0, 1, 3, 2)
+    di = np.arange(nvir)
+    x[:, :, di, di] *= 0.5
+    idx_f, idx_b = np.tril_indices(nvir)
+    return np.ascontiguousarray(x[:, :, idx_f, idx_b])
+
+
+def host_arrays(t1, t2, mo_energy, fock, ovoo, ovov, ovvv):
+    """The reference's host arrays (ccsd_t.py:88-106)."""
+    nocc, nvir = t1.shape
+    nmo = nocc + nvir
+    t1T = np.ascontiguousarray(np.asarray(t1, dtype=np.float64).T)
+    t2T = np.ascontiguousarray(np.asarray(t2, dtype=np.float64).transpose(2, 3, 1, 0))
+    mo_energy = np.ascontiguousarray(mo_energy, dtype=np.float64)
+    fvo = np.ascontiguousarray(np.asarray(fock, dtype=np.float64)[nocc:, :nocc])
+    vooo = np.ascontiguousarray(np.asarray(ovoo, dtype=np.float64).transpose(1, 0, 3, 2))
+    vvop = np.empty((nvir, nvir, nocc, nmo))
+    vvop[:, :, :, :nocc] = np.asarray(ovov, dtype=np.float64).transpose(1, 3, 0, 2)
+    vvop[:, :, :, nocc:] = unpack_ovvv(ovvv, nvir).transpose(1, 3, 0, 2)
+    return t1T, t2T, mo_energy, fvo, vooo, vvop
+
+
+def njobs(nvir):
+    return _load().oracle_njobs(nvir)
+
+
+def num_threads():
+    return _load().oracle_num_threads()
+
+
+def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, job_range=None):
+    """E(T).  ``job_range=(j0, j1)`` restricts the sum to a slice of the
+    a>=b>=c job list (used for bounded CPU timing samples)."""
+    nocc, nvir = np.shape(t1)
+    t1T, t2T, me, fvo, vooo, vvop = host_arrays(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+    j0, j1 = job_range if job_range is not None else (0, njobs(nvir))
+    return _load().oracle_ccsd_t_energy_range(nocc, nvir, _ptr(me), _ptr(t1T), _ptr(t2T),
+                                              _ptr(vooo), _ptr(fvo), _ptr(vvop), j0, j1)
+
+
+def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, job_range=None):
+    """Cotangents of E(T), in the reference's output conventions
+    (ccsd_t.py:179-192): dict with t1, t2, fock (full (n,n), only the vo block
+    nonzero), mo_energy, ovoo, ovov, ovvv (packed if the input was packed)."""
+    nocc, nvir = np.shape(t1)
+    nmo = nocc + nvir
+    t1T, t2T, me, fvo, vooo, vvop = host_arrays(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+    me_b = np.zeros_like(me)
+    t1T_b = np.zeros_like(t1T)
+    t2T_b = np.zeros_like(t2T)
+    vooo_b = np.zeros_like(vooo)
+    fvo_b = np.zeros_like(fvo)
+    vvop_b = np.zeros_like(vvop)
+    j0, j1 = job_range if job_range is not None else (0, njobs(nvir))
+    _load().oracle_ccsd_t_energy_vjp_range(
+        nocc, nvir, _ptr(me), _ptr(t1T), _ptr(t2T), _ptr(vooo), _ptr(fvo), _ptr(vvop),
+        float(et_bar), _ptr(me_b), _ptr(t1T_b), _ptr(t2T_b), _ptr(vooo_b), _ptr(fvo_b),
+        _ptr(vvop_b), j0, j1)
+    fock_b = np.zeros((nmo, nmo))
+    fock_b[nocc:, :nocc] = fvo_b
+    ovvv_full_b = vvop_b[:, :, :, nocc:].transpose(2, 0, 3, 1)
+    packed = np.ndim(ovvv) == 3
+    return {
+        "t1": np.ascontiguousarray(t1T_b.T),
+        "t2": np.ascontiguousarray(t2T_b.transpose(3, 2, 0, 1)),
+        "fock": fock_b,
+        "mo_energy": me_b,
+        "ovoo": np.ascontiguousarray(vooo_b.transpose(1, 0, 3, 2)),
+        "ovov": np.ascontiguousarray(vvop_b[:, :, :, :nocc].transpose(2, 0, 3, 1)),
+        "ovvv": pack_ovvv_bar(ovvv_full_b) if packed else np.ascontiguousarray(ovvv_full_b),
+    }

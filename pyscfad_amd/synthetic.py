@@ -1,0 +1,126 @@
+"""Seeded synthetic (T) inputs with the structure of canonical RHF-CCSD data.
+
+One generator for every synthetic workload (single instances and the batched
+set); all randomness comes from ``numpy.random.default_rng(seed)`` (PCG64) on
+the host, so a seed fixes the instance bit for bit wherever it is built.
+
+For an instance with ``nocc = o``, ``nvir = v``, ``n = o + v``:
+  * mo_energy: occupied ~ U[-1.5, -0.3] Eh, virtual ~ U[0.1, 3.0] Eh, each
+    sorted ascending (so every denominator satisfies |D| >= 1.2 Eh).
+  * a density-fitting factor L of shape (naux = 3n, n, n): the upper triangle
+    (diagonal included) is drawn from N(0, 0.05^2) -- 0.05 is the standard
+    deviation -- and mirrored, so L[P,p,q] = L[P,q,p].
+  * integrals in chemists' notation built from L (never the full n^4 tensor):
+        ovov[i,a,j,b] = sum_P L[P,i,o+a] L[P,j,o+b]
+        ovoo[i,a,k,j] = sum_P L[P,i,o+a] L[P,k,j]
+        ovvv[i,a,b,c] = sum_P L[P,i,o+a] L[P,o+b,o+c], stored lower-triangle
+                        packed over (b,c) as (o, v, v(v+1)/2), b >= c.
+  * t1 ~ N(0, 1e-2^2).
+  * t2 = ovov.transpose(0,2,1,3) / D_ijab + N(0, 1e-3^2), then symmetrised
+    t2 <- (t2 + t2.transpose(1,0,3,2)) / 2, so t2[i,j,a,b] = t2[j,i,b,a];
+    D_ijab = e_i + e_j - e_a - e_b.
+  * fock = diag(mo_energy) (canonical, so the vo block is zero); with
+    ``fvo_scale > 0`` the vo and ov blocks get N(0, fvo_scale^2) entries to
+    exercise the t2*fvo term.
+Draw order: occupied energies, virtual energies, L, t1, t2 noise, fvo.
+
+Large instances can build the integral slices with torch on a GPU
+(``device='cuda'``); the random draws are identical either way.
+"""
+import numpy as np
+
+
+def _draws(nocc, nvir, seed, fvo_scale):
+    rng = np.random.default_rng(seed)
+    n = nocc + nvir
+    e_occ = np.sort(rng.uniform(-1.5, -0.3, nocc))
+    e_vir = np.sort(rng.uniform(0.1, 3.0, nvir))
+    mo_energy = np.concatenate([e_occ, e_vir])
+    naux = 3 * n
+    iu = np.triu_indices(n)
+    L = np.zeros((naux, n, n))
+    upper = rng.normal(0.0, 0.05, size=(naux, iu[0].size))
+    L[:, iu[0], iu[1]] = upper
+    L[:, iu[1], iu[0]] = upper
+    del upper
+    t1 = rng.normal(0.0, 1e-2, size=(nocc, nvir))
+    t2_noise = rng.normal(0.0, 1e-3, size=(nocc, nocc, nvir, nvir))
+    fvo = rng.normal(0.0, fvo_scale, size=(nvir, nocc)) if fvo_scale > 0 else None
+    return mo_energy, L, t1, t2_noise, fvo
+
+
+def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None):
+    """Return a dict with t1, t2, mo_energy, fock, ovoo, ovov, ovvv (FP64).
+
+    With ``device=None`` everything is NumPy on the host.  With a torch device
+    string (e.g. 'cuda') the arrays are torch FP64 tensors on that device and
+    the O(o v^3) slice is built there.
+    """
+    o, v = nocc, nvir
+    mo_energy, L, t1, t2_noise, fvo = _draws(o, v, seed, fvo_scale)
+    n = o + v
+    fock = np.diag(mo_energy)
+    if fvo is not None:
+        fock[o:, :o] = fvo
+        fock[:o, o:] = fvo.T
+    e_occ, e_vir = mo_energy[:o], mo_energy[o:]
+    d_ijab = (e_occ[:, None, None, None] + e_occ[None, :, None, None]
+              - e_vir[None, None, :, None] - e_vir[None, None, None, :])
+    idx_b, idx_c = np.tril_indices(v)[0], np.tril_indices(v)[1]
+    if device is None:
+        Lov = L[:, :o, o:].reshape(L.shape[0], o * v)
+        ovov = (Lov.T @ Lov).reshape(o, v, o, v)
+        ovoo = (Lov.T @ L[:, :o, :o].reshape(L.shape[0], o * o)).reshape(o, v, o, o)
+        Lvv = L[:, o:, o:]
+        if packed:
+            ovvv = (Lov.T @ Lvv[:, idx_b, idx_c]).reshape(o, v, v * (v + 1) // 2)
+        else:
+            ovvv = (Lov.T @ Lvv.reshape(L.shape[0], v * v)).reshape(o, v, v, v)
+        t2 = ovov.transpose(0, 2, 1, 3) / d_ijab + t2_noise
+        t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+        return dict(t1=t1, t2=np.ascontiguousarray(t2), mo_energy=mo_energy, fock=fock,
+                    ovoo=ovoo, ovov=ovov, ovvv=ovvv)
+    import torch
+    dev = torch.device(device)
+    Lt = torch.from_numpy(L).to(dev)
+    del L
+    naux = Lt.shape[0]
+    Lov = Lt[:, :o, o:].reshape(naux, o * v)
+    ovov = (Lov.T @ Lov).reshape(o, v, o, v)
+    ovoo = (Lov.T @ Lt[:, :o, :o].reshape(naux, o * o)).reshape(o, v, o, o)
+    Lvv = Lt[:, o:, o:]
+    if packed:
+        ib = torch.from_numpy(idx_b).to(dev)
+        ic = torch.from_numpy(idx_c).to(dev)
+        Lvv_p = Lvv[:, ib, ic]
+        ovvv = torch.empty((o, v, v * (v + 1) // 2), dtype=torch.float64, device=dev)
+        # build one occupied slab at a time to bound the temporary
+        for i in range(o):
+            ovvv[i] = Lov[:, i * v:(i + 1) * v].T @ Lvv_p
+        del Lvv_p
+    else:
+        ovvv = (Lov.T @ Lvv.reshape(naux, v * v)).reshape(o, v, v, v)
+    d = torch.from_numpy(d_ijab).to(dev)
+    t2 = ovov.permute(0, 2, 1, 3) / d + torch.from_numpy(t2_noise).to(dev)
+    t2 = (0.5 * (t2 + t2.permute(1, 0, 3, 2))).contiguous()
+    return dict(t1=torch.from_numpy(t1).to(dev), t2=t2,
+                mo_energy=torch.from_numpy(mo_energy).to(dev),
+                fock=torch.from_numpy(fock).to(dev), ovoo=ovoo.contiguous(),
+                ovov=ovov.contiguous(), ovvv=ovvv)
+
+
+def batched_shapes(count=64, seed=4, nocc_range=(4, 24), nvir_range=(40, 200)):
+    """(nocc, nvir, seed) of each instance of the batched workload: sizes drawn
+    uniformly from the given inclusive ranges with ``seed``, instance k built
+    with seed 4000 + k."""
+    rng = np.random.default_rng(seed)
+    occ = rng.integers(nocc_range[0], nocc_range[1] + 1, size=count)
+    vir = rng.integers(nvir_range[0], nvir_range[1] + 1, size=count)
+    return [(int(o), int(v), 4000 + k) for k, (o, v) in enumerate(zip(occ, vir))]
+
+
+def flops_energy(nocc, nvir):
+    """Algorithmic flops of the E(T) contraction, F_E = 2 o^3 v^3 (v + o):
+    v^3 W evaluations of a (o^2 x v)(v x o) and an (o x o)(o x o^2) product."""
+    o, v = float(nocc), float(nvir)
+    return 2.0 * o ** 3 * v ** 3 * (v + o)
