@@ -1,0 +1,90 @@
+/*
+ * C ABI of the MI355X (gfx950) CCSD(T) perturbative-triples energy and VJP.
+ *
+ * All tensor arguments are DEVICE pointers to C-contiguous FP64 arrays in the
+ * reference's own conventions (pyscfad/cc/ccsd.py:470-498, ccsd_t.py:88-106):
+ *     mo_energy (n)            t1 (o,v)              t2 (o,o,v,v)
+ *     fock (n,n)               ovoo (o,v,o,o)        ovov (o,v,o,v)
+ *     ovvv (o,v,v(v+1)/2) lower-triangle packed over the last pair
+ *          (ovvv_packed = 1), or (o,v,v,v) (ovvv_packed = 0)
+ * with o = nocc, v = nvir, n = o + v.  Work is enqueued on `stream` (a
+ * hipStream_t passed as void*; NULL is the default stream) and the calls
+ * return once it is enqueued and the workspace released (a call that
+ * allocates its own workspace synchronises the stream before freeing it).
+ *
+ * Return value: PYSCFAD_AMD_OK or an error code; pyscfad_amd_last_error()
+ * describes the last failure of the calling thread.
+ */
+#ifndef PYSCFAD_AMD_H
+#define PYSCFAD_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYSCFAD_AMD_OK 0
+#define PYSCFAD_AMD_ERR_ARGS 1        /* bad dimensions, null pointer, bad flag */
+#define PYSCFAD_AMD_ERR_PRECISION 2   /* unknown precision mode */
+#define PYSCFAD_AMD_ERR_OOM 3         /* device allocation failed */
+#define PYSCFAD_AMD_ERR_WORKSPACE 4   /* caller workspace smaller than required */
+#define PYSCFAD_AMD_ERR_HIP 5         /* a HIP runtime or launch error */
+
+/* Arithmetic of the W/V contractions.  Denominators, the E(T) sum and every
+ * cross-block cotangent accumulation are FP64 in all modes. */
+#define PYSCFAD_AMD_PREC_F64 0        /* FP64 MFMA throughout: the drop-in path */
+#define PYSCFAD_AMD_PREC_F32 1        /* FP32 MFMA contractions */
+#define PYSCFAD_AMD_PREC_BF16X3 2     /* split-bf16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulate */
+
+/*
+ * E(T) of one (t1, t2, eris) instance, written to e_out (DEVICE pointer to one
+ * double).  Replaces the forward of the custom_vjp, pyscf's ccsd_t.kernel
+ * called at pyscfad/cc/ccsd_t.py:41 (pyscf libcc CCsd_t_contract).
+ * workspace may be NULL (the library allocates and frees it); otherwise it
+ * must hold pyscfad_amd_ccsd_t_workspace_bytes(nocc, nvir, precision, 0) bytes.
+ */
+int pyscfad_amd_ccsd_t_energy(int nocc, int nvir, const double *mo_energy, const double *t1,
+                              const double *t2, const double *fock, const double *ovoo,
+                              const double *ovov, const double *ovvv, int ovvv_packed,
+                              int precision, double *e_out, void *workspace,
+                              size_t workspace_bytes, void *stream);
+
+/*
+ * Cotangents of E(T) for the upstream cotangent et_bar.  Replaces the
+ * backward of the custom_vjp: the host preparation, block loop and
+ * back-transformation of _ccsd_t_energy_vjp (pyscfad/cc/ccsd_t.py:82-192)
+ * together with the C kernel it calls, ccsd_t_energy_vjp
+ * (pyscfadlib/pyscfadlib/vjp/cc/ccsd_t_vjp.c:332-346).
+ * Outputs (DEVICE, FP64, overwritten) have the reference's shapes and element
+ * conventions (ccsd_t.py:179-192):
+ *     t1_bar (o,v), t2_bar (o,o,v,v), fock_bar (n,n) (only the [o:, :o] block
+ *     nonzero), mo_energy_bar (n), ovoo_bar (o,v,o,o), ovov_bar (o,v,o,v),
+ *     ovvv_bar packed (o,v,v(v+1)/2) if ovvv_packed else (o,v,v,v).
+ * e_out may be NULL; otherwise it receives E(T) of the same instance.
+ */
+int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double *mo_energy, const double *t1,
+                                  const double *t2, const double *fock, const double *ovoo,
+                                  const double *ovov, const double *ovvv, int ovvv_packed,
+                                  double et_bar, int precision, double *t1_bar, double *t2_bar,
+                                  double *fock_bar, double *mo_energy_bar, double *ovoo_bar,
+                                  double *ovov_bar, double *ovvv_bar, double *e_out,
+                                  void *workspace, size_t workspace_bytes, void *stream);
+
+/* Workspace needed by the calls above (with_vjp = 0: energy, 1: energy_vjp);
+ * 0 for invalid arguments.  No counterpart in the reference, whose C kernel
+ * mallocs its own scratch (ccsd_t_vjp.c:352-444); this follows the XLA FFI
+ * scratch-allocator pattern of pyscfadlib/pyscfadlib/ffi_helpers.h:40-50. */
+size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp);
+
+/* Text of the last error of the calling thread ("" if none). */
+const char *pyscfad_amd_last_error(void);
+
+/* ABI version, currently 1. */
+int pyscfad_amd_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYSCFAD_AMD_H */

@@ -1,0 +1,57 @@
+"""ctypes binding of libpyscfad_amd.so (include/pyscfad_amd.h).
+
+Loading fails loudly: there is no CPU fallback for the product path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpyscfad_amd.so")
+
+PREC_F64 = 0
+PREC_F32 = 1
+PREC_BF16X3 = 2
+PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
+
+# every symbol include/pyscfad_amd.h declares
+EXPORTS = (
+    "pyscfad_amd_ccsd_t_energy",
+    "pyscfad_amd_ccsd_t_energy_vjp",
+    "pyscfad_amd_ccsd_t_workspace_bytes",
+    "pyscfad_amd_last_error",
+    "pyscfad_amd_abi_version",
+)
+
+_lib = None
+
+
+class T3Error(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise T3Error(f"{LIB_PATH} is missing: build it with `python -m pyscfad_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, dp, i, d, sz = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    lib.pyscfad_amd_ccsd_t_energy.restype = i
+    lib.pyscfad_amd_ccsd_t_energy.argtypes = [i, i] + [dp] * 7 + [i, i, dp, vp, sz, vp]
+    lib.pyscfad_amd_ccsd_t_energy_vjp.restype = i
+    lib.pyscfad_amd_ccsd_t_energy_vjp.argtypes = [i, i] + [dp] * 7 + [i, d, i] + [dp] * 8 + [vp, sz, vp]
+    lib.pyscfad_amd_ccsd_t_workspace_bytes.restype = sz
+    lib.pyscfad_amd_ccsd_t_workspace_bytes.argtypes = [i, i, i, i]
+    lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
+    lib.pyscfad_amd_last_error.argtypes = []
+    lib.pyscfad_amd_abi_version.restype = i
+    lib.pyscfad_amd_abi_version.argtypes = []
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().pyscfad_amd_last_error().decode()
+        raise T3Error(f"pyscfad_amd error {rc}: {msg}")

@@ -1,0 +1,201 @@
+"""Drop-in for ``pyscfad.cc.ccsd_t`` (pyscfad/cc/ccsd_t.py) on MI355X.
+
+Same entry points, argument meaning and cotangent conventions as the
+reference:
+
+* ``kernel(mycc, eris, t1=None, t2=None, verbose=None)`` -- E(T) with a
+  ``jax.custom_vjp`` over ``(eris, t1, t2)`` whose backward maps the
+  cotangents onto the eris leaves by name and zero-fills the others
+  (ccsd_t.py:28-80).  JAX is optional: without it ``kernel`` returns E(T)
+  as a float.
+* ``_ccsd_t_energy_vjp(eris, t1, t2, et_bar, max_memory)`` -- returns
+  ``(t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar)``
+  in the layouts of ccsd_t.py:179-192 (ovvv_bar lower-triangle packed).
+* ``CCSDTEnergy`` -- the same pair as a ``torch.autograd.Function`` over
+  ``(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)``.
+
+Both passes run on the GPU through the C ABI of ``libpyscfad_amd.so``;
+arrays may be NumPy (copied to the device) or torch tensors (used in place
+when already contiguous FP64 on the device).  ``precision`` selects the
+arithmetic of the W/V contractions: 'f64' (default, the drop-in), 'f32' or
+'bf16x3'; denominators, the E(T) sum and cotangent accumulation stay FP64.
+"""
+import numpy
+import torch
+
+from .. import _lib
+
+DEFAULT_PRECISION = "f64"
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise _lib.T3Error("pyscfad_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _dev(x):
+    """Contiguous FP64 device tensor view/copy of x."""
+    dev = _device()
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float64).contiguous()
+    return torch.as_tensor(numpy.ascontiguousarray(x, dtype=numpy.float64)).to(dev)
+
+
+def _ptr(t):
+    return t.data_ptr()
+
+
+def _prec(precision):
+    if isinstance(precision, int):
+        return precision
+    try:
+        return _lib.PRECISIONS[precision.lower()]
+    except KeyError:
+        raise _lib.T3Error(f"unknown precision {precision!r}; use one of {sorted(_lib.PRECISIONS)}")
+
+
+def _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv):
+    t1d, t2d = _dev(t1), _dev(t2)
+    nocc, nvir = t1d.shape
+    me, fk, oo, ov, vv = (_dev(mo_energy), _dev(fock), _dev(ovoo), _dev(ovov), _dev(ovvv))
+    n = nocc + nvir
+    packed = vv.dim() == 3
+    expect = {
+        "t2": (t2d, (nocc, nocc, nvir, nvir)), "mo_energy": (me, (n,)), "fock": (fk, (n, n)),
+        "ovoo": (oo, (nocc, nvir, nocc, nocc)), "ovov": (ov, (nocc, nvir, nocc, nvir)),
+        "ovvv": (vv, (nocc, nvir, nvir * (nvir + 1) // 2) if packed else (nocc, nvir, nvir, nvir)),
+    }
+    for name, (arr, shape) in expect.items():
+        if tuple(arr.shape) != shape:
+            raise ValueError(f"{name} has shape {tuple(arr.shape)}, expected {shape}")
+    return nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed
+
+
+def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISION, stream=None):
+    """E(T) as a 0-d FP64 device tensor (the forward of the custom_vjp)."""
+    nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    lib = _lib.load()
+    _lib.check(lib.pyscfad_amd_ccsd_t_energy(
+        nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
+        int(packed), _prec(precision), _ptr(out), None, 0, s.cuda_stream))
+    return out
+
+
+def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=DEFAULT_PRECISION,
+               stream=None, return_energy=False):
+    """Cotangents of E(T) as FP64 device tensors, in the order and layouts of
+    the reference's _ccsd_t_energy_vjp: (t1, t2, fock, mo_energy, ovoo, ovov, ovvv)."""
+    nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+    z = lambda a: torch.empty_like(a)
+    bars = [z(t1d), z(t2d), z(fk), z(me), z(oo), z(ov), z(vv)]
+    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    lib = _lib.load()
+    if hasattr(et_bar, "item"):
+        et_bar = et_bar.item()
+    _lib.check(lib.pyscfad_amd_ccsd_t_energy_vjp(
+        nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
+        int(packed), float(et_bar), _prec(precision), *[_ptr(b) for b in bars], _ptr(out),
+        None, 0, s.cuda_stream))
+    if return_energy:
+        return tuple(bars), out
+    return tuple(bars)
+
+
+def workspace_bytes(nocc, nvir, precision=DEFAULT_PRECISION, with_vjp=True):
+    return _lib.load().pyscfad_amd_ccsd_t_workspace_bytes(nocc, nvir, _prec(precision), int(with_vjp))
+
+
+def _ccsd_t_energy_vjp(eris, t1, t2, et_bar, max_memory=None, precision=DEFAULT_PRECISION):
+    """Same signature and return value as pyscfad/cc/ccsd_t.py:82 (max_memory is
+    accepted for compatibility; device memory is sized by the library).
+    Returns NumPy arrays, like the reference."""
+    bars = energy_vjp(t1, t2, eris.mo_energy, eris.fock, eris.ovoo, eris.ovov, eris.ovvv,
+                      et_bar=et_bar, precision=precision)
+    return tuple(b.cpu().numpy() for b in bars)
+
+
+class CCSDTEnergy(torch.autograd.Function):
+    """E(T) with the hand-written VJP, as a torch autograd function of
+    (t1, t2, mo_energy, fock, ovoo, ovov, ovvv)."""
+
+    @staticmethod
+    def forward(ctx, t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISION):
+        ctx.save_for_backward(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+        ctx.precision = precision
+        return energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=precision)
+
+    @staticmethod
+    def backward(ctx, gbar):
+        t1, t2, me, fk, oo, ov, vv = ctx.saved_tensors
+        t1b, t2b, fkb, meb, oob, ovb, vvb = energy_vjp(t1, t2, me, fk, oo, ov, vv,
+                                                        et_bar=float(gbar), precision=ctx.precision)
+        return t1b, t2b, meb, fkb, oob, ovb, vvb, None
+
+
+class ERIs:
+    """Minimal container with the eris leaves the (T) path reads
+    (the dynamic attributes of pyscfad's _ChemistsERIs, ccsd.py:30-31)."""
+
+    def __init__(self, fock, mo_energy, ovoo, ovov, ovvv, **others):
+        self.fock = fock
+        self.mo_energy = mo_energy
+        self.ovoo = ovoo
+        self.ovov = ovov
+        self.ovvv = ovvv
+        for k, val in others.items():
+            setattr(self, k, val)
+
+
+def kernel(mycc, eris, t1=None, t2=None, verbose=None, precision=DEFAULT_PRECISION):
+    """E(T) of ``mycc``'s amplitudes (pyscfad/cc/ccsd_t.py:28).
+
+    With JAX importable this is a ``custom_vjp`` over (eris, t1, t2) whose
+    backward returns cotangents for the leaves fock, mo_energy, ovoo, ovov and
+    ovvv by name and zeros for every other leaf (ccsd_t.py:48-77).
+    """
+    if t1 is None:
+        t1 = mycc.t1
+    if t2 is None:
+        t2 = mycc.t2
+    try:
+        import jax
+        from jax import custom_vjp
+        from jax.tree_util import tree_flatten_with_path, tree_unflatten
+    except ImportError:
+        e = energy(t1, t2, eris.mo_energy, eris.fock, eris.ovoo, eris.ovov, eris.ovvv, precision=precision)
+        return float(e)
+
+    @custom_vjp
+    def _ccsd_t_kernel(eris, t1, t2):
+        e = energy(numpy.asarray(t1), numpy.asarray(t2), numpy.asarray(eris.mo_energy),
+                   numpy.asarray(eris.fock), numpy.asarray(eris.ovoo), numpy.asarray(eris.ovov),
+                   numpy.asarray(eris.ovvv), precision=precision)
+        return float(e)
+
+    def _fwd(eris, t1, t2):
+        return _ccsd_t_kernel(eris, t1, t2), (eris, t1, t2)
+
+    def _bwd(res, ybar):
+        eris, t1, t2 = res
+        path_vals, treedef = tree_flatten_with_path(eris)
+        keys = [item[0][0].name for item in path_vals]
+        shapes = [item[1].shape for item in path_vals]
+        t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar = _ccsd_t_energy_vjp(
+            eris, numpy.asarray(t1), numpy.asarray(t2), float(ybar), precision=precision)
+        key_to_bar = {"fock": fock_bar, "mo_energy": mo_energy_bar, "ovoo": ovoo_bar,
+                      "ovov": ovov_bar, "ovvv": ovvv_bar}
+        leaves = [None] * len(keys)
+        for k, val in key_to_bar.items():
+            leaves[keys.index(k)] = val
+        for idx, leaf in enumerate(leaves):
+            if leaf is None:
+                leaves[idx] = numpy.zeros(shapes[idx])
+        return tree_unflatten(treedef, leaves), t1_bar, t2_bar
+
+    _ccsd_t_kernel.defvjp(_fwd, _bwd)
+    del jax
+    return _ccsd_t_kernel(eris, t1, t2)

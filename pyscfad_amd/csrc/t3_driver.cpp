@@ -1,0 +1,774 @@
+// Host driver of the gfx950 CCSD(T) (T) energy and VJP: workspace layout,
+// the static launch plan over unique occupied triples, and the C ABI
+// declared in include/pyscfad_amd.h.
+//
+// Algorithm (unique occupied triples i >= j >= k, in batches of B triples):
+//   1. W-build: two grouped GEMM launches produce, per triple, the cube
+//      w0[a][b][c] = sum over the six simultaneous orderings rho of
+//        X_{t_rho}[s_rho],  X_{pqr}[x,y,z] = sum_f ovvv[p,x,f,y] t2[r,q,z,f]
+//                                            - sum_m ovoo[p,x,m,q] t2[r,m,z,y]
+//      (the W of ccsd_t.c:121-152 for every virtual triple at once).
+//   2. Orbit epilogue (t3_epilogue.hip): V term, r3, FP64 denominators, E(T)
+//      partials; backward also G = dE/dw0, H = dE/dv0 and mo_energy partials.
+//   3. Backward only: the adjoint contractions of ccsd_t_vjp.c:139-246 over G as
+//      grouped GEMMs into FP64 accumulators, and over H as memory-bound kernels.
+//   4. Backward only: finalisation into the reference's cotangent layouts
+//      (ccsd_t.py:179-192).
+// Every output element has one writer per launch and launches are stream
+// ordered, so results are deterministic and no atomics are used.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "t3_common.h"
+#include "../../include/pyscfad_amd.h"
+
+using namespace t3;
+
+extern "C" {
+int t3_gemm_tile_m(int cfg);
+int t3_gemm_tile_n(int cfg);
+int t3_gemm_chunk(int prec);
+hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob, const GemmSeg* segs,
+                          long long nblocks, hipStream_t stream);
+hipError_t t3_launch_reduce(int prec, const ReduceProb* probs, int nprob, long long total, hipStream_t stream);
+hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, const EpiTriple* triples, int ntriple,
+                              hipStream_t stream);
+hipError_t t3_launch_epi_reduce(const EpiReduceArgs* a, hipStream_t stream);
+hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void* ovooT, double* eps_occ,
+                          double* eps_vir, const double* ovvv, int packed, const double* t2, const double* ovoo,
+                          const double* mo_energy, int o, int v, int Op, int V, hipStream_t s);
+hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs, int nslab,
+                           const VpContrib* contribs, int o, int v, int V, const double* t1, const double* fock,
+                           const double* ovov, const double* t2, double* ovov_acc, double* t2v_acc,
+                           double* t1_acc, double* fvo_acc, hipStream_t s);
+hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
+                       double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
+                       const double* t2hacc, const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
+                       const double* t1_acc, const double* fvo_acc, const double* eps_bar, int o, int v, int Op,
+                       int V, hipStream_t s);
+}
+
+namespace {
+
+constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr int GEMM_TARGET_BLOCKS = 2048;
+constexpr int MAX_KSPLIT = 16;
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+inline long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
+
+struct Dims {
+  int o, v, n, V, Op, prec, BK;
+  size_t elt;      // bytes of a contraction operand (and GEMM accumulator) element
+  int tp_is_f64;
+  int ntile, ncube;
+};
+
+Dims make_dims(int o, int v, int prec) {
+  Dims d;
+  d.o = o;
+  d.v = v;
+  d.n = o + v;
+  d.prec = prec;
+  d.BK = t3_gemm_chunk(prec);
+  d.V = (int)round_up(v, 32);
+  d.Op = (int)round_up(o, d.BK);
+  d.tp_is_f64 = prec == PREC_F64;
+  d.elt = d.tp_is_f64 ? 8 : 4;
+  d.ntile = d.V / EPI_T;
+  d.ncube = d.ntile * (d.ntile + 1) * (d.ntile + 2) / 6;
+  return d;
+}
+
+// ----------------------------------------------------------------------------
+// Workspace: one device allocation carved into named regions.
+struct Layout {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    size_t o = off;
+    off = (size_t)round_up((long long)(off + bytes), 256);
+    return o;
+  }
+};
+
+struct Regions {
+  size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, energy;
+  size_t ovvv_acc, t2acc, t2hacc, t2vacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
+  size_t slots, slot_bytes;
+  size_t slot_w0, slot_G, slot_H, slot_part, slot_vp;
+  size_t split, split_bytes;
+  size_t desc, desc_bytes;
+  size_t total;
+};
+
+int choose_batch(const Dims& d) {
+  // enough W-build blocks (V x (V/64)^2 per triple) to keep 256 CUs busy
+  const long long tiles = (long long)d.V * ((d.V + 63) / 64) * ((d.V + 63) / 64);
+  long long b = (8192 + tiles - 1) / tiles;
+  return (int)std::max<long long>(1, std::min<long long>(b, 16));
+}
+
+Regions plan_regions(const Dims& d, int B, bool backward) {
+  Layout L;
+  Regions r{};
+  const size_t V = d.V, o = d.o, Op = d.Op;
+  r.ovvvT = L.take(o * V * V * V * d.elt);
+  r.t2T = L.take(Op * Op * V * V * d.elt);
+  r.t2H = L.take(Op * Op * V * V * d.elt);
+  r.ovooT = L.take(o * o * V * Op * d.elt);
+  r.eps_occ = L.take(o * 8);
+  r.eps_vir = L.take(V * 8);
+  r.energy = L.take(8);
+  if (backward) {
+    r.ovvv_acc = L.take(o * V * V * V * 8);
+    r.t2acc = L.take(Op * Op * V * V * 8);
+    r.t2hacc = L.take(Op * Op * V * V * 8);
+    r.t2vacc = L.take(o * o * V * V * 8);
+    r.ovov_acc = L.take(o * o * V * V * 8);
+    r.ovoo_acc = L.take(o * o * V * Op * 8);
+    r.t1_acc = L.take(o * V * 8);
+    r.fvo_acc = L.take(o * V * 8);
+    r.eps_bar = L.take((o + V) * 8);
+  }
+  Layout S;
+  r.slot_w0 = S.take(V * V * V * d.elt);
+  if (backward) {
+    r.slot_G = S.take(V * V * V * d.elt);
+    r.slot_H = S.take(V * V * V * d.elt);
+    r.slot_vp = S.take(12 * V * V * 8);
+  }
+  r.slot_part = S.take((size_t)d.ncube * EPI_PSTRIDE * 8);
+  r.slot_bytes = S.off;
+  r.slots = L.take(r.slot_bytes * B);
+  // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6 per triple
+  r.split_bytes = backward ? (size_t)6 * B * MAX_KSPLIT * V * V * d.elt : 0;
+  r.split = L.take(r.split_bytes);
+  r.desc = L.off;
+  return r;
+}
+
+// ----------------------------------------------------------------------------
+// Launch plan
+enum LaunchKind { L_GEMM, L_REDUCE, L_EPI, L_EPIRED, L_VPART };
+
+struct Launch {
+  LaunchKind kind;
+  int cfg = 0;
+  size_t off_a = 0, off_b = 0, off_c = 0;   // descriptor offsets in the blob
+  int n_a = 0, n_b = 0, n_c = 0;
+  long long work = 0;                        // GEMM blocks / reduce elements
+  int aux = -1;                              // index of host-side argument struct
+};
+
+struct Blob {
+  std::vector<char> data;
+  template <class T>
+  size_t put(const std::vector<T>& v) {
+    size_t off = (size_t)round_up((long long)data.size(), 256);
+    data.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
+    if (!v.empty()) std::memcpy(data.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+struct Triple {
+  int t[3];
+  double fo;
+};
+
+std::vector<Triple> enumerate_triples(int o) {
+  std::vector<Triple> out;
+  for (int i = 0; i < o; i++)
+    for (int j = 0; j <= i; j++)
+      for (int k = 0; k <= j; k++) {
+        Triple tr;
+        tr.t[0] = i; tr.t[1] = j; tr.t[2] = k;
+        tr.fo = (i == k) ? 1.0 / 6.0 : ((i == j || j == k) ? 0.5 : 1.0);
+        out.push_back(tr);
+      }
+  return out;
+}
+
+// Grouped GEMM under construction: problems keyed by their output slab.
+struct GemmBuild {
+  int BK;
+  std::map<long long, int> index;
+  std::vector<GemmProb> probs;
+  std::vector<std::vector<GemmSeg>> segs;
+
+  explicit GemmBuild(int bk) : BK(bk) {}
+  int problem(long long key, const GemmProb& tmpl) {
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    int p = (int)probs.size();
+    index[key] = p;
+    probs.push_back(tmpl);
+    segs.emplace_back();
+    return p;
+  }
+  void add(int p, const GemmSeg& s) { segs[p].push_back(s); }
+  bool empty() const { return probs.empty(); }
+};
+
+struct Ctx {
+  Dims d;
+  Regions r;
+  char* ws = nullptr;   // device workspace base (may be null while sizing)
+  int B = 1;
+  bool backward = false;
+  double et_bar = 1.0;
+  // inputs (device, reference layouts)
+  const double *mo_energy = nullptr, *t1 = nullptr, *t2 = nullptr, *fock = nullptr, *ovoo = nullptr,
+               *ovov = nullptr, *ovvv = nullptr;
+  int packed = 1;
+  Blob blob;
+  std::vector<Launch> launches;
+  std::vector<EpiArgs> epi_args;
+  std::vector<EpiReduceArgs> epired_args;
+
+  template <class T = char>
+  T* at(size_t off) const { return reinterpret_cast<T*>(ws + off); }
+  char* slot(int s, size_t field) const { return ws + r.slots + (size_t)s * r.slot_bytes + field; }
+  long long str(int pos) const {
+    return pos == 0 ? (long long)d.V * d.V : (pos == 1 ? (long long)d.V : 1LL);
+  }
+  const char* el(size_t region, long long elems) const { return ws + region + elems * (long long)d.elt; }
+  const double* dl(size_t region, long long elems) const {
+    return reinterpret_cast<const double*>(ws + region) + elems;
+  }
+
+  // Flatten a GemmBuild into a launch.
+  void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split) {
+    if (gb.empty()) return;
+    const int BM = t3_gemm_tile_m(cfg), BN = t3_gemm_tile_n(cfg);
+    long long tiles = 0;
+    int min_chunks = 1 << 30;
+    for (size_t p = 0; p < gb.probs.size(); p++) {
+      GemmProb& P = gb.probs[p];
+      P.tiles_m = (P.M + BM - 1) / BM;
+      P.tiles_n = (P.N + BN - 1) / BN;
+      tiles += (long long)P.nb * P.tiles_m * P.tiles_n;
+      int c = 0;
+      for (auto& s : gb.segs[p]) {
+        s.chunk0 = c;
+        c += (int)((long long)s.K1 * s.K2 / gb.BK);
+      }
+      P.nchunks = c;
+      min_chunks = std::min(min_chunks, c);
+    }
+    int ksplit = 1;
+    if (allow_split) {
+      ksplit = (int)std::max<long long>(1, (GEMM_TARGET_BLOCKS + tiles - 1) / tiles);
+      ksplit = std::min(ksplit, std::min(MAX_KSPLIT, std::max(1, min_chunks)));
+    }
+    std::vector<GemmSeg> flat;
+    std::vector<ReduceProb> red;
+    long long nblocks = 0, relem = 0;
+    size_t scratch = 0;
+    for (size_t p = 0; p < gb.probs.size(); p++) {
+      GemmProb& P = gb.probs[p];
+      P.seg0 = (int)flat.size();
+      P.nseg = (int)gb.segs[p].size();
+      for (auto& s : gb.segs[p]) flat.push_back(s);
+      P.c_is_f64 = c_is_f64;
+      P.ksplit = allow_split ? ksplit : 1;
+      P.block0 = nblocks;
+      nblocks += (long long)P.nb * P.tiles_m * P.tiles_n * P.ksplit;
+      if (allow_split) {
+        const size_t bytes = (size_t)P.ksplit * P.nb * P.M * P.N * d.elt;
+        P.mode = GEMM_PARTIAL;
+        P.P = ws ? ws + r.split + scratch : nullptr;
+        ReduceProb R{};
+        R.P = P.P;
+        R.ksplit = P.ksplit;
+        R.nb = P.nb;
+        R.M = P.M;
+        R.N = P.N;
+        R.C = P.C;
+        R.c_sb = P.c_sb;
+        R.c_sm = P.c_sm;
+        R.c_sn = P.c_sn;
+        R.accumulate = mode == GEMM_ACCUM;
+        R.c_is_f64 = c_is_f64;
+        R.alpha = P.alpha;
+        R.elem0 = relem;
+        relem += (long long)P.nb * P.M * P.N;
+        red.push_back(R);
+        scratch = (size_t)round_up((long long)(scratch + bytes), 256);
+      } else {
+        P.mode = mode;
+        P.P = nullptr;
+      }
+    }
+    Launch L;
+    L.kind = L_GEMM;
+    L.cfg = cfg;
+    L.off_a = blob.put(gb.probs);
+    L.n_a = (int)gb.probs.size();
+    L.off_b = blob.put(flat);
+    L.n_b = (int)flat.size();
+    L.work = nblocks;
+    launches.push_back(L);
+    if (allow_split) {
+      Launch R;
+      R.kind = L_REDUCE;
+      R.off_a = blob.put(red);
+      R.n_a = (int)red.size();
+      R.work = relem;
+      launches.push_back(R);
+    }
+  }
+
+  // ---- W-build segments (output layout: batch position ib, row position im, col position 2) ----
+  GemmSeg particle_seg(const Triple& tr, int rho, int ib, int im) const {
+    const int* p = PERM[rho];
+    const long long V = d.V, Op = d.Op;
+    GemmSeg s{};
+    const char* L = el(r.ovvvT, (long long)tr.t[p[0]] * V * V * V);
+    const char* R = el(r.t2T, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
+    if (p[2] == 2) {
+      s.A = L; s.a_sb = (p[0] == ib) ? V * V : V; s.a_sm = (p[0] == im) ? V * V : V; s.a_sk2 = 1;
+      s.B = R; s.b_sb = 0; s.b_sn = V; s.b_sk2 = 1;
+    } else {   // p[2] == im
+      s.A = R; s.a_sb = 0; s.a_sm = V; s.a_sk2 = 1;
+      s.B = L; s.b_sb = (p[0] == ib) ? V * V : V; s.b_sn = (p[0] == 2) ? V * V : V; s.b_sk2 = 1;
+    }
+    s.K1 = 1; s.K2 = d.V; s.scale = 1.0f;
+    return s;
+  }
+  GemmSeg hole_seg(const Triple& tr, int rho, int ib, int im) const {
+    const int* p = PERM[rho];
+    const long long V = d.V, Op = d.Op;
+    GemmSeg s{};
+    const char* Lp = el(r.ovooT, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
+    const char* T2 = el(r.t2T, (long long)tr.t[p[2]] * Op * V * V);   // [m][s_p2][s_p1]
+    const char* T2H = el(r.t2H, (long long)tr.t[p[2]] * Op * V * V);  // [m][s_p1][s_p2]
+    if (p[0] == 2) {
+      s.B = Lp; s.b_sb = 0; s.b_sn = Op; s.b_sk2 = 1;
+      s.A = (p[1] == im) ? T2 : T2H;
+      s.a_sm = 1; s.a_sb = V; s.a_sk2 = V * V;
+    } else {   // p[0] == im
+      s.A = Lp; s.a_sb = 0; s.a_sm = Op; s.a_sk2 = 1;
+      s.B = (p[1] == 2) ? T2 : T2H;
+      s.b_sn = 1; s.b_sb = V; s.b_sk2 = V * V;
+    }
+    s.K1 = 1; s.K2 = d.Op; s.scale = -1.0f;
+    return s;
+  }
+
+  void emit_wbuild(const std::vector<Triple>& bt) {
+    const long long V = d.V;
+    static const int P1[4] = {0, 2, 3, 5}, H1[4] = {0, 1, 4, 5};   // batch b (pos 1), rows a
+    static const int P2[2] = {1, 4}, H2[2] = {2, 3};               // batch a (pos 0), rows b
+    for (int pass = 0; pass < 2; pass++) {
+      GemmBuild gb(d.BK);
+      for (size_t s = 0; s < bt.size(); s++) {
+        GemmProb P{};
+        P.nb = d.V; P.M = d.V; P.N = d.V;
+        P.C = slot((int)s, r.slot_w0);
+        P.c_sn = 1; P.alpha = 1.0;
+        if (pass == 0) { P.c_sb = V; P.c_sm = V * V; }
+        else { P.c_sb = V * V; P.c_sm = V; }
+        const int p = gb.problem((long long)s, P);
+        const int ib = pass == 0 ? 1 : 0, im = pass == 0 ? 0 : 1;
+        const int np = pass == 0 ? 4 : 2;
+        for (int q = 0; q < np; q++) gb.add(p, particle_seg(bt[s], pass == 0 ? P1[q] : P2[q], ib, im));
+        for (int q = 0; q < np; q++) gb.add(p, hole_seg(bt[s], pass == 0 ? H1[q] : H2[q], ib, im));
+      }
+      emit_gemm(gb, 0, pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false);
+    }
+  }
+
+  void emit_epilogue(const std::vector<Triple>& bt, bool bw) {
+    std::vector<EpiTriple> et;
+    for (size_t s = 0; s < bt.size(); s++) {
+      EpiTriple e{};
+      e.i = bt[s].t[0]; e.j = bt[s].t[1]; e.k = bt[s].t[2];
+      e.fo = bt[s].fo;
+      e.w0 = slot((int)s, r.slot_w0);
+      e.G = bw ? slot((int)s, r.slot_G) : nullptr;
+      e.H = bw ? slot((int)s, r.slot_H) : nullptr;
+      e.part = reinterpret_cast<double*>(slot((int)s, r.slot_part));
+      et.push_back(e);
+    }
+    EpiArgs a{};
+    a.nocc = d.o; a.nvir = d.v; a.V = d.V; a.ntile = d.ntile; a.ncube = d.ncube;
+    a.backward = bw; a.et_bar = et_bar;
+    a.eps_occ = dl(r.eps_occ, 0); a.eps_vir = dl(r.eps_vir, 0);
+    a.ovov = ovov; a.t2 = t2; a.t1 = t1; a.fock = fock;
+    epi_args.push_back(a);
+    Launch L;
+    L.kind = L_EPI;
+    L.off_a = blob.put(et);
+    L.n_a = (int)et.size();
+    L.aux = (int)epi_args.size() - 1;
+    launches.push_back(L);
+    EpiReduceArgs ra{};
+    ra.ntriple = (int)et.size(); ra.ncube = d.ncube; ra.ntile = d.ntile; ra.nocc = d.o; ra.V = d.V;
+    ra.triples = nullptr;   // patched at execution: descriptor offset L.off_a
+    ra.energy = reinterpret_cast<double*>(ws ? ws + r.energy : nullptr);
+    ra.eps_bar = bw ? reinterpret_cast<double*>(ws + r.eps_bar) : nullptr;
+    ra.backward = bw;
+    epired_args.push_back(ra);
+    Launch R;
+    R.kind = L_EPIRED;
+    R.off_a = L.off_a;
+    R.aux = (int)epired_args.size() - 1;
+    launches.push_back(R);
+  }
+
+  // ---- backward contractions over G ----
+  void emit_b1(const std::vector<Triple>& bt) {
+    const long long V = d.V, Op = d.Op;
+    for (int orient = 0; orient < 2; orient++) {
+      GemmBuild gb(d.BK);
+      for (size_t s = 0; s < bt.size(); s++)
+        for (int rho = 0; rho < 6; rho++) {
+          const int* p = PERM[rho];
+          if ((p[0] == 2) != (orient == 1)) continue;
+          const Triple& tr = bt[s];
+          GemmProb P{};
+          P.nb = d.V; P.M = d.V; P.N = d.V;
+          P.C = ws ? ws + r.ovvv_acc + (size_t)tr.t[p[0]] * V * V * V * 8 : nullptr;
+          P.c_sn = 1; P.alpha = 1.0;
+          if (orient == 0) { P.c_sb = V * V; P.c_sm = V; } else { P.c_sb = V; P.c_sm = V * V; }
+          const int q = gb.problem(tr.t[p[0]], P);
+          GemmSeg sg{};
+          sg.A = slot((int)s, r.slot_G);
+          if (orient == 0) { sg.a_sb = str(p[0]); sg.a_sm = str(p[1]); }
+          else { sg.a_sb = str(p[1]); sg.a_sm = str(p[0]); }
+          sg.a_sk2 = str(p[2]);
+          sg.B = el(r.t2T, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
+          sg.b_sb = 0; sg.b_sk2 = V; sg.b_sn = 1;
+          sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
+          gb.add(q, sg);
+        }
+      emit_gemm(gb, 0, GEMM_ACCUM, 1, false);
+    }
+  }
+
+  void emit_b2(const std::vector<Triple>& bt) {
+    const long long V = d.V, Op = d.Op;
+    GemmBuild gb(d.BK);
+    for (size_t s = 0; s < bt.size(); s++)
+      for (int rho = 0; rho < 6; rho++) {
+        const int* p = PERM[rho];
+        const Triple& tr = bt[s];
+        GemmProb P{};
+        P.nb = 1; P.M = d.V; P.N = d.V;
+        P.C = ws ? ws + r.t2acc + ((size_t)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V * 8 : nullptr;
+        P.c_sm = V; P.c_sn = 1; P.alpha = 1.0;
+        const int q = gb.problem((long long)tr.t[p[2]] * 4096 + tr.t[p[1]], P);
+        GemmSeg sg{};
+        sg.A = slot((int)s, r.slot_G);
+        sg.a_sm = str(p[2]);
+        sg.B = el(r.ovvvT, (long long)tr.t[p[0]] * V * V * V);
+        sg.b_sn = 1;
+        if (p[0] != 2) {
+          sg.a_sk1 = str(p[0]); sg.a_sk2 = str(p[1]); sg.b_sk1 = V * V; sg.b_sk2 = V;
+        } else {
+          sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[0]); sg.b_sk1 = V; sg.b_sk2 = V * V;
+        }
+        sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
+        gb.add(q, sg);
+      }
+    emit_gemm(gb, 0, GEMM_ACCUM, 1, true);
+  }
+
+  void emit_b3(const std::vector<Triple>& bt) {
+    const long long V = d.V, Op = d.Op;
+    GemmBuild gb(d.BK);
+    for (size_t s = 0; s < bt.size(); s++)
+      for (int rho = 0; rho < 6; rho++) {
+        const int* p = PERM[rho];
+        const Triple& tr = bt[s];
+        GemmProb P{};
+        P.nb = 1; P.M = d.V; P.N = d.Op;
+        P.C = ws ? ws + r.ovoo_acc + ((size_t)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op * 8 : nullptr;
+        P.c_sm = Op; P.c_sn = 1; P.alpha = 1.0;
+        const int q = gb.problem((long long)tr.t[p[0]] * 4096 + tr.t[p[1]], P);
+        GemmSeg sg{};
+        sg.A = slot((int)s, r.slot_G);
+        sg.a_sm = str(p[0]);
+        sg.b_sn = V * V;
+        if (p[2] == 2) {   // k = (s_p1, s_p2), B from t2H[t_p2][m][s_p1][s_p2]
+          sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[2]);
+          sg.B = el(r.t2H, (long long)tr.t[p[2]] * Op * V * V);
+        } else {           // k = (s_p2, s_p1), B from t2T[t_p2][m][s_p2][s_p1]
+          sg.a_sk1 = str(p[2]); sg.a_sk2 = str(p[1]);
+          sg.B = el(r.t2T, (long long)tr.t[p[2]] * Op * V * V);
+        }
+        sg.b_sk1 = V; sg.b_sk2 = 1;
+        sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
+        gb.add(q, sg);
+      }
+    emit_gemm(gb, 0, GEMM_ACCUM, 1, true);
+  }
+
+  void emit_b4(const std::vector<Triple>& bt) {
+    const long long V = d.V, Op = d.Op;
+    GemmBuild gb(d.BK);
+    for (size_t s = 0; s < bt.size(); s++)
+      for (int rho = 0; rho < 6; rho++) {
+        const int* p = PERM[rho];
+        const Triple& tr = bt[s];
+        const bool t2layout = (p[1] == 2);
+        GemmProb P{};
+        P.nb = d.V; P.M = d.Op; P.N = d.V;
+        const size_t region = t2layout ? r.t2acc : r.t2hacc;
+        P.C = ws ? ws + region + (size_t)tr.t[p[2]] * Op * V * V * 8 : nullptr;
+        P.c_sm = V * V; P.c_sb = V; P.c_sn = 1; P.alpha = 1.0;
+        const int q = gb.problem((t2layout ? 0 : 1) * 4096LL + tr.t[p[2]], P);
+        GemmSeg sg{};
+        sg.A = el(r.ovooT, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
+        sg.a_sb = 0; sg.a_sm = 1; sg.a_sk2 = Op;
+        sg.B = slot((int)s, r.slot_G);
+        sg.b_sk2 = str(p[0]);
+        if (t2layout) { sg.b_sb = str(p[2]); sg.b_sn = str(p[1]); }
+        else { sg.b_sb = str(p[1]); sg.b_sn = str(p[2]); }
+        sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
+        gb.add(q, sg);
+      }
+    emit_gemm(gb, 0, GEMM_ACCUM, 1, false);
+  }
+
+  void emit_vpart(const std::vector<Triple>& bt) {
+    std::vector<VpTriple> trs;
+    std::map<long long, std::vector<VpContrib>> by_slab;
+    std::vector<long long> order;
+    for (size_t s = 0; s < bt.size(); s++) {
+      VpTriple t{};
+      t.i = bt[s].t[0]; t.j = bt[s].t[1]; t.k = bt[s].t[2];
+      t.H = slot((int)s, r.slot_H);
+      t.out = reinterpret_cast<double*>(slot((int)s, r.slot_vp));
+      trs.push_back(t);
+      for (int rho = 0; rho < 6; rho++) {
+        const int* p = PERM[rho];
+        const long long key = (long long)bt[s].t[p[0]] * 4096 + bt[s].t[p[1]];
+        if (!by_slab.count(key)) order.push_back(key);
+        by_slab[key].push_back(VpContrib{(int)s, p[2], p[0] > p[1] ? 1 : 0});
+      }
+    }
+    std::vector<VpSlab> slabs;
+    std::vector<VpContrib> contribs;
+    for (long long key : order) {
+      VpSlab sl{};
+      sl.p = (int)(key / 4096); sl.q = (int)(key % 4096);
+      sl.c0 = (int)contribs.size();
+      for (auto& c : by_slab[key]) contribs.push_back(c);
+      sl.nc = (int)contribs.size() - sl.c0;
+      slabs.push_back(sl);
+    }
+    Launch L;
+    L.kind = L_VPART;
+    L.off_a = blob.put(trs);
+    L.n_a = (int)trs.size();
+    L.off_b = blob.put(slabs);
+    L.n_b = (int)slabs.size();
+    L.off_c = blob.put(contribs);
+    L.n_c = (int)contribs.size();
+    launches.push_back(L);
+  }
+
+  void build_plan() {
+    const std::vector<Triple> all = enumerate_triples(d.o);
+    for (size_t b0 = 0; b0 < all.size(); b0 += B) {
+      std::vector<Triple> bt(all.begin() + b0, all.begin() + std::min(all.size(), b0 + B));
+      emit_wbuild(bt);
+      emit_epilogue(bt, backward);
+      if (backward) {
+        emit_b1(bt);
+        emit_b2(bt);
+        emit_b3(bt);
+        emit_b4(bt);
+        emit_vpart(bt);
+      }
+    }
+  }
+
+  hipError_t execute(hipStream_t st) {
+    char* db = ws + r.desc;
+    hipError_t e = hipSuccess;
+    for (const Launch& L : launches) {
+      switch (L.kind) {
+        case L_GEMM:
+          e = t3_launch_gemm(d.prec, L.cfg, reinterpret_cast<const GemmProb*>(db + L.off_a), L.n_a,
+                             reinterpret_cast<const GemmSeg*>(db + L.off_b), L.work, st);
+          break;
+        case L_REDUCE:
+          e = t3_launch_reduce(d.prec, reinterpret_cast<const ReduceProb*>(db + L.off_a), L.n_a, L.work, st);
+          break;
+        case L_EPI:
+          e = t3_launch_epilogue(d.tp_is_f64, &epi_args[L.aux], reinterpret_cast<const EpiTriple*>(db + L.off_a),
+                                 L.n_a, st);
+          break;
+        case L_EPIRED: {
+          EpiReduceArgs a = epired_args[L.aux];
+          a.triples = reinterpret_cast<const EpiTriple*>(db + L.off_a);
+          e = t3_launch_epi_reduce(&a, st);
+          break;
+        }
+        case L_VPART:
+          e = t3_launch_vpart(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a,
+                              reinterpret_cast<const VpSlab*>(db + L.off_b), L.n_b,
+                              reinterpret_cast<const VpContrib*>(db + L.off_c), d.o, d.v, d.V, t1, fock, ovov,
+                              t2, at<double>(r.ovov_acc), at<double>(r.t2vacc), at<double>(r.t1_acc),
+                              at<double>(r.fvo_acc), st);
+          break;
+      }
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+};
+
+int check_args(int nocc, int nvir, int precision) {
+  if (nocc < 1 || nvir < 1) return fail(PYSCFAD_AMD_ERR_ARGS, "nocc and nvir must be >= 1");
+  if (nocc > 4095 || nvir > 4095) return fail(PYSCFAD_AMD_ERR_ARGS, "nocc and nvir must be < 4096");
+  if (precision < PYSCFAD_AMD_PREC_F64 || precision > PYSCFAD_AMD_PREC_BF16X3)
+    return fail(PYSCFAD_AMD_ERR_PRECISION, "precision must be 0 (F64), 1 (F32) or 2 (BF16X3)");
+  return PYSCFAD_AMD_OK;
+}
+
+// Size the descriptor blob by building the plan against a null workspace.
+size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B) {
+  Ctx c;
+  c.d = make_dims(nocc, nvir, precision);
+  c.B = choose_batch(c.d);
+  c.backward = backward;
+  c.r = plan_regions(c.d, c.B, backward);
+  c.build_plan();
+  c.r.desc_bytes = c.blob.data.size();
+  c.r.total = c.r.desc + (size_t)round_up((long long)c.r.desc_bytes, 256);
+  if (out_r) *out_r = c.r;
+  if (out_B) *out_B = c.B;
+  return c.r.total;
+}
+
+int run(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2, const double* fock,
+        const double* ovoo, const double* ovov, const double* ovvv, int packed, bool backward, double et_bar,
+        int precision, double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
+        double* ovoo_bar, double* ovov_bar, double* ovvv_bar, double* e_out, void* workspace,
+        size_t workspace_bytes, void* stream) {
+  g_last_error.clear();
+  int rc = check_args(nocc, nvir, precision);
+  if (rc) return rc;
+  if (!mo_energy || !t1 || !t2 || !fock || !ovoo || !ovov || !ovvv)
+    return fail(PYSCFAD_AMD_ERR_ARGS, "null input pointer");
+  if (packed != 0 && packed != 1) return fail(PYSCFAD_AMD_ERR_ARGS, "ovvv_packed must be 0 or 1");
+  if (backward && (!t1_bar || !t2_bar || !fock_bar || !mo_energy_bar || !ovoo_bar || !ovov_bar || !ovvv_bar))
+    return fail(PYSCFAD_AMD_ERR_ARGS, "null output pointer");
+  if (!backward && !e_out) return fail(PYSCFAD_AMD_ERR_ARGS, "null e_out");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+
+  Ctx c;
+  c.d = make_dims(nocc, nvir, precision);
+  c.backward = backward;
+  c.et_bar = et_bar;
+  const size_t need = plan_bytes(nocc, nvir, precision, backward, &c.r, &c.B);
+  bool own = false;
+  if (workspace) {
+    if (workspace_bytes < need)
+      return fail(PYSCFAD_AMD_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+    c.ws = reinterpret_cast<char*>(workspace);
+  } else {
+    if (hipMalloc(&c.ws, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PYSCFAD_AMD_ERR_OOM, "hipMalloc of " + std::to_string(need) + " bytes failed");
+    }
+    own = true;
+  }
+  c.mo_energy = mo_energy; c.t1 = t1; c.t2 = t2; c.fock = fock; c.ovoo = ovoo; c.ovov = ovov; c.ovvv = ovvv;
+  c.packed = packed;
+  c.build_plan();
+  auto cleanup = [&](int code) {
+    if (own) {
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(c.ws);
+    }
+    return code;
+  };
+  auto hip_fail = [&](hipError_t e, const char* where) {
+    return cleanup(fail(PYSCFAD_AMD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e)));
+  };
+  hipError_t e;
+  e = hipMemcpyAsync(c.ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+  // the upload reads host memory that must stay alive until it completes
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+  e = hipMemsetAsync(c.ws + c.r.energy, 0, 8, st);
+  if (e != hipSuccess) return hip_fail(e, "memset");
+  if (backward) {
+    const size_t acc0 = c.r.ovvv_acc, acc1 = c.r.eps_bar + (c.d.o + c.d.V) * 8;
+    e = hipMemsetAsync(c.ws + acc0, 0, acc1 - acc0, st);
+    if (e != hipSuccess) return hip_fail(e, "memset");
+    // G and H cubes are fully rewritten per triple; the split scratch is fully
+    // rewritten before each reduction
+  }
+  e = t3_prep_inputs(c.d.tp_is_f64, c.ws + c.r.ovvvT, c.ws + c.r.t2T, c.ws + c.r.t2H, c.ws + c.r.ovooT,
+                     c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), ovvv, packed, t2, ovoo, mo_energy,
+                     c.d.o, c.d.v, c.d.Op, c.d.V, st);
+  if (e != hipSuccess) return hip_fail(e, "prep");
+  e = c.execute(st);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  if (backward) {
+    e = t3_finalize(t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, packed,
+                    c.at<double>(c.r.ovvv_acc), c.at<double>(c.r.t2acc), c.at<double>(c.r.t2hacc),
+                    c.at<double>(c.r.t2vacc), c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc),
+                    c.at<double>(c.r.t1_acc), c.at<double>(c.r.fvo_acc), c.at<double>(c.r.eps_bar), c.d.o, c.d.v,
+                    c.d.Op, c.d.V, st);
+    if (e != hipSuccess) return hip_fail(e, "finalize");
+  }
+  if (e_out) {
+    e = hipMemcpyAsync(e_out, c.ws + c.r.energy, 8, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "energy copy");
+  }
+  return cleanup(PYSCFAD_AMD_OK);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pyscfad_amd_ccsd_t_energy(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
+                              const double* fock, const double* ovoo, const double* ovov, const double* ovvv,
+                              int ovvv_packed, int precision, double* e_out, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, false, 1.0, precision, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out, workspace, workspace_bytes, stream);
+}
+
+int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
+                                  const double* fock, const double* ovoo, const double* ovov, const double* ovvv,
+                                  int ovvv_packed, double et_bar, int precision, double* t1_bar, double* t2_bar,
+                                  double* fock_bar, double* mo_energy_bar, double* ovoo_bar, double* ovov_bar,
+                                  double* ovvv_bar, double* e_out, void* workspace, size_t workspace_bytes,
+                                  void* stream) {
+  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, precision, t1_bar,
+             t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out, workspace, workspace_bytes,
+             stream);
+}
+
+size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp) {
+  if (check_args(nocc, nvir, precision)) return 0;
+  return plan_bytes(nocc, nvir, precision, with_vjp != 0, nullptr, nullptr);
+}
+
+const char* pyscfad_amd_last_error(void) { return g_last_error.c_str(); }
+
+int pyscfad_amd_abi_version(void) { return 1; }
+
+}  // extern "C"

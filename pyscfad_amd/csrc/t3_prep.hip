@@ -1,0 +1,368 @@
+// Layout preparation, V-term adjoints and cotangent finalisation.
+//
+// Device layouts used by the contractions (V = nvir padded to a multiple of
+// 32, Op = nocc padded to a multiple of the K-chunk, all padding zero):
+//   ovvvT [o ][V ][V][V]  = ovvv[p,x,f,y]           (OVVV[p][x][y][f])
+//   t2T   [Op][Op][V][V]  = t2[r,q,z,f]
+//   t2H   [Op][Op][V][V]  = t2[r,m,z,y] at [r][m][y][z]
+//   ovooT [o ][o ][V][Op] = ovoo[p,x,m,q] at [p][q][x][m]
+// The reference builds the analogous host arrays at pyscfad/cc/ccsd_t.py:88-106.
+#include "t3_common.h"
+
+namespace t3 {
+
+__device__ __forceinline__ long long tri_index(long long f, long long y) {
+  return f >= y ? f * (f + 1) / 2 + y : y * (y + 1) / 2 + f;
+}
+
+template <class TP>
+__global__ void prep_ovvv(TP* __restrict__ out, const double* __restrict__ ovvv, int o, int v, int V,
+                          int packed) {
+  const long long total = (long long)o * V * V * V;
+  const long long npair = (long long)v * (v + 1) / 2;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(e % V);
+    const int y = (int)((e / V) % V);
+    const int x = (int)((e / ((long long)V * V)) % V);
+    const int p = (int)(e / ((long long)V * V * V));
+    double val = 0.0;
+    if (x < v && y < v && f < v) {
+      const long long px = (long long)p * v + x;
+      val = packed ? ovvv[px * npair + tri_index(f, y)] : ovvv[(px * v + f) * v + y];
+    }
+    out[e] = (TP)val;
+  }
+}
+
+// t2T[r][q][z][f] = t2[r,q,z,f];  t2H[r][m][y][z] = t2[r,m,z,y]
+template <class TP>
+__global__ void prep_t2(TP* __restrict__ t2T, TP* __restrict__ t2H, const double* __restrict__ t2,
+                        int o, int v, int Op, int V) {
+  const long long total = (long long)Op * Op * V * V;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int c1 = (int)(e % V);
+    const int c0 = (int)((e / V) % V);
+    const int q = (int)((e / ((long long)V * V)) % Op);
+    const int r = (int)(e / ((long long)V * V * Op));
+    double a = 0.0, b = 0.0;
+    if (r < o && q < o && c0 < v && c1 < v) {
+      const long long base = ((long long)r * o + q) * v;
+      a = t2[(base + c0) * v + c1];
+      b = t2[(base + c1) * v + c0];
+    }
+    t2T[e] = (TP)a;
+    t2H[e] = (TP)b;
+  }
+}
+
+template <class TP>
+__global__ void prep_ovoo(TP* __restrict__ out, const double* __restrict__ ovoo, int o, int v, int V,
+                          int Op) {
+  const long long total = (long long)o * o * V * Op;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int m = (int)(e % Op);
+    const int x = (int)((e / Op) % V);
+    const int q = (int)((e / ((long long)Op * V)) % o);
+    const int p = (int)(e / ((long long)Op * V * o));
+    double val = 0.0;
+    if (m < o && x < v) val = ovoo[(((long long)p * v + x) * o + m) * o + q];
+    out[e] = (TP)val;
+  }
+}
+
+__global__ void prep_eps(double* __restrict__ eps_occ, double* __restrict__ eps_vir,
+                         const double* __restrict__ mo_energy, int o, int v, int V) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < o) eps_occ[e] = mo_energy[e];
+  // padded virtuals get a large orbital energy: every padded term has w0 = v0 = 0,
+  // this only keeps its denominator finite and away from zero
+  if (e < V) eps_vir[e] = e < v ? mo_energy[o + e] : 1.0e4;
+}
+
+// ---------------- V-term adjoints (ccsd_t_vjp.c:171-204, 228-245) -----------------
+// Per triple (i,j,k) and contracted position d of the H cube:
+//   S_d[e1][e2] = sum_{s_d} H[s] t1[t_d, s_d]            (task d)
+//   F_d[e1][e2] = sum_{s_d} H[s] fock[o + s_d, t_d]      (task d)
+//   P_d[*][*]   : partial sums of sum_{e1,e2} H[s] CM_d[s_e1][s_e2]   (tasks 3..5)
+// with e1 < e2 the two other positions, CM_d / CN_d the two-ordering sums of
+//   M_{pq}[x][y] = ovov[p,x,q,y]  and  N_{pq}[x][y] = t2[q,p,y,x].
+// Output per triple: 12 V x V FP64 arrays: [S0 F0 S1 F1 S2 F2 PM0 PN0 PM1 PN1 PM2 PN2].
+
+template <class TP>
+__global__ void __launch_bounds__(256)
+vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ t1,
+               const double* __restrict__ fock, const double* __restrict__ ovov,
+               const double* __restrict__ t2) {
+  const VpTriple tr = trs[blockIdx.z];
+  const int task = blockIdx.y;          // 0..5
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)V * V) return;
+  const int u = (int)(e / V), w = (int)(e % V);   // output (row, col)
+  const TP* H = reinterpret_cast<const TP*>(tr.H);
+  const int occ[3] = {tr.i, tr.j, tr.k};
+  const long long VV = (long long)V * V;
+  const int n = o + v;
+  if (task < 3) {
+    const int d = task;
+    // H index with s_d = x and the other two = (u, w) in increasing position order
+    double s = 0.0, f = 0.0;
+    const int td = occ[d];
+    for (int x = 0; x < v; x++) {
+      long long h;
+      if (d == 0) h = (long long)x * VV + (long long)u * V + w;
+      else if (d == 1) h = (long long)u * VV + (long long)x * V + w;
+      else h = (long long)u * VV + (long long)w * V + x;
+      const double hv = (double)H[h];
+      s += hv * t1[(long long)td * v + x];
+      f += hv * fock[(long long)(o + x) * n + td];
+    }
+    tr.out[(2 * d) * VV + e] = s;
+    tr.out[(2 * d + 1) * VV + e] = f;
+    return;
+  }
+  // partial sums of the two-axis contractions
+  const int d = task - 3;
+  int e1, e2;
+  if (d == 0) { e1 = 1; e2 = 2; } else if (d == 1) { e1 = 0; e2 = 2; } else { e1 = 0; e2 = 1; }
+  const int p1 = occ[e1], p2 = occ[e2];
+  auto cm = [&](int x, int y, double& mval, double& nval) {
+    // CM_d[x][y] = M_{p1 p2}[x][y] + M_{p2 p1}[y][x];  CN_d likewise with N
+    mval = 0.0; nval = 0.0;
+    if (x >= v || y >= v) return;
+    mval = ovov[(((long long)p1 * v + x) * o + p2) * v + y] + ovov[(((long long)p2 * v + y) * o + p1) * v + x];
+    nval = t2[(((long long)p2 * o + p1) * v + y) * v + x] + t2[(((long long)p1 * o + p2) * v + x) * v + y];
+  };
+  double sm = 0.0, sn = 0.0;
+  if (d == 2) {
+    // P2[a][c] = sum_b H[a,b,c] CM2[a][b]      (u = a, w = c)
+    for (int b = 0; b < v; b++) {
+      double mv, nv; cm(u, b, mv, nv);
+      const double hv = (double)H[(long long)u * VV + (long long)b * V + w];
+      sm += hv * mv; sn += hv * nv;
+    }
+  } else if (d == 1) {
+    // P1[a][b] = sum_c H[a,b,c] CM1[a][c]      (u = a, w = b)
+    for (int c = 0; c < v; c++) {
+      double mv, nv; cm(u, c, mv, nv);
+      const double hv = (double)H[(long long)u * VV + (long long)w * V + c];
+      sm += hv * mv; sn += hv * nv;
+    }
+  } else {
+    // P0[a][b] = sum_c H[a,b,c] CM0[b][c]      (u = a, w = b)
+    for (int c = 0; c < v; c++) {
+      double mv, nv; cm(w, c, mv, nv);
+      const double hv = (double)H[(long long)u * VV + (long long)w * V + c];
+      sm += hv * mv; sn += hv * nv;
+    }
+  }
+  tr.out[(6 + 2 * d) * VV + e] = sm;
+  tr.out[(7 + 2 * d) * VV + e] = sn;
+}
+
+// Scatter of the V-term adjoints of one batch into the FP64 accumulators.
+// Slab tasks (blockIdx.y < nslab): element (x,y) of ovov_acc[p][q] and
+// t2v_acc[p][q] += 1/2 of every (triple, d, transposed) contribution, in list
+// order.  Vector tasks (blockIdx.y == nslab): t1_acc[r][z], fvo_acc[r][z].
+
+__global__ void __launch_bounds__(256)
+vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __restrict__ contribs,
+              const VpTriple* __restrict__ trs, int ntriple, int o, int V, double* __restrict__ ovov_acc,
+              double* __restrict__ t2v_acc, double* __restrict__ t1_acc, double* __restrict__ fvo_acc) {
+  const long long VV = (long long)V * V;
+  if ((int)blockIdx.y < nslab) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= VV) return;
+    const int x = (int)(e / V), y = (int)(e % V);
+    const VpSlab sl = slabs[blockIdx.y];
+    double am = 0.0, af = 0.0;
+    for (int c = sl.c0; c < sl.c0 + sl.nc; c++) {
+      const VpContrib ct = contribs[c];
+      const double* out = trs[ct.slot].out;
+      const long long idx = ct.transposed ? (long long)y * V + x : e;
+      am += out[(2 * ct.d) * VV + idx];
+      af += out[(2 * ct.d + 1) * VV + idx];
+    }
+    const long long off = ((long long)sl.p * o + sl.q) * VV + e;
+    ovov_acc[off] += 0.5 * am;
+    t2v_acc[off] += 0.5 * af;
+    return;
+  }
+  // vectors: thread per (r, z)
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)o * V) return;
+  const int r = (int)(e / V), z = (int)(e % V);
+  double sm = 0.0, sn = 0.0;
+  for (int t = 0; t < ntriple; t++) {
+    const VpTriple& tr = trs[t];
+    const int occ[3] = {tr.i, tr.j, tr.k};
+    for (int d = 0; d < 3; d++) {
+      if (occ[d] != r) continue;
+      const double* PM = tr.out + (6 + 2 * d) * VV;
+      const double* PN = tr.out + (7 + 2 * d) * VV;
+      if (d == 2) {          // sum over a of P2[a][z]
+        for (int a = 0; a < V; a++) { sm += PM[(long long)a * V + z]; sn += PN[(long long)a * V + z]; }
+      } else if (d == 1) {   // sum over a of P1[a][z]
+        for (int a = 0; a < V; a++) { sm += PM[(long long)a * V + z]; sn += PN[(long long)a * V + z]; }
+      } else {               // sum over b of P0[z][b]
+        for (int b = 0; b < V; b++) { sm += PM[(long long)z * V + b]; sn += PN[(long long)z * V + b]; }
+      }
+    }
+  }
+  t1_acc[e] += 0.5 * sm;
+  fvo_acc[e] += 0.5 * sn;
+}
+
+// ---------------- finalisation into the reference's cotangent layouts --------------
+// (ccsd_t.py:179-192)
+__global__ void fin_ovvv(double* __restrict__ out, const double* __restrict__ acc, int o, int v, int V,
+                         int packed) {
+  const long long npair = (long long)v * (v + 1) / 2;
+  const long long total = packed ? (long long)o * v * npair : (long long)o * v * v * v;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    if (packed) {
+      const long long pr = e % npair;
+      const long long px = e / npair;
+      const int x = (int)(px % v), p = (int)(px / v);
+      long long f = (long long)((sqrt(8.0 * (double)pr + 1.0) - 1.0) / 2.0);
+      while (f * (f + 1) / 2 > pr) f--;
+      while ((f + 1) * (f + 2) / 2 <= pr) f++;
+      const long long y = pr - f * (f + 1) / 2;
+      const long long base = ((long long)p * V + x) * V;
+      const double a = acc[(base + y) * V + f];
+      out[e] = (f == y) ? a : a + acc[(base + f) * V + y];
+    } else {
+      const int y = (int)(e % v), f = (int)((e / v) % v), x = (int)((e / ((long long)v * v)) % v);
+      const int p = (int)(e / ((long long)v * v * v));
+      out[e] = acc[((((long long)p * V + x) * V + y) * V) + f];
+    }
+  }
+}
+
+// t2_bar[i,j,a,b] = t2acc[i][j][a][b] + t2hacc[i][j][b][a] + t2vacc[j][i][b][a]
+__global__ void fin_t2(double* __restrict__ out, const double* __restrict__ t2acc,
+                       const double* __restrict__ t2hacc, const double* __restrict__ t2vacc, int o, int v,
+                       int Op, int V) {
+  const long long total = (long long)o * o * v * v;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(e % v), a = (int)((e / v) % v);
+    const int j = (int)((e / ((long long)v * v)) % o), i = (int)(e / ((long long)v * v * o));
+    const long long pij = ((long long)i * Op + j) * V * V;
+    const long long vji = ((long long)j * o + i) * V * V;
+    out[e] = t2acc[pij + (long long)a * V + b] + t2hacc[pij + (long long)b * V + a] +
+             t2vacc[vji + (long long)b * V + a];
+  }
+}
+
+// ovoo_bar[p,x,m,q] = acc[p][q][x][m];  ovov_bar[p,x,q,y] = acc[p][q][x][y]
+__global__ void fin_small(double* __restrict__ ovoo_bar, double* __restrict__ ovov_bar,
+                          double* __restrict__ t1_bar, double* __restrict__ fock_bar,
+                          double* __restrict__ mo_energy_bar, const double* __restrict__ ovoo_acc,
+                          const double* __restrict__ ovov_acc, const double* __restrict__ t1_acc,
+                          const double* __restrict__ fvo_acc, const double* __restrict__ eps_bar, int o,
+                          int v, int Op, int V) {
+  const int n = o + v;
+  const long long n1 = (long long)o * v * o * o, n2 = (long long)o * v * o * v, n3 = (long long)o * v,
+                  n4 = (long long)n * n, n5 = n;
+  const long long total = n1 + n2 + n3 + n4 + n5;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    long long r = e;
+    if (r < n1) {
+      const int q = (int)(r % o), m = (int)((r / o) % o), x = (int)((r / ((long long)o * o)) % v);
+      const int p = (int)(r / ((long long)o * o * v));
+      ovoo_bar[r] = ovoo_acc[(((long long)p * o + q) * V + x) * Op + m];
+      continue;
+    }
+    r -= n1;
+    if (r < n2) {
+      const int y = (int)(r % v), q = (int)((r / v) % o), x = (int)((r / ((long long)v * o)) % v);
+      const int p = (int)(r / ((long long)v * o * v));
+      ovov_bar[r] = ovov_acc[(((long long)p * o + q) * V + x) * V + y];
+      continue;
+    }
+    r -= n2;
+    if (r < n3) {
+      const int z = (int)(r % v), i = (int)(r / v);
+      t1_bar[r] = t1_acc[(long long)i * V + z];
+      continue;
+    }
+    r -= n3;
+    if (r < n4) {
+      const int col = (int)(r % n), row = (int)(r / n);
+      fock_bar[r] = (row >= o && col < o) ? fvo_acc[(long long)col * V + (row - o)] : 0.0;
+      continue;
+    }
+    r -= n4;
+    mo_energy_bar[r] = r < o ? eps_bar[r] : eps_bar[o + (r - o)];
+  }
+}
+
+}  // namespace t3
+
+using namespace t3;
+
+static inline dim3 grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 65536LL * 8) b = 65536LL * 8;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void* ovooT,
+                                     double* eps_occ, double* eps_vir, const double* ovvv, int packed,
+                                     const double* t2, const double* ovoo, const double* mo_energy, int o,
+                                     int v, int Op, int V, hipStream_t s) {
+  const long long n_ovvv = (long long)o * V * V * V, n_t2 = (long long)Op * Op * V * V,
+                  n_ovoo = (long long)o * o * V * Op;
+  if (tp_is_f64) {
+    hipLaunchKernelGGL(prep_ovvv<double>, grid_for(n_ovvv), dim3(256), 0, s, (double*)ovvvT, ovvv, o, v, V, packed);
+    hipLaunchKernelGGL(prep_t2<double>, grid_for(n_t2), dim3(256), 0, s, (double*)t2T, (double*)t2H, t2, o, v, Op, V);
+    hipLaunchKernelGGL(prep_ovoo<double>, grid_for(n_ovoo), dim3(256), 0, s, (double*)ovooT, ovoo, o, v, V, Op);
+  } else {
+    hipLaunchKernelGGL(prep_ovvv<float>, grid_for(n_ovvv), dim3(256), 0, s, (float*)ovvvT, ovvv, o, v, V, packed);
+    hipLaunchKernelGGL(prep_t2<float>, grid_for(n_t2), dim3(256), 0, s, (float*)t2T, (float*)t2H, t2, o, v, Op, V);
+    hipLaunchKernelGGL(prep_ovoo<float>, grid_for(n_ovoo), dim3(256), 0, s, (float*)ovooT, ovoo, o, v, V, Op);
+  }
+  hipLaunchKernelGGL(prep_eps, grid_for(V > o ? V : o), dim3(256), 0, s, eps_occ, eps_vir, mo_energy, o, v, V);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs,
+                                      int nslab, const VpContrib* contribs, int o, int v, int V,
+                                      const double* t1, const double* fock, const double* ovov,
+                                      const double* t2, double* ovov_acc, double* t2v_acc, double* t1_acc,
+                                      double* fvo_acc, hipStream_t s) {
+  if (ntriple <= 0) return hipSuccess;
+  const long long VV = (long long)V * V;
+  dim3 g1((unsigned)((VV + 255) / 256), 6, ntriple);
+  if (tp_is_f64) hipLaunchKernelGGL(vpart_contract<double>, g1, dim3(256), 0, s, trs, o, v, V, t1, fock, ovov, t2);
+  else hipLaunchKernelGGL(vpart_contract<float>, g1, dim3(256), 0, s, trs, o, v, V, t1, fock, ovov, t2);
+  const long long nx = VV > (long long)o * V ? VV : (long long)o * V;
+  dim3 g2((unsigned)((nx + 255) / 256), nslab + 1);
+  hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, V, ovov_acc,
+                     t2v_acc, t1_acc, fvo_acc);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
+                                  double* ovoo_bar, double* ovov_bar, double* ovvv_bar, int packed,
+                                  const double* ovvv_acc, const double* t2acc, const double* t2hacc,
+                                  const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
+                                  const double* t1_acc, const double* fvo_acc, const double* eps_bar, int o,
+                                  int v, int Op, int V, hipStream_t s) {
+  const long long npair = (long long)v * (v + 1) / 2;
+  const long long n_ovvv = packed ? (long long)o * v * npair : (long long)o * v * v * v;
+  hipLaunchKernelGGL(fin_ovvv, grid_for(n_ovvv), dim3(256), 0, s, ovvv_bar, ovvv_acc, o, v, V, packed);
+  hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2acc, t2hacc, t2vacc,
+                     o, v, Op, V);
+  const long long n = o + v;
+  const long long nsmall = (long long)o * v * o * o + (long long)o * v * o * v + (long long)o * v + n * n + n;
+  hipLaunchKernelGGL(fin_small, grid_for(nsmall), dim3(256), 0, s, ovoo_bar, ovov_bar, t1_bar, fock_bar,
+                     mo_energy_bar, ovoo_acc, ovov_acc, t1_acc, fvo_acc, eps_bar, o, v, Op, V);
+  return hipGetLastError();
+}
