@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""(T) energy+VJP throughput on one MI355X.
+
+One step = one forward call (E(T), the custom_vjp forward) plus one backward
+call (all cotangents; the backward recomputes W/V instead of storing them) of
+the CCSD(T) perturbative-triples correction, on the synthetic single instance
+nocc=32, nvir=256 (seed 2; generator in pyscfad_amd/synthetic.py), with the
+inputs resident in HBM before the timed region starts.
+
+metric   : TFLOP/s of energy+VJP credited as 3 * F_E, F_E = 2 o^3 v^3 (v + o)
+value    : that rate for the --precision path (default f32 MFMA)
+roofline : the grouped MFMA GEMM kernel (it executes 4 * F_E algorithmic flops
+           per step: forward W, backward W recompute, two adjoint contractions),
+           its device time measured with hipEvents around every launch during
+           the warm-up step, against the MI355X dense matrix peak of the
+           precision used
+cpu_baseline: the repository's FP64 C/OpenMP restatement of the reference
+           algorithm (oracle/), energy + VJP on a bounded slice of the
+           a>=b>=c virtual-triple jobs, on the host cores of the same machine
+The FP64 GPU path is run once as the accuracy reference of the lower
+precisions (|dE| and max relative cotangent error are reported).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+MI355X_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2516.6 / 3.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nocc", type=int, default=32)
+    ap.add_argument("--nvir", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 accuracy reference run")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline sample time")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(inst, nocc, nvir, target_s):
+    """Oracle energy + VJP on a slice of the job list, timed on the host cores."""
+    import numpy as np
+    from oracle import ccsd_t_oracle as orc
+    host = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in inst.items()}
+    flops_per_job = 12.0 * nocc ** 3 * (nvir + nocc)          # forward, one a>=b>=c job
+    njobs_all = orc.njobs(nvir)
+
+    def run(j0, j1):
+        t0 = time.perf_counter()
+        orc.energy(**host, job_range=(j0, j1))
+        orc.energy_vjp(**host, et_bar=1.0, job_range=(j0, j1))
+        return time.perf_counter() - t0
+
+    j0 = njobs_all // 2
+    probe = 16
+    t_probe = run(j0, j0 + probe)
+    n = int(max(probe, min(njobs_all - j0, probe * target_s / max(t_probe, 1e-3))))
+    t = run(j0, j0 + n)
+    rate = 3.0 * flops_per_job * n / t / 1e12
+    return {"value": rate, "unit": "TFLOP/s", "cores": orc.num_threads(), "kind": "port",
+            "sample": f"energy+VJP of a>=b>=c jobs [{j0}, {j0 + n}) of {njobs_all} "
+                      f"(nocc={nocc}, nvir={nvir}) in {t:.1f} s, FP64, credited 3*F_E per job"}
+
+
+def max_rel(a, b):
+    import torch
+    scale = torch.max(torch.abs(b)).item()
+    return (torch.max(torch.abs(a - b)).item() / scale) if scale > 0 else 0.0
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+                                   "(the multi-GPU sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    from pyscfad_amd import _lib
+    from pyscfad_amd.cc import ccsd_t
+    from pyscfad_amd.synthetic import make_instance, flops_energy
+
+    o, v = args.nocc, args.nvir
+    log(f"[bench] building synthetic instance nocc={o} nvir={v} seed={args.seed}")
+    inst = make_instance(o, v, args.seed, device="cuda")
+    torch.cuda.synchronize()
+    keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")
+    a = [inst[k] for k in keys]
+    fe = flops_energy(o, v)
+
+    def step(prec, timing=False):
+        _lib.set_timing(timing)
+        e = ccsd_t.energy(*a, precision=prec)
+        t_fwd = _lib.last_timing() if timing else None
+        bars = ccsd_t.energy_vjp(*a, et_bar=1.0, precision=prec)
+        t_bwd = _lib.last_timing() if timing else None
+        _lib.set_timing(False)
+        return e, bars, t_fwd, t_bwd
+
+    gemm_ms = None
+    fam = {}
+    for w in range(args.warmup):
+        log(f"[bench] warm-up step {w} ({args.precision}, kernel timing on)")
+        _, _, tf, tb = step(args.precision, timing=(w == 0))
+        if w == 0:
+            fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf}
+            gemm_ms = fam["gemm"][0]
+    torch.cuda.synchronize()
+    log(f"[bench] timing {args.steps} step(s)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e, bars, _, _ = step(args.precision)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    tflops = 3.0 * fe / dt / 1e12
+    if gemm_ms is None:
+        _, _, tf, tb = step(args.precision, timing=True)
+        fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf}
+        gemm_ms = fam["gemm"][0]
+    peak = MI355X_PEAK_TFLOPS[args.precision]
+    achieved = 4.0 * fe / (gemm_ms / 1e3) / 1e12
+
+    accuracy = None
+    if not args.no_compare and args.precision != "f64":
+        log("[bench] FP64 reference step for the accuracy report")
+        e64, bars64, _, _ = step("f64")
+        names = ("t1", "t2", "fock", "mo_energy", "ovoo", "ovov", "ovvv")
+        accuracy = {"dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64),
+                    "max_rel": {n: max_rel(b, r) for n, b, r in zip(names, bars, bars64)}}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline sample (~{args.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline(inst, o, v, args.cpu_seconds)
+
+    out = {
+        "metric": "(T) energy+VJP TFLOP/s (3*F_E credited)",
+        "value": tflops,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "s_per_solve": dt,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32 (FP64 denominators and sums)",
+                  "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"single (T) instance nocc={o} nvir={v} seed={args.seed}",
+                   "precision": args.precision, "E_T": float(e)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "gemm_grouped", "gemm_ms_per_step": gemm_ms,
+                     "gemm_launches_per_step": fam.get("gemm", (0, 0))[1],
+                     "family_ms": {k: val[0] for k, val in fam.items()}},
+        "cpu_baseline": cpu,
+        "accuracy_vs_f64": accuracy,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -80,6 +80,16 @@ size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int
 /* Text of the last error of the calling thread ("" if none). */
 const char *pyscfad_amd_last_error(void);
 
+/* Kernel timing of the calling thread's calls (no counterpart in the
+ * reference; for benchmarks and profiling).  With timing enabled every launch
+ * is bracketed by hipEvents on the call's stream and the call synchronises
+ * that stream before returning.  pyscfad_amd_last_timing fills up to n
+ * entries of device milliseconds and launch counts of the last call, by
+ * kernel family: 0 grouped MFMA GEMM, 1 orbit epilogue (+ its reduction),
+ * 2 V-term adjoint kernels, 3 split-K reductions; returns the count filled. */
+void pyscfad_amd_set_timing(int enable);
+int pyscfad_amd_last_timing(double *ms, long long *launches, int n);
+
 /* ABI version, currently 1. */
 int pyscfad_amd_abi_version(void);
 

@@ -19,6 +19,8 @@ EXPORTS = (
     "pyscfad_amd_ccsd_t_energy_vjp",
     "pyscfad_amd_ccsd_t_workspace_bytes",
     "pyscfad_amd_last_error",
+    "pyscfad_amd_set_timing",
+    "pyscfad_amd_last_timing",
     "pyscfad_amd_abi_version",
 )
 
@@ -45,6 +47,10 @@ def load():
     lib.pyscfad_amd_ccsd_t_workspace_bytes.argtypes = [i, i, i, i]
     lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
     lib.pyscfad_amd_last_error.argtypes = []
+    lib.pyscfad_amd_set_timing.restype = None
+    lib.pyscfad_amd_set_timing.argtypes = [i]
+    lib.pyscfad_amd_last_timing.restype = i
+    lib.pyscfad_amd_last_timing.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong), i]
     lib.pyscfad_amd_abi_version.restype = i
     lib.pyscfad_amd_abi_version.argtypes = []
     _lib = lib
@@ -55,3 +61,18 @@ def check(rc):
     if rc != 0:
         msg = load().pyscfad_amd_last_error().decode()
         raise T3Error(f"pyscfad_amd error {rc}: {msg}")
+
+
+FAMILIES = ("gemm", "epilogue", "vpart", "reduce")
+
+
+def set_timing(enable):
+    load().pyscfad_amd_set_timing(int(bool(enable)))
+
+
+def last_timing():
+    """{family: (ms, launches)} of the calling thread's last call with timing on."""
+    ms = (ctypes.c_double * 8)()
+    nl = (ctypes.c_longlong * 8)()
+    m = load().pyscfad_amd_last_timing(ms, nl, 8)
+    return {FAMILIES[f]: (ms[f], nl[f]) for f in range(min(m, len(FAMILIES)))}

@@ -61,6 +61,11 @@ constexpr int GEMM_TARGET_BLOCKS = 2048;
 constexpr int MAX_KSPLIT = 16;
 
 thread_local std::string g_last_error;
+// per-family device time (ms) of the last call, recorded with hipEvents
+enum Family { FAM_GEMM = 0, FAM_EPILOGUE = 1, FAM_VPART = 2, FAM_REDUCE = 3, FAM_PREP = 4, FAM_COUNT = 5 };
+thread_local int g_timing = 0;
+thread_local double g_family_ms[FAM_COUNT];
+thread_local long long g_family_launches[FAM_COUNT];
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
@@ -153,7 +158,7 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   r.slot_bytes = S.off;
   r.slots = L.take(r.slot_bytes * B);
   // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6 per triple
-  r.split_bytes = backward ? (size_t)6 * B * MAX_KSPLIT * V * V * d.elt : 0;
+  r.split_bytes = backward ? (size_t)6 * B * MAX_KSPLIT * V * std::max(V, Op) * d.elt + (size_t)6 * B * 256 : 0;
   r.split = L.take(r.split_bytes);
   r.desc = L.off;
   return r;
@@ -601,7 +606,20 @@ struct Ctx {
   hipError_t execute(hipStream_t st) {
     char* db = ws + r.desc;
     hipError_t e = hipSuccess;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> fam;
     for (const Launch& L : launches) {
+      if (g_timing) {
+        hipEvent_t a, b;
+        if ((e = hipEventCreate(&a)) != hipSuccess) return e;
+        if ((e = hipEventCreate(&b)) != hipSuccess) return e;
+        ev.push_back(a);
+        ev.push_back(b);
+        fam.push_back(L.kind == L_GEMM ? FAM_GEMM
+                      : (L.kind == L_EPI || L.kind == L_EPIRED) ? FAM_EPILOGUE
+                      : L.kind == L_VPART ? FAM_VPART : FAM_REDUCE);
+        if ((e = hipEventRecord(a, st)) != hipSuccess) return e;
+      }
       switch (L.kind) {
         case L_GEMM:
           e = t3_launch_gemm(d.prec, L.cfg, reinterpret_cast<const GemmProb*>(db + L.off_a), L.n_a,
@@ -629,6 +647,17 @@ struct Ctx {
           break;
       }
       if (e != hipSuccess) return e;
+      if (g_timing && (e = hipEventRecord(ev.back(), st)) != hipSuccess) return e;
+    }
+    if (g_timing) {
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      for (size_t x = 0; x < fam.size(); x++) {
+        float ms = 0.f;
+        if ((e = hipEventElapsedTime(&ms, ev[2 * x], ev[2 * x + 1])) != hipSuccess) return e;
+        g_family_ms[fam[x]] += ms;
+        g_family_launches[fam[x]]++;
+      }
+      for (auto& x : ev) (void)hipEventDestroy(x);
     }
     return hipSuccess;
   }
@@ -703,6 +732,7 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   auto hip_fail = [&](hipError_t e, const char* where) {
     return cleanup(fail(PYSCFAD_AMD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e)));
   };
+  for (int f = 0; f < FAM_COUNT; f++) { g_family_ms[f] = 0.0; g_family_launches[f] = 0; }
   hipError_t e;
   e = hipMemcpyAsync(c.ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
@@ -768,6 +798,17 @@ size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int
 }
 
 const char* pyscfad_amd_last_error(void) { return g_last_error.c_str(); }
+
+void pyscfad_amd_set_timing(int enable) { g_timing = enable != 0; }
+
+int pyscfad_amd_last_timing(double* ms, long long* launches, int n) {
+  const int m = n < FAM_COUNT ? n : FAM_COUNT;
+  for (int f = 0; f < m; f++) {
+    if (ms) ms[f] = g_family_ms[f];
+    if (launches) launches[f] = g_family_launches[f];
+  }
+  return m;
+}
 
 int pyscfad_amd_abi_version(void) { return 1; }
 
