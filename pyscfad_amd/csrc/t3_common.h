@@ -73,6 +73,7 @@ struct EpiTriple {
   void* G;            // V^3 (backward)
   void* H;            // V^3 (backward)
   double* part;       // [ncube][EPI_PSTRIDE]
+  const double* mn6;  // [6 ordered pairs][M, N][V][V]: M_pq[x][y] = ovov[p,x,q,y], N_pq[x][y] = t2[q,p,y,x]
 };
 
 struct EpiArgs {
@@ -103,8 +104,9 @@ struct EpiReduceArgs {
 struct VpTriple {
   int i, j, k;
   const void* H;
-  double* out;     // 12 V^2
+  double* out;     // 30 V^2: [0,12) contractions, [12,24) mn6, [24,30) CM_d / CN_d
 };
+constexpr int VP_OUT = 0, VP_MN6 = 12, VP_CM = 24, VP_TOTAL = 30;   // offsets in V^2 units
 
 struct VpSlab {
   int p, q;

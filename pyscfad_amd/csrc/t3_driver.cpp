@@ -47,6 +47,8 @@ hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, cons
                            const VpContrib* contribs, int o, int v, int V, const double* t1, const double* fock,
                            const double* ovov, const double* t2, double* ovov_acc, double* t2v_acc,
                            double* t1_acc, double* fvo_acc, hipStream_t s);
+hipError_t t3_launch_vmn(const VpTriple* trs, int ntriple, int o, int v, int V, int with_cm, const double* ovov,
+                         const double* t2, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
                        const double* t2hacc, const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
@@ -152,8 +154,8 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   if (backward) {
     r.slot_G = S.take(V * V * V * d.elt);
     r.slot_H = S.take(V * V * V * d.elt);
-    r.slot_vp = S.take(12 * V * V * 8);
   }
+  r.slot_vp = S.take((size_t)VP_TOTAL * V * V * 8);
   r.slot_part = S.take((size_t)d.ncube * EPI_PSTRIDE * 8);
   r.slot_bytes = S.off;
   r.slots = L.take(r.slot_bytes * B);
@@ -166,7 +168,7 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
 
 // ----------------------------------------------------------------------------
 // Launch plan
-enum LaunchKind { L_GEMM, L_REDUCE, L_EPI, L_EPIRED, L_VPART };
+enum LaunchKind { L_GEMM, L_REDUCE, L_EPI, L_EPIRED, L_VPART, L_VMN };
 
 struct Launch {
   LaunchKind kind;
@@ -254,9 +256,19 @@ struct Ctx {
     return reinterpret_cast<const double*>(ws + region) + elems;
   }
 
+  // 128x128 tiles for large FP32 / bf16x3 problems; FP64 stays on 64x64 (its
+  // 128x128 variant needs 64 KB of LDS and ~340 registers: one wave per SIMD).
+  int pick_cfg(const GemmBuild& gb) const {
+    if (d.prec == PREC_F64) return 0;
+    for (const auto& P : gb.probs)
+      if (P.M < 128 || P.N < 128) return 0;
+    return 1;
+  }
+
   // Flatten a GemmBuild into a launch.
   void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split) {
     if (gb.empty()) return;
+    if (cfg < 0) cfg = pick_cfg(gb);
     const int BM = t3_gemm_tile_m(cfg), BN = t3_gemm_tile_n(cfg);
     long long tiles = 0;
     int min_chunks = 1 << 30;
@@ -392,7 +404,7 @@ struct Ctx {
         for (int q = 0; q < np; q++) gb.add(p, particle_seg(bt[s], pass == 0 ? P1[q] : P2[q], ib, im));
         for (int q = 0; q < np; q++) gb.add(p, hole_seg(bt[s], pass == 0 ? H1[q] : H2[q], ib, im));
       }
-      emit_gemm(gb, 0, pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false);
+      emit_gemm(gb, -1, pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false);
     }
   }
 
@@ -406,8 +418,24 @@ struct Ctx {
       e.G = bw ? slot((int)s, r.slot_G) : nullptr;
       e.H = bw ? slot((int)s, r.slot_H) : nullptr;
       e.part = reinterpret_cast<double*>(slot((int)s, r.slot_part));
+      e.mn6 = reinterpret_cast<const double*>(slot((int)s, r.slot_vp)) + (size_t)VP_MN6 * d.V * d.V;
       et.push_back(e);
     }
+    // compact V-term operands of the batch (and the backward's CM/CN)
+    std::vector<VpTriple> vt;
+    for (size_t s = 0; s < bt.size(); s++) {
+      VpTriple t{};
+      t.i = bt[s].t[0]; t.j = bt[s].t[1]; t.k = bt[s].t[2];
+      t.H = bw ? slot((int)s, r.slot_H) : nullptr;
+      t.out = reinterpret_cast<double*>(slot((int)s, r.slot_vp));
+      vt.push_back(t);
+    }
+    Launch M;
+    M.kind = L_VMN;
+    M.off_a = blob.put(vt);
+    M.n_a = (int)vt.size();
+    M.aux = bw ? 1 : 0;
+    launches.push_back(M);
     EpiArgs a{};
     a.nocc = d.o; a.nvir = d.v; a.V = d.V; a.ntile = d.ntile; a.ncube = d.ncube;
     a.backward = bw; a.et_bar = et_bar;
@@ -460,7 +488,7 @@ struct Ctx {
           sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
           gb.add(q, sg);
         }
-      emit_gemm(gb, 0, GEMM_ACCUM, 1, false);
+      emit_gemm(gb, -1, GEMM_ACCUM, 1, false);
     }
   }
 
@@ -489,7 +517,7 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, 0, GEMM_ACCUM, 1, true);
+    emit_gemm(gb, -1, GEMM_ACCUM, 1, true);
   }
 
   void emit_b3(const std::vector<Triple>& bt) {
@@ -617,7 +645,7 @@ struct Ctx {
         ev.push_back(b);
         fam.push_back(L.kind == L_GEMM ? FAM_GEMM
                       : (L.kind == L_EPI || L.kind == L_EPIRED) ? FAM_EPILOGUE
-                      : L.kind == L_VPART ? FAM_VPART : FAM_REDUCE);
+                      : (L.kind == L_VPART || L.kind == L_VMN) ? FAM_VPART : FAM_REDUCE);
         if ((e = hipEventRecord(a, st)) != hipSuccess) return e;
       }
       switch (L.kind) {
@@ -638,6 +666,10 @@ struct Ctx {
           e = t3_launch_epi_reduce(&a, st);
           break;
         }
+        case L_VMN:
+          e = t3_launch_vmn(reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a, d.o, d.v, d.V, L.aux, ovov, t2,
+                            st);
+          break;
         case L_VPART:
           e = t3_launch_vpart(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a,
                               reinterpret_cast<const VpSlab*>(db + L.off_b), L.n_b,

@@ -28,9 +28,12 @@
 
 namespace t3 {
 
-// Position permutations in the order (012),(021),(102),(120),(201),(210)
-__constant__ int c_perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-__constant__ int c_parity[6] = {0, 1, 1, 0, 0, 1};
+// Position permutations in the order (012),(021),(102),(120),(201),(210).
+// constexpr so that, inside fully unrolled loops, every lookup is a constant.
+constexpr int c_perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr int c_parity[6] = {0, 1, 1, 0, 0, 1};
+
+__device__ __forceinline__ int sel3(int i, int a, int b, int c) { return i == 0 ? a : (i == 1 ? b : c); }
 
 
 
@@ -50,16 +53,14 @@ __device__ __forceinline__ void decode_cube(int m, int& A, int& B, int& C) {
   A = a; B = b; C = m;
 }
 
-template <class TP>
-__global__ void __launch_bounds__(256)
+template <class TP, bool BW>
+__global__ void __launch_bounds__(256, 2)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   constexpr int T = EPI_T, T3 = T * T * T;
   const EpiTriple tr = triples[blockIdx.y];
   const int cube = blockIdx.x;
   int TA, TB, TC;
   decode_cube(cube, TA, TB, TC);
-  const int tile[3] = {TA, TB, TC};
-  const int occ[3] = {tr.i, tr.j, tr.k};
   const int V = args.V, v = args.nvir, o = args.nocc;
   const int tid = threadIdx.x;
 
@@ -74,28 +75,35 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   for (int e = tid; e < 6 * T3; e += 256) {
     const int p = e / T3, r = e % T3;
     const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
-    const long long g0 = tile[c_perm[p][0]] * T + x0, g1 = tile[c_perm[p][1]] * T + x1,
-                    g2 = tile[c_perm[p][2]] * T + x2;
+    int q0, q1, q2;
+    switch (p) {
+      case 0: q0 = TA; q1 = TB; q2 = TC; break;
+      case 1: q0 = TA; q1 = TC; q2 = TB; break;
+      case 2: q0 = TB; q1 = TA; q2 = TC; break;
+      case 3: q0 = TB; q1 = TC; q2 = TA; break;
+      case 4: q0 = TC; q1 = TA; q2 = TB; break;
+      default: q0 = TC; q1 = TB; q2 = TA; break;
+    }
+    const long long g0 = q0 * T + x0, g1 = q1 * T + x1, g2 = q2 * T + x2;
     sc[p][r] = w0[(g0 * V + g1) * V + g2];
   }
-  // --- M_{pq}[x][y] = ovov[p,x,q,y], N_{pq}[x][y] = t2[q,p,y,x] on the tile pairs ---
-  for (int e = tid; e < 2 * 36 * T * T; e += 256) {
-    const int which = e / (36 * T * T);
-    const int r = e % (36 * T * T);
-    const int rho = r / (6 * T * T), mu = (r / (T * T)) % 6, xy = r % (T * T);
-    const int p = occ[c_perm[rho][0]], q = occ[c_perm[rho][1]];
-    const int x = tile[c_perm[mu][0]] * T + xy / T, y = tile[c_perm[mu][1]] * T + xy % T;
-    double val = 0.0;
-    if (x < v && y < v) {
-      if (which == 0) val = args.ovov[(((long long)p * v + x) * o + q) * v + y];
-      else val = args.t2[(((long long)q * o + p) * v + y) * v + x];
+  // --- M_{pq}[x][y] = ovov[p,x,q,y], N_{pq}[x][y] = t2[q,p,y,x] on the tile pairs,
+  //     from the triple's compact copy mn6[rho][M/N][V][V] (rho = ordered pair (t_rho0, t_rho1)) ---
+  {
+    const long long VV = (long long)V * V;
+    for (int e = tid; e < 2 * 36 * T * T; e += 256) {
+      const int which = e / (36 * T * T);
+      const int r = e % (36 * T * T);
+      const int rho = r / (6 * T * T), mu = (r / (T * T)) % 6, xy = r % (T * T);
+      const int x = sel3(c_perm[mu][0], TA, TB, TC) * T + xy / T;
+      const int y = sel3(c_perm[mu][1], TA, TB, TC) * T + xy % T;
+      mn[which][rho][mu][xy] = tr.mn6[(2 * rho + which) * VV + (long long)x * V + y];
     }
-    mn[which][rho][mu][xy] = val;
   }
   for (int e = tid; e < 2 * 3 * 3 * T; e += 256) {
     const int which = e / (9 * T), r = e % (9 * T);
-    const int op = r / (3 * T), tp = (r / T) % 3, z = tile[tp] * T + r % T;
-    const int rr = occ[op];
+    const int op = r / (3 * T), tp = (r / T) % 3, z = sel3(tp, TA, TB, TC) * T + r % T;
+    const int rr = sel3(op, tr.i, tr.j, tr.k);
     double val = 0.0;
     if (z < v) val = which == 0 ? args.t1[(long long)rr * v + z]
                                 : args.fock[(long long)(o + z) * (o + v) + rr];
@@ -113,10 +121,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
     rbuf[0][pt] = 0.0;
     rbuf[1][pt] = 0.0;
-    if ((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2])) {
-      for (int p = 0; p < 6; p++) { gval[npt][p] = 0.0; hval[npt][p] = 0.0; }
-      continue;
-    }
+    if ((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2])) continue;
     const int s0 = TA * T + al[0], s1 = TB * T + al[1], s2 = TC * T + al[2];
     const double fp = (s0 == s2) ? (1.0 / 6.0) : ((s0 == s1 || s1 == s2) ? 0.5 : 1.0);
     const double D = eijk - args.eps_vir[s0] - args.eps_vir[s1] - args.eps_vir[s2];
@@ -149,8 +154,10 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       const double Z = 3.0 * U[p] + sU[par] - 2.0 * sU[par ^ 1];
       const double CW = 3.0 * W[p] + sW[par] - 2.0 * sW[par ^ 1];
       wz += W[p] * Z;
-      gval[npt][p] = s_bw * (Z + CW) * invD;
-      hval[npt][p] = s_bw * CW * invD;
+      if constexpr (BW) {
+        gval[npt][p] = s_bw * (Z + CW) * invD;
+        hval[npt][p] = s_bw * CW * invD;
+      }
     }
     rbuf[0][pt] = fp * wz * invD;                    // Q contribution
     rbuf[1][pt] = s_bw * fp * wz * invD * invD;      // mo_energy adjoint weight
@@ -166,7 +173,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     __syncthreads();
   }
   if (tid == 0) part[0] = red[0];
-  if (!args.backward) return;
+  if constexpr (!BW) return;
 
   // mo_energy adjoint partials
   __syncthreads();
@@ -212,34 +219,36 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   // block also holds it (one writer per element).
   TP* G = reinterpret_cast<TP*>(tr.G);
   TP* H = reinterpret_cast<TP*>(tr.H);
-  for (int e = tid; e < 6 * T3; e += 256) {
-    const int p = e / T3, r = e % T3;
-    const int x[3] = {r / (T * T), (r / T) % T, r % T};
-    bool write = false;
-    for (int q = 0; q <= p && !write; q++) {
-      if (tile[c_perm[q][0]] != tile[c_perm[p][0]] || tile[c_perm[q][1]] != tile[c_perm[p][1]] ||
-          tile[c_perm[q][2]] != tile[c_perm[p][2]])
-        continue;
-      // local canonical coordinates of this entry as seen from sub-cube q
-      int al[3];
-      al[c_perm[q][0]] = x[0];
-      al[c_perm[q][1]] = x[1];
-      al[c_perm[q][2]] = x[2];
-      const bool canon = !((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2]));
-      if (canon) {
-        if (q == p) write = true;
-        else break;   // a lower sub-cube owns it
+  const int tt[3] = {TA, TB, TC};
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    const long long b0 = (long long)tt[c_perm[p][0]] * T, b1 = (long long)tt[c_perm[p][1]] * T,
+                    b2 = (long long)tt[c_perm[p][2]] * T;
+    for (int r = tid; r < T3; r += 256) {
+      const int x[3] = {r / (T * T), (r / T) % T, r % T};
+      bool write = false, owned = false;
+#pragma unroll
+      for (int q = 0; q <= p; q++) {
+        const bool same = tt[c_perm[q][0]] == tt[c_perm[p][0]] && tt[c_perm[q][1]] == tt[c_perm[p][1]] &&
+                          tt[c_perm[q][2]] == tt[c_perm[p][2]];
+        // local canonical coordinates of this entry as seen from sub-cube q
+        int al[3];
+        al[c_perm[q][0]] = x[0];
+        al[c_perm[q][1]] = x[1];
+        al[c_perm[q][2]] = x[2];
+        const bool canon = !((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2]));
+        if (same && canon && !owned) {
+          if (q == p) write = true;
+          else owned = true;   // a lower sub-cube owns it
+        }
       }
+      if (!write) continue;
+      const long long off = ((b0 + x[0]) * V + (b1 + x[1])) * V + (b2 + x[2]);
+      G[off] = sc[p][r];
+      H[off] = (TP)hl[p * T3 + r];
     }
-    if (!write) continue;
-    const long long g0 = tile[c_perm[p][0]] * T + x[0], g1 = tile[c_perm[p][1]] * T + x[1],
-                    g2 = tile[c_perm[p][2]] * T + x[2];
-    const long long off = (g0 * V + g1) * V + g2;
-    G[off] = sc[p][r];
-    H[off] = (TP)hl[p * T3 + r];
   }
 }
-
 
 __device__ __forceinline__ int cube_id(int A, int B, int C) {
   return A * (A + 1) * (A + 2) / 6 + B * (B + 1) / 2 + C;
@@ -317,8 +326,13 @@ extern "C" hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, con
                                          int ntriple, hipStream_t stream) {
   if (ntriple <= 0) return hipSuccess;
   dim3 grid(args->ncube, ntriple), block(256);
-  if (tp_is_f64) hipLaunchKernelGGL(orbit_epilogue<double>, grid, block, 0, stream, *args, triples);
-  else hipLaunchKernelGGL(orbit_epilogue<float>, grid, block, 0, stream, *args, triples);
+  if (args->backward) {
+    if (tp_is_f64) hipLaunchKernelGGL((orbit_epilogue<double, true>), grid, block, 0, stream, *args, triples);
+    else hipLaunchKernelGGL((orbit_epilogue<float, true>), grid, block, 0, stream, *args, triples);
+  } else {
+    if (tp_is_f64) hipLaunchKernelGGL((orbit_epilogue<double, false>), grid, block, 0, stream, *args, triples);
+    else hipLaunchKernelGGL((orbit_epilogue<float, false>), grid, block, 0, stream, *args, triples);
+  }
   return hipGetLastError();
 }
 

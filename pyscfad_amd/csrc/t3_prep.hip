@@ -105,6 +105,8 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
   const int occ[3] = {tr.i, tr.j, tr.k};
   const long long VV = (long long)V * V;
   const int n = o + v;
+  (void)ovov;
+  (void)t2;
   if (task < 3) {
     const int d = task;
     // H index with s_d = x and the other two = (u, w) in increasing position order
@@ -127,13 +129,11 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
   const int d = task - 3;
   int e1, e2;
   if (d == 0) { e1 = 1; e2 = 2; } else if (d == 1) { e1 = 0; e2 = 2; } else { e1 = 0; e2 = 1; }
-  const int p1 = occ[e1], p2 = occ[e2];
+  const double* CM = tr.out + (VP_CM + 2 * d) * VV;
+  const double* CN = tr.out + (VP_CM + 2 * d + 1) * VV;
   auto cm = [&](int x, int y, double& mval, double& nval) {
-    // CM_d[x][y] = M_{p1 p2}[x][y] + M_{p2 p1}[y][x];  CN_d likewise with N
-    mval = 0.0; nval = 0.0;
-    if (x >= v || y >= v) return;
-    mval = ovov[(((long long)p1 * v + x) * o + p2) * v + y] + ovov[(((long long)p2 * v + y) * o + p1) * v + x];
-    nval = t2[(((long long)p2 * o + p1) * v + y) * v + x] + t2[(((long long)p1 * o + p2) * v + x) * v + y];
+    mval = CM[(long long)x * V + y];
+    nval = CN[(long long)x * V + y];
   };
   double sm = 0.0, sn = 0.0;
   if (d == 2) {
@@ -160,6 +160,46 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
   }
   tr.out[(6 + 2 * d) * VV + e] = sm;
   tr.out[(7 + 2 * d) * VV + e] = sn;
+}
+
+// Compact per-triple copies of the V-term operands (FP64):
+//   mn6[rho][0] = M_pq[x][y] = ovov[p,x,q,y],  mn6[rho][1] = N_pq[x][y] = t2[q,p,y,x]
+//   for the six ordered pairs (p,q) = (t_rho0, t_rho1) of the triple, and
+//   CM_d[x][y] = M_{p1 p2}[x][y] + M_{p2 p1}[y][x] (CN_d likewise with N),
+//   (p1,p2) = (t_e1, t_e2), e1 < e2 the positions other than d.
+// Tasks 0..11: mn6 (rho = task/2, M/N = task%2); tasks 12..17: CM/CN.
+__constant__ int c_vperm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+__global__ void __launch_bounds__(256)
+vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
+          const double* __restrict__ ovov, const double* __restrict__ t2) {
+  const VpTriple tr = trs[blockIdx.z];
+  const int task = blockIdx.y;
+  if (task >= ntask) return;
+  const long long VV = (long long)V * V;
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= VV) return;
+  const int x = (int)(e / V), y = (int)(e % V);
+  const int occ[3] = {tr.i, tr.j, tr.k};
+  auto M = [&](int p, int q, int a, int b) -> double {
+    return (a < v && b < v) ? ovov[(((long long)p * v + a) * o + q) * v + b] : 0.0;
+  };
+  auto N = [&](int p, int q, int a, int b) -> double {
+    return (a < v && b < v) ? t2[(((long long)q * o + p) * v + b) * v + a] : 0.0;
+  };
+  if (task < 12) {
+    const int rho = task / 2, which = task % 2;
+    const int p = occ[c_vperm[rho][0]], q = occ[c_vperm[rho][1]];
+    tr.out[(VP_MN6 + task) * VV + e] = which == 0 ? M(p, q, x, y) : N(p, q, x, y);
+    (void)rho;
+    return;
+  }
+  const int d = (task - 12) / 2, which = (task - 12) % 2;
+  int e1, e2;
+  if (d == 0) { e1 = 1; e2 = 2; } else if (d == 1) { e1 = 0; e2 = 2; } else { e1 = 0; e2 = 1; }
+  const int p1 = occ[e1], p2 = occ[e2];
+  tr.out[(VP_CM + task - 12) * VV + e] =
+      which == 0 ? M(p1, p2, x, y) + M(p2, p1, y, x) : N(p1, p2, x, y) + N(p2, p1, y, x);
 }
 
 // Scatter of the V-term adjoints of one batch into the FP64 accumulators.
@@ -346,6 +386,16 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
   dim3 g2((unsigned)((nx + 255) / 256), nslab + 1);
   hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, V, ovov_acc,
                      t2v_acc, t1_acc, fvo_acc);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_launch_vmn(const VpTriple* trs, int ntriple, int o, int v, int V, int with_cm,
+                                    const double* ovov, const double* t2, hipStream_t s) {
+  if (ntriple <= 0) return hipSuccess;
+  const long long VV = (long long)V * V;
+  const int ntask = with_cm ? 18 : 12;
+  dim3 g((unsigned)((VV + 255) / 256), ntask, ntriple);
+  hipLaunchKernelGGL(vmn_build, g, dim3(256), 0, s, trs, o, v, V, ntask, ovov, t2);
   return hipGetLastError();
 }
 
