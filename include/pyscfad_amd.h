@@ -90,6 +90,29 @@ const char *pyscfad_amd_last_error(void);
 void pyscfad_amd_set_timing(int enable);
 int pyscfad_amd_last_timing(double *ms, long long *launches, int n);
 
+/*
+ * XLA custom-call handlers for jax on ROCm, legacy form (api_version 0):
+ *     void handler(hipStream_t stream, void **buffers, const char *opaque, size_t opaque_len)
+ * registered with jax.ffi.register_ffi_target(..., platform="ROCM", api_version=0).
+ * They replace the two calls of the reference's custom_vjp: the forward
+ * (pyscfad/cc/ccsd_t.py:41) and the backward's ctypes call of
+ * ccsd_t_energy_vjp (ccsd_t.py:107-133, ccsd_t_vjp.c:332-346).  The
+ * reference registers its GPU handlers through typed XLA FFI
+ * (pyscfadlib/pyscfadlib/cuda/solver_kernels.cc:144-157); the legacy form is
+ * used here because it needs no XLA headers.
+ *   energy:     buffers = {mo_energy, t1, t2, fock, ovoo, ovov, ovvv, e_out}
+ *   energy_vjp: buffers = {mo_energy, t1, t2, fock, ovoo, ovov, ovvv,
+ *                          t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar}
+ * The legacy ABI has no error channel: failures are printed on stderr.
+ */
+typedef struct {
+    int nocc, nvir, precision, ovvv_packed;
+    double et_bar; /* energy_vjp only */
+} PyscfadAmdXlaOpaque;
+
+void pyscfad_amd_xla_ccsd_t_energy(void *stream, void **buffers, const char *opaque, size_t opaque_len);
+void pyscfad_amd_xla_ccsd_t_energy_vjp(void *stream, void **buffers, const char *opaque, size_t opaque_len);
+
 /* ABI version, currently 1. */
 int pyscfad_amd_abi_version(void);
 

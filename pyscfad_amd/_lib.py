@@ -21,6 +21,8 @@ EXPORTS = (
     "pyscfad_amd_last_error",
     "pyscfad_amd_set_timing",
     "pyscfad_amd_last_timing",
+    "pyscfad_amd_xla_ccsd_t_energy",
+    "pyscfad_amd_xla_ccsd_t_energy_vjp",
     "pyscfad_amd_abi_version",
 )
 
@@ -51,6 +53,10 @@ def load():
     lib.pyscfad_amd_set_timing.argtypes = [i]
     lib.pyscfad_amd_last_timing.restype = i
     lib.pyscfad_amd_last_timing.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong), i]
+    for name in ("pyscfad_amd_xla_ccsd_t_energy", "pyscfad_amd_xla_ccsd_t_energy_vjp"):
+        fn = getattr(lib, name)
+        fn.restype = None
+        fn.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, sz]
     lib.pyscfad_amd_abi_version.restype = i
     lib.pyscfad_amd_abi_version.argtypes = []
     _lib = lib
@@ -76,3 +82,9 @@ def last_timing():
     nl = (ctypes.c_longlong * 8)()
     m = load().pyscfad_amd_last_timing(ms, nl, 8)
     return {FAMILIES[f]: (ms[f], nl[f]) for f in range(min(m, len(FAMILIES)))}
+
+
+class XlaOpaque(ctypes.Structure):
+    """PyscfadAmdXlaOpaque of include/pyscfad_amd.h (the opaque bytes of the XLA custom calls)."""
+    _fields_ = [("nocc", ctypes.c_int), ("nvir", ctypes.c_int), ("precision", ctypes.c_int),
+                ("ovvv_packed", ctypes.c_int), ("et_bar", ctypes.c_double)]

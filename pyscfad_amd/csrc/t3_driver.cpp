@@ -831,6 +831,43 @@ size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int
 
 const char* pyscfad_amd_last_error(void) { return g_last_error.c_str(); }
 
+// XLA legacy custom-call entry points (api_version 0, platform ROCM): the
+// handler signature needs no XLA headers.  buffers[] holds the operands then
+// the results, opaque a PyscfadAmdXlaOpaque.  The legacy ABI has no error
+// channel: a failure is reported on stderr and the results are left unset.
+void pyscfad_amd_xla_ccsd_t_energy(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+  PyscfadAmdXlaOpaque op{};
+  if (!opaque || opaque_len < sizeof(op)) {
+    std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy: opaque of %zu bytes, need %zu\n", opaque_len, sizeof(op));
+    return;
+  }
+  std::memcpy(&op, opaque, sizeof(op));
+  const double* in[7];
+  for (int x = 0; x < 7; x++) in[x] = reinterpret_cast<const double*>(buffers[x]);
+  const int rc = pyscfad_amd_ccsd_t_energy(op.nocc, op.nvir, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
+                                           op.ovvv_packed, op.precision, reinterpret_cast<double*>(buffers[7]),
+                                           nullptr, 0, stream);
+  if (rc) std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy: error %d: %s\n", rc, g_last_error.c_str());
+}
+
+void pyscfad_amd_xla_ccsd_t_energy_vjp(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+  PyscfadAmdXlaOpaque op{};
+  if (!opaque || opaque_len < sizeof(op)) {
+    std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy_vjp: opaque of %zu bytes, need %zu\n", opaque_len,
+                 sizeof(op));
+    return;
+  }
+  std::memcpy(&op, opaque, sizeof(op));
+  const double* in[7];
+  double* out[7];
+  for (int x = 0; x < 7; x++) in[x] = reinterpret_cast<const double*>(buffers[x]);
+  for (int x = 0; x < 7; x++) out[x] = reinterpret_cast<double*>(buffers[7 + x]);
+  const int rc = pyscfad_amd_ccsd_t_energy_vjp(op.nocc, op.nvir, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
+                                               op.ovvv_packed, op.et_bar, op.precision, out[0], out[1], out[2],
+                                               out[3], out[4], out[5], out[6], nullptr, nullptr, 0, stream);
+  if (rc) std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy_vjp: error %d: %s\n", rc, g_last_error.c_str());
+}
+
 void pyscfad_amd_set_timing(int enable) { g_timing = enable != 0; }
 
 int pyscfad_amd_last_timing(double* ms, long long* launches, int n) {

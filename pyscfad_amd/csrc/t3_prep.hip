@@ -168,7 +168,6 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
 //   CM_d[x][y] = M_{p1 p2}[x][y] + M_{p2 p1}[y][x] (CN_d likewise with N),
 //   (p1,p2) = (t_e1, t_e2), e1 < e2 the positions other than d.
 // Tasks 0..11: mn6 (rho = task/2, M/N = task%2); tasks 12..17: CM/CN.
-__constant__ int c_vperm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
 __global__ void __launch_bounds__(256)
 vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
@@ -189,7 +188,9 @@ vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
   };
   if (task < 12) {
     const int rho = task / 2, which = task % 2;
-    const int p = occ[c_vperm[rho][0]], q = occ[c_vperm[rho][1]];
+    // ordered pair (rho0, rho1) of the permutation rho in the order (01),(02),(10),(12),(20),(21)
+    const int r0 = rho / 2, r1 = (rho % 2 == 0) ? (r0 == 0 ? 1 : 0) : (r0 == 2 ? 1 : 2);
+    const int p = occ[r0], q = occ[r1];
     tr.out[(VP_MN6 + task) * VV + e] = which == 0 ? M(p, q, x, y) : N(p, q, x, y);
     (void)rho;
     return;

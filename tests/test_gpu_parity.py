@@ -60,3 +60,29 @@ def test_vjp_f64(gpu_lib, o, v, seed, fs, packed):
         scale = max(np.abs(r).max(), 1e-300)
         err = np.abs(g - r).max()
         assert err <= max(REL_TOL * scale, ABS_FLOOR), (name, err, scale)
+
+
+def test_xla_legacy_handlers(gpu_lib):
+    """The XLA custom-call entry points give the same results as the direct ABI."""
+    import ctypes
+    import torch
+    from pyscfad_amd import _lib
+    from pyscfad_amd.cc import ccsd_t
+    d = _inst(4, 21, 8, 0.02, True)
+    keys = ("mo_energy", "t1", "t2", "fock", "ovoo", "ovov", "ovvv")
+    ins = [torch.as_tensor(d[k]).cuda().contiguous() for k in keys]
+    e_out = torch.zeros((), dtype=torch.float64, device="cuda")
+    op = _lib.XlaOpaque(4, 21, _lib.PREC_F64, 1, 0.5)
+    stream = torch.cuda.current_stream().cuda_stream
+    bufs = (ctypes.c_void_p * 8)(*[t.data_ptr() for t in ins], e_out.data_ptr())
+    gpu_lib.pyscfad_amd_xla_ccsd_t_energy(stream, bufs, bytes(op), ctypes.sizeof(op))
+    torch.cuda.synchronize()
+    assert abs(float(e_out) - orc.energy(**d)) <= E_TOL
+    outs = [torch.empty_like(ins[i]) for i in (1, 2, 3, 0, 4, 5, 6)]   # t1,t2,fock,mo_energy,ovoo,ovov,ovvv
+    bufs = (ctypes.c_void_p * 14)(*[t.data_ptr() for t in ins], *[t.data_ptr() for t in outs])
+    gpu_lib.pyscfad_amd_xla_ccsd_t_energy_vjp(stream, bufs, bytes(op), ctypes.sizeof(op))
+    torch.cuda.synchronize()
+    ref = ccsd_t.energy_vjp(*[d[k] for k in ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")],
+                            et_bar=0.5)
+    for a, b in zip(outs, ref):
+        assert torch.equal(a, b)
