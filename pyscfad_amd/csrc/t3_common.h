@@ -73,7 +73,7 @@ struct EpiTriple {
   void* G;            // V^3 (backward)
   void* H;            // V^3 (backward)
   double* part;       // [ncube][EPI_PSTRIDE]
-  const double* mn6;  // [6 ordered pairs][M, N][V][V]: M_pq[x][y] = ovov[p,x,q,y], N_pq[x][y] = t2[q,p,y,x]
+  const void* v0;     // V^3 cube of the V term v0 (built by v0_build)
 };
 
 struct EpiArgs {
@@ -104,9 +104,10 @@ struct EpiReduceArgs {
 struct VpTriple {
   int i, j, k;
   const void* H;
-  double* out;     // 30 V^2: [0,12) contractions, [12,24) mn6, [24,30) CM_d / CN_d
+  double* out;     // 42 V^2: [0,12) contractions, [12,24) mn6, [24,30) CM_d / CN_d, [30,42) mn6 transposed
+  void* v0;        // V^3 cube of the V term (TP)
 };
-constexpr int VP_OUT = 0, VP_MN6 = 12, VP_CM = 24, VP_TOTAL = 30;   // offsets in V^2 units
+constexpr int VP_OUT = 0, VP_MN6 = 12, VP_CM = 24, VP_MN6T = 30, VP_TOTAL = 42;   // offsets in V^2 units
 
 struct VpSlab {
   int p, q;

@@ -49,6 +49,8 @@ hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, cons
                            double* t1_acc, double* fvo_acc, hipStream_t s);
 hipError_t t3_launch_vmn(const VpTriple* trs, int ntriple, int o, int v, int V, int with_cm, const double* ovov,
                          const double* t2, hipStream_t s);
+hipError_t t3_launch_v0(int tp_is_f64, const VpTriple* trs, int ntriple, int o, int v, int V, const double* t1,
+                        const double* fock, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
                        const double* t2hacc, const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
@@ -114,7 +116,7 @@ struct Regions {
   size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, energy;
   size_t ovvv_acc, t2acc, t2hacc, t2vacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
   size_t slots, slot_bytes;
-  size_t slot_w0, slot_G, slot_H, slot_part, slot_vp;
+  size_t slot_w0, slot_v0, slot_G, slot_H, slot_part, slot_vp;
   size_t split, split_bytes;
   size_t desc, desc_bytes;
   size_t total;
@@ -151,6 +153,7 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   }
   Layout S;
   r.slot_w0 = S.take(V * V * V * d.elt);
+  r.slot_v0 = S.take(V * V * V * d.elt);
   if (backward) {
     r.slot_G = S.take(V * V * V * d.elt);
     r.slot_H = S.take(V * V * V * d.elt);
@@ -418,7 +421,7 @@ struct Ctx {
       e.G = bw ? slot((int)s, r.slot_G) : nullptr;
       e.H = bw ? slot((int)s, r.slot_H) : nullptr;
       e.part = reinterpret_cast<double*>(slot((int)s, r.slot_part));
-      e.mn6 = reinterpret_cast<const double*>(slot((int)s, r.slot_vp)) + (size_t)VP_MN6 * d.V * d.V;
+      e.v0 = slot((int)s, r.slot_v0);
       et.push_back(e);
     }
     // compact V-term operands of the batch (and the backward's CM/CN)
@@ -428,6 +431,7 @@ struct Ctx {
       t.i = bt[s].t[0]; t.j = bt[s].t[1]; t.k = bt[s].t[2];
       t.H = bw ? slot((int)s, r.slot_H) : nullptr;
       t.out = reinterpret_cast<double*>(slot((int)s, r.slot_vp));
+      t.v0 = slot((int)s, r.slot_v0);
       vt.push_back(t);
     }
     Launch M;
@@ -669,6 +673,9 @@ struct Ctx {
         case L_VMN:
           e = t3_launch_vmn(reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a, d.o, d.v, d.V, L.aux, ovov, t2,
                             st);
+          if (e == hipSuccess)
+            e = t3_launch_v0(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a, d.o, d.v, d.V,
+                             t1, fock, st);
           break;
         case L_VPART:
           e = t3_launch_vpart(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a,

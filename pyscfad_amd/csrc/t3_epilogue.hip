@@ -54,7 +54,7 @@ __device__ __forceinline__ void decode_cube(int m, int& A, int& B, int& C) {
 }
 
 template <class TP, bool BW>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   constexpr int T = EPI_T, T3 = T * T * T;
   const EpiTriple tr = triples[blockIdx.y];
@@ -65,13 +65,13 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   const int tid = threadIdx.x;
 
   __shared__ TP sc[6][T3];                 // w0 sub-cubes, later G
-  __shared__ double mn[2][6][6][T * T];    // [M/N][rho][mu-pair][8x8]; later H
-  __shared__ double vec[2][3][3][T];       // [t1/fvo][occ pos][tile pos][8]
+  __shared__ TP sv[6][T3];                 // v0 sub-cubes, later H
   __shared__ double rbuf[2][T3];           // per canonical point: Q and R contributions
   __shared__ double red[256];
 
-  // --- load the six permuted sub-cubes of w0 ---
+  // --- load the six permuted sub-cubes of w0 and v0 ---
   const TP* w0 = reinterpret_cast<const TP*>(tr.w0);
+  const TP* v0 = reinterpret_cast<const TP*>(tr.v0);
   for (int e = tid; e < 6 * T3; e += 256) {
     const int p = e / T3, r = e % T3;
     const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
@@ -84,30 +84,9 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       case 4: q0 = TC; q1 = TA; q2 = TB; break;
       default: q0 = TC; q1 = TB; q2 = TA; break;
     }
-    const long long g0 = q0 * T + x0, g1 = q1 * T + x1, g2 = q2 * T + x2;
-    sc[p][r] = w0[(g0 * V + g1) * V + g2];
-  }
-  // --- M_{pq}[x][y] = ovov[p,x,q,y], N_{pq}[x][y] = t2[q,p,y,x] on the tile pairs,
-  //     from the triple's compact copy mn6[rho][M/N][V][V] (rho = ordered pair (t_rho0, t_rho1)) ---
-  {
-    const long long VV = (long long)V * V;
-    for (int e = tid; e < 2 * 36 * T * T; e += 256) {
-      const int which = e / (36 * T * T);
-      const int r = e % (36 * T * T);
-      const int rho = r / (6 * T * T), mu = (r / (T * T)) % 6, xy = r % (T * T);
-      const int x = sel3(c_perm[mu][0], TA, TB, TC) * T + xy / T;
-      const int y = sel3(c_perm[mu][1], TA, TB, TC) * T + xy % T;
-      mn[which][rho][mu][xy] = tr.mn6[(2 * rho + which) * VV + (long long)x * V + y];
-    }
-  }
-  for (int e = tid; e < 2 * 3 * 3 * T; e += 256) {
-    const int which = e / (9 * T), r = e % (9 * T);
-    const int op = r / (3 * T), tp = (r / T) % 3, z = sel3(tp, TA, TB, TC) * T + r % T;
-    const int rr = sel3(op, tr.i, tr.j, tr.k);
-    double val = 0.0;
-    if (z < v) val = which == 0 ? args.t1[(long long)rr * v + z]
-                                : args.fock[(long long)(o + z) * (o + v) + rr];
-    vec[which][op][tp][r % T] = val;
+    const long long g = (((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2);
+    sc[p][r] = w0[g];
+    sv[p][r] = v0[g];
   }
   __syncthreads();
 
@@ -130,18 +109,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     for (int p = 0; p < 6; p++) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
       W[p] = (double)sc[p][(x0 * T + x1) * T + x2];
-      // v0 at the orbit point pi(s): 1/2 sum_rho (M t1 + N fvo) with mu = pi o rho
-      double vv = 0.0;
-#pragma unroll
-      for (int rho = 0; rho < 6; rho++) {
-        const int mu0 = c_perm[p][c_perm[rho][0]], mu1 = c_perm[p][c_perm[rho][1]],
-                  mu2 = c_perm[p][c_perm[rho][2]];
-        const int mp = perm_index(mu0, mu1);
-        const int xy = al[mu0] * T + al[mu1];
-        vv += mn[0][rho][mp][xy] * vec[0][c_perm[rho][2]][mu2][al[mu2]]
-            + mn[1][rho][mp][xy] * vec[1][c_perm[rho][2]][mu2][al[mu2]];
-      }
-      U[p] = W[p] + 0.5 * vv;
+      U[p] = W[p] + (double)sv[p][(x0 * T + x1) * T + x2];
     }
     double sU[2] = {0.0, 0.0}, sW[2] = {0.0, 0.0};
 #pragma unroll
@@ -198,8 +166,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     part[2 + tid] = s;
   }
 
-  // G and H into LDS (G over the w0 sub-cubes, H over the M/N blocks)
-  double* hl = &mn[0][0][0][0];
+  // G and H into LDS, over the w0 and v0 sub-cubes
   npt = 0;
   for (int pt = tid; pt < T3; pt += 256, npt++) {
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
@@ -209,7 +176,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
       const int r = (x0 * T + x1) * T + x2;
       sc[p][r] = (TP)gval[npt][p];
-      hl[p * T3 + r] = hval[npt][p];
+      sv[p][r] = (TP)hval[npt][p];
     }
   }
   __syncthreads();
@@ -245,7 +212,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       if (!write) continue;
       const long long off = ((b0 + x[0]) * V + (b1 + x[1])) * V + (b2 + x[2]);
       G[off] = sc[p][r];
-      H[off] = (TP)hl[p * T3 + r];
+      H[off] = sv[p][r];
     }
   }
 }

@@ -23,7 +23,7 @@ enum LoadMode : int { LOAD_ROWCONT = 0, LOAD_KCONT = 1, LOAD_SCALAR = 2 };
 
 template <int PREC> struct PrecTraits;
 template <> struct PrecTraits<PREC_F64> { using T = double; using Acc = double; static constexpr int BK = 16; static constexpr int VEC = 2; };
-template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; static constexpr int BK = 16; static constexpr int VEC = 4; };
+template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; static constexpr int BK = 32; static constexpr int VEC = 4; };
 template <> struct PrecTraits<PREC_BF16X3> { using T = float; using Acc = float; static constexpr int BK = 32; static constexpr int VEC = 4; };
 
 template <class T, int VEC> struct VecT;
@@ -211,7 +211,15 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   constexpr int FM = WM / 16, FN = WN / 16;
   static_assert(FM >= 1 && FN >= 1, "wave tile");
 
-  const TileCoord tc = find_tile(probs, nprob, blockIdx.x);
+  // Blocks are dealt round-robin to the 8 XCDs (each with its own L2); map
+  // them so that every XCD works through a contiguous range of logical tiles,
+  // which share operand rows.
+  long long bid = blockIdx.x;
+  {
+    const long long nb = gridDim.x, q = nb / 8;
+    if (bid < 8 * q) bid = (bid % 8) * q + bid / 8;
+  }
+  const TileCoord tc = find_tile(probs, nprob, bid);
   const GemmProb& P = probs[tc.p];
   const int m0 = tc.tm * BM, n0 = tc.tn * BN;
   const int rows_a = min(BM, P.M - m0), rows_b = min(BN, P.N - n0);
@@ -395,7 +403,7 @@ using namespace t3;
 // tile config ids: 0 -> 64x64 (2x2 waves), 1 -> 128x128 (2x2), 2 -> 64x16 (4x1)
 extern "C" int t3_gemm_tile_m(int cfg) { return cfg == 1 ? 128 : 64; }
 extern "C" int t3_gemm_tile_n(int cfg) { return cfg == 1 ? 128 : (cfg == 2 ? 16 : 64); }
-extern "C" int t3_gemm_chunk(int prec) { return prec == PREC_BF16X3 ? 32 : 16; }
+extern "C" int t3_gemm_chunk(int prec) { return prec == PREC_F64 ? 16 : 32; }
 
 extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob,
                                      const GemmSeg* segs, long long nblocks, hipStream_t stream) {

@@ -96,8 +96,8 @@ __global__ void __launch_bounds__(256)
 vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ t1,
                const double* __restrict__ fock, const double* __restrict__ ovov,
                const double* __restrict__ t2) {
-  const VpTriple tr = trs[blockIdx.z];
-  const int task = blockIdx.y;          // 0..5
+  const VpTriple tr = trs[blockIdx.y / 6];
+  const int task = blockIdx.y % 6;      // 0..5
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
   if (e >= (long long)V * V) return;
   const int u = (int)(e / V), w = (int)(e % V);   // output (row, col)
@@ -172,9 +172,9 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
 __global__ void __launch_bounds__(256)
 vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
           const double* __restrict__ ovov, const double* __restrict__ t2) {
-  const VpTriple tr = trs[blockIdx.z];
-  const int task = blockIdx.y;
-  if (task >= ntask) return;
+  const VpTriple tr = trs[blockIdx.y / 30];
+  const int task = blockIdx.y % 30;
+  if (ntask < 0 && task >= 12 && task < 18) return;   // forward: no CM/CN
   const long long VV = (long long)V * V;
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
   if (e >= VV) return;
@@ -186,13 +186,15 @@ vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
   auto N = [&](int p, int q, int a, int b) -> double {
     return (a < v && b < v) ? t2[(((long long)q * o + p) * v + b) * v + a] : 0.0;
   };
-  if (task < 12) {
-    const int rho = task / 2, which = task % 2;
+  if (task < 12 || task >= 18) {
+    const bool tr_out = task >= 18;
+    const int tk = tr_out ? task - 18 : task;
+    const int rho = tk / 2, which = tk % 2;
     // ordered pair (rho0, rho1) of the permutation rho in the order (01),(02),(10),(12),(20),(21)
     const int r0 = rho / 2, r1 = (rho % 2 == 0) ? (r0 == 0 ? 1 : 0) : (r0 == 2 ? 1 : 2);
     const int p = occ[r0], q = occ[r1];
-    tr.out[(VP_MN6 + task) * VV + e] = which == 0 ? M(p, q, x, y) : N(p, q, x, y);
-    (void)rho;
+    if (!tr_out) tr.out[(VP_MN6 + tk) * VV + e] = which == 0 ? M(p, q, x, y) : N(p, q, x, y);
+    else tr.out[(VP_MN6T + tk) * VV + e] = which == 0 ? M(p, q, y, x) : N(p, q, y, x);
     return;
   }
   const int d = (task - 12) / 2, which = (task - 12) % 2;
@@ -201,6 +203,47 @@ vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
   const int p1 = occ[e1], p2 = occ[e2];
   tr.out[(VP_CM + task - 12) * VV + e] =
       which == 0 ? M(p1, p2, x, y) + M(p2, p1, y, x) : N(p1, p2, x, y) + N(p2, p1, y, x);
+}
+
+// The V term of a triple as a V^3 cube (computed in FP64, stored as TP):
+//   v0[s] = 1/2 sum_rho ( M_rho[s_rho0][s_rho1] t1[t_rho2, s_rho2]
+//                       + N_rho[s_rho0][s_rho1] fock[o + s_rho2, t_rho2] )
+// with M_rho = M_{t_rho0 t_rho1} (ccsd_t.c:149-150 summed over the six orderings).
+// Thread e = (a V + b) V + c; the mn6 / mn6T copies keep every read either
+// broadcast or contiguous in c.
+template <class TP>
+__global__ void __launch_bounds__(256)
+v0_build(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ t1,
+         const double* __restrict__ fock) {
+  const VpTriple tr = trs[blockIdx.y];
+  const long long VV = (long long)V * V;
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= VV * V) return;
+  const int s[3] = {(int)(e / VV), (int)((e / V) % V), (int)(e % V)};
+  const int t[3] = {tr.i, tr.j, tr.k};
+  const int n = o + v;
+  const double* mn6 = tr.out + VP_MN6 * VV;
+  const double* mn6T = tr.out + VP_MN6T * VV;
+  constexpr int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  double acc = 0.0;
+#pragma unroll
+  for (int rho = 0; rho < 6; rho++) {
+    const int z = s[P[rho][2]];
+    if (z >= v) continue;
+    const int r = t[P[rho][2]];
+    double m, nn;
+    if (P[rho][0] == 2) {   // first index contiguous: read the transposed copy
+      const long long idx = (long long)s[P[rho][1]] * V + s[P[rho][0]];
+      m = mn6T[(2 * rho) * VV + idx];
+      nn = mn6T[(2 * rho + 1) * VV + idx];
+    } else {
+      const long long idx = (long long)s[P[rho][0]] * V + s[P[rho][1]];
+      m = mn6[(2 * rho) * VV + idx];
+      nn = mn6[(2 * rho + 1) * VV + idx];
+    }
+    acc += m * t1[(long long)r * v + z] + nn * fock[(long long)(o + z) * n + r];
+  }
+  reinterpret_cast<TP*>(tr.v0)[e] = (TP)(0.5 * acc);
 }
 
 // Scatter of the V-term adjoints of one batch into the FP64 accumulators.
@@ -380,7 +423,7 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
                                       double* fvo_acc, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
-  dim3 g1((unsigned)((VV + 255) / 256), 6, ntriple);
+  dim3 g1((unsigned)((VV + 255) / 256), 6 * ntriple);
   if (tp_is_f64) hipLaunchKernelGGL(vpart_contract<double>, g1, dim3(256), 0, s, trs, o, v, V, t1, fock, ovov, t2);
   else hipLaunchKernelGGL(vpart_contract<float>, g1, dim3(256), 0, s, trs, o, v, V, t1, fock, ovov, t2);
   const long long nx = VV > (long long)o * V ? VV : (long long)o * V;
@@ -394,9 +437,19 @@ extern "C" hipError_t t3_launch_vmn(const VpTriple* trs, int ntriple, int o, int
                                     const double* ovov, const double* t2, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
-  const int ntask = with_cm ? 18 : 12;
-  dim3 g((unsigned)((VV + 255) / 256), ntask, ntriple);
-  hipLaunchKernelGGL(vmn_build, g, dim3(256), 0, s, trs, o, v, V, ntask, ovov, t2);
+  // tasks: 0..11 mn6, 12..17 CM/CN (backward only), 18..29 mn6 transposed
+  dim3 g((unsigned)((VV + 255) / 256), 30 * ntriple);
+  hipLaunchKernelGGL(vmn_build, g, dim3(256), 0, s, trs, o, v, V, with_cm ? 30 : -1, ovov, t2);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_launch_v0(int tp_is_f64, const VpTriple* trs, int ntriple, int o, int v, int V,
+                                   const double* t1, const double* fock, hipStream_t s) {
+  if (ntriple <= 0) return hipSuccess;
+  const long long n = (long long)V * V * V;
+  dim3 g((unsigned)((n + 255) / 256), ntriple);
+  if (tp_is_f64) hipLaunchKernelGGL(v0_build<double>, g, dim3(256), 0, s, trs, o, v, V, t1, fock);
+  else hipLaunchKernelGGL(v0_build<float>, g, dim3(256), 0, s, trs, o, v, V, t1, fock);
   return hipGetLastError();
 }
 
