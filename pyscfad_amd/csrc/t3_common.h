@@ -73,7 +73,7 @@ struct EpiTriple {
   void* G;            // V^3 (backward)
   void* H;            // V^3 (backward)
   double* part;       // [ncube][EPI_PSTRIDE]
-  const void* v0;     // V^3 cube of the V term v0 (built by v0_build)
+  const double* cm;   // [d][CM/CN][V][V] pair matrices of the V term (vcm_build)
 };
 
 struct EpiArgs {
@@ -83,10 +83,8 @@ struct EpiArgs {
   double et_bar;
   const double* eps_occ;   // (o)
   const double* eps_vir;   // (V), padded entries large and finite
-  const double* ovov;      // (o, v, o, v) reference layout
-  const double* t2;        // (o, o, v, v) reference layout
-  const double* t1;        // (o, v)
-  const double* fock;      // (n, n)
+  const double* t1;        // (o, v) reference layout
+  const double* fvoP;      // (o, V): fvoP[r][z] = fock[o+z, r], zero padded
 };
 
 // Per-batch reduction of the epilogue partials (one block, fixed order):
@@ -104,10 +102,9 @@ struct EpiReduceArgs {
 struct VpTriple {
   int i, j, k;
   const void* H;
-  double* out;     // 42 V^2: [0,12) contractions, [12,24) mn6, [24,30) CM_d / CN_d, [30,42) mn6 transposed
-  void* v0;        // V^3 cube of the V term (TP)
+  double* out;     // 18 V^2: [0,12) contractions, [12,18) CM_d / CN_d
 };
-constexpr int VP_OUT = 0, VP_MN6 = 12, VP_CM = 24, VP_MN6T = 30, VP_TOTAL = 42;   // offsets in V^2 units
+constexpr int VP_OUT = 0, VP_CM = 12, VP_TOTAL = 18;   // offsets in V^2 units
 
 struct VpSlab {
   int p, q;

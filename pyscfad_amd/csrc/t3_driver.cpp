@@ -41,16 +41,14 @@ hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, const EpiTripl
                               hipStream_t stream);
 hipError_t t3_launch_epi_reduce(const EpiReduceArgs* a, hipStream_t stream);
 hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void* ovooT, double* eps_occ,
-                          double* eps_vir, const double* ovvv, int packed, const double* t2, const double* ovoo,
-                          const double* mo_energy, int o, int v, int Op, int V, hipStream_t s);
+                          double* eps_vir, double* fvoP, const double* ovvv, int packed, const double* t2,
+                          const double* ovoo, const double* mo_energy, const double* fock, int o, int v, int Op,
+                          int V, hipStream_t s);
 hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs, int nslab,
-                           const VpContrib* contribs, int o, int v, int V, const double* t1, const double* fock,
-                           const double* ovov, const double* t2, double* ovov_acc, double* t2v_acc,
-                           double* t1_acc, double* fvo_acc, hipStream_t s);
-hipError_t t3_launch_vmn(const VpTriple* trs, int ntriple, int o, int v, int V, int with_cm, const double* ovov,
+                           const VpContrib* contribs, int o, int v, int V, const double* t1, const double* fvoP,
+                           double* ovov_acc, double* t2v_acc, double* t1_acc, double* fvo_acc, hipStream_t s);
+hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
                          const double* t2, hipStream_t s);
-hipError_t t3_launch_v0(int tp_is_f64, const VpTriple* trs, int ntriple, int o, int v, int V, const double* t1,
-                        const double* fock, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
                        const double* t2hacc, const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
@@ -113,10 +111,10 @@ struct Layout {
 };
 
 struct Regions {
-  size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, energy;
+  size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, fvoP, energy;
   size_t ovvv_acc, t2acc, t2hacc, t2vacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
   size_t slots, slot_bytes;
-  size_t slot_w0, slot_v0, slot_G, slot_H, slot_part, slot_vp;
+  size_t slot_w0, slot_G, slot_H, slot_part, slot_vp;
   size_t split, split_bytes;
   size_t desc, desc_bytes;
   size_t total;
@@ -139,6 +137,7 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   r.ovooT = L.take(o * o * V * Op * d.elt);
   r.eps_occ = L.take(o * 8);
   r.eps_vir = L.take(V * 8);
+  r.fvoP = L.take(o * V * 8);
   r.energy = L.take(8);
   if (backward) {
     r.ovvv_acc = L.take(o * V * V * V * 8);
@@ -153,7 +152,6 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   }
   Layout S;
   r.slot_w0 = S.take(V * V * V * d.elt);
-  r.slot_v0 = S.take(V * V * V * d.elt);
   if (backward) {
     r.slot_G = S.take(V * V * V * d.elt);
     r.slot_H = S.take(V * V * V * d.elt);
@@ -421,7 +419,7 @@ struct Ctx {
       e.G = bw ? slot((int)s, r.slot_G) : nullptr;
       e.H = bw ? slot((int)s, r.slot_H) : nullptr;
       e.part = reinterpret_cast<double*>(slot((int)s, r.slot_part));
-      e.v0 = slot((int)s, r.slot_v0);
+      e.cm = reinterpret_cast<const double*>(slot((int)s, r.slot_vp)) + (size_t)VP_CM * d.V * d.V;
       et.push_back(e);
     }
     // compact V-term operands of the batch (and the backward's CM/CN)
@@ -431,7 +429,6 @@ struct Ctx {
       t.i = bt[s].t[0]; t.j = bt[s].t[1]; t.k = bt[s].t[2];
       t.H = bw ? slot((int)s, r.slot_H) : nullptr;
       t.out = reinterpret_cast<double*>(slot((int)s, r.slot_vp));
-      t.v0 = slot((int)s, r.slot_v0);
       vt.push_back(t);
     }
     Launch M;
@@ -444,7 +441,7 @@ struct Ctx {
     a.nocc = d.o; a.nvir = d.v; a.V = d.V; a.ntile = d.ntile; a.ncube = d.ncube;
     a.backward = bw; a.et_bar = et_bar;
     a.eps_occ = dl(r.eps_occ, 0); a.eps_vir = dl(r.eps_vir, 0);
-    a.ovov = ovov; a.t2 = t2; a.t1 = t1; a.fock = fock;
+    a.t1 = t1; a.fvoP = dl(r.fvoP, 0);
     epi_args.push_back(a);
     Launch L;
     L.kind = L_EPI;
@@ -551,7 +548,7 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, 0, GEMM_ACCUM, 1, true);
+    emit_gemm(gb, d.Op <= 32 ? 3 : 0, GEMM_ACCUM, 1, true);
   }
 
   void emit_b4(const std::vector<Triple>& bt) {
@@ -578,7 +575,7 @@ struct Ctx {
         sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, 0, GEMM_ACCUM, 1, false);
+    emit_gemm(gb, d.Op <= 32 ? 4 : 0, GEMM_ACCUM, 1, false);
   }
 
   void emit_vpart(const std::vector<Triple>& bt) {
@@ -671,18 +668,14 @@ struct Ctx {
           break;
         }
         case L_VMN:
-          e = t3_launch_vmn(reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a, d.o, d.v, d.V, L.aux, ovov, t2,
-                            st);
-          if (e == hipSuccess)
-            e = t3_launch_v0(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a, d.o, d.v, d.V,
-                             t1, fock, st);
+          e = t3_launch_vcm(reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a, d.o, d.v, d.V, ovov, t2, st);
           break;
         case L_VPART:
           e = t3_launch_vpart(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a,
                               reinterpret_cast<const VpSlab*>(db + L.off_b), L.n_b,
-                              reinterpret_cast<const VpContrib*>(db + L.off_c), d.o, d.v, d.V, t1, fock, ovov,
-                              t2, at<double>(r.ovov_acc), at<double>(r.t2vacc), at<double>(r.t1_acc),
-                              at<double>(r.fvo_acc), st);
+                              reinterpret_cast<const VpContrib*>(db + L.off_c), d.o, d.v, d.V, t1,
+                              at<double>(r.fvoP), at<double>(r.ovov_acc), at<double>(r.t2vacc),
+                              at<double>(r.t1_acc), at<double>(r.fvo_acc), st);
           break;
       }
       if (e != hipSuccess) return e;
@@ -788,8 +781,8 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
     // rewritten before each reduction
   }
   e = t3_prep_inputs(c.d.tp_is_f64, c.ws + c.r.ovvvT, c.ws + c.r.t2T, c.ws + c.r.t2H, c.ws + c.r.ovooT,
-                     c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), ovvv, packed, t2, ovoo, mo_energy,
-                     c.d.o, c.d.v, c.d.Op, c.d.V, st);
+                     c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), c.at<double>(c.r.fvoP), ovvv, packed,
+                     t2, ovoo, mo_energy, fock, c.d.o, c.d.v, c.d.Op, c.d.V, st);
   if (e != hipSuccess) return hip_fail(e, "prep");
   e = c.execute(st);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");

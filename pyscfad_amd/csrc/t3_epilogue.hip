@@ -14,9 +14,15 @@
 // reference's a>=b>=c job loop (ccsd_t.c:6-72, lno_ccsd_t.c:6-29) taken in a
 // different order.
 //
+// The V term (ccsd_t.c:149-150 summed over the six orderings) is built here,
+// in FP64, from three pair matrices per triple:
+//     v0[x] = 1/2 sum_d ( CM_d[x_e1][x_e2] t1[t_d, x_d] + CN_d[x_e1][x_e2] fvo[x_d, t_d] )
+// with e1 < e2 the positions other than d and
+//     CM_d[x][y] = ovov[t_e1,x,t_e2,y] + ovov[t_e2,y,t_e1,x],
+//     CN_d[x][y] = t2[t_e2,t_e1,y,x] + t2[t_e1,t_e2,x,y]      (vcm_build).
+//
 // A block takes one unique tile triple TA >= TB >= TC (8 virtual indices per
-// tile), loads the six permuted 8^3 sub-cubes of w0 into LDS, builds v0 of the
-// orbit points from ovov, t2, t1 and fock (FP64), and computes in FP64:
+// tile) and computes in FP64:
 //   forward : the block's share of Q_ijk = sum f_abc sum_pi W Z / D
 //   backward: G = s (Z + C W) / D and H = s (C W) / D at every orbit point
 //             (the adjoints of w0 and v0, s = 2 f_ijk et_bar, C the r3 matrix,
@@ -32,13 +38,12 @@ namespace t3 {
 // constexpr so that, inside fully unrolled loops, every lookup is a constant.
 constexpr int c_perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 constexpr int c_parity[6] = {0, 1, 1, 0, 0, 1};
+// the two positions other than d, in increasing order
+constexpr int c_oth[3][2] = {{1, 2}, {0, 2}, {0, 1}};
 
 __device__ __forceinline__ int sel3(int i, int a, int b, int c) { return i == 0 ? a : (i == 1 ? b : c); }
 
-
-
-
-__device__ __forceinline__ int perm_index(int p0, int p1) {
+__host__ __device__ constexpr int perm_index(int p0, int p1) {
   // index of the permutation (p0, p1, 3-p0-p1) in c_perm order
   return 2 * p0 + (p1 > 3 - p0 - p1 ? 1 : 0);
 }
@@ -57,21 +62,24 @@ template <class TP, bool BW>
 __global__ void __launch_bounds__(256)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   constexpr int T = EPI_T, T3 = T * T * T;
+  constexpr int NCM = 3 * 6 * 2 * T * T;                   // CM/CN blocks
+  constexpr int NH = (6 * T3 * (int)sizeof(TP) + 7) / 8;   // H sub-cubes, in doubles
+  constexpr int NU = NCM > NH ? NCM : NH;
   const EpiTriple tr = triples[blockIdx.y];
   const int cube = blockIdx.x;
   int TA, TB, TC;
   decode_cube(cube, TA, TB, TC);
-  const int V = args.V, v = args.nvir, o = args.nocc;
+  const int V = args.V, v = args.nvir;
   const int tid = threadIdx.x;
 
-  __shared__ TP sc[6][T3];                 // w0 sub-cubes, later G
-  __shared__ TP sv[6][T3];                 // v0 sub-cubes, later H
-  __shared__ double rbuf[2][T3];           // per canonical point: Q and R contributions
+  __shared__ TP sc[6][T3];             // w0 sub-cubes, later G
+  __shared__ double cmh[NU];           // CM/CN tile-pair blocks [d][pair][M/N][8x8], later H
+  __shared__ double vec[2][3][3][T];   // [t1/fvo][d][tile pos][8]
+  __shared__ double rbuf[2][T3];       // per canonical point: Q and mo_energy contributions
   __shared__ double red[256];
 
-  // --- load the six permuted sub-cubes of w0 and v0 ---
+  // --- the six permuted sub-cubes of w0 ---
   const TP* w0 = reinterpret_cast<const TP*>(tr.w0);
-  const TP* v0 = reinterpret_cast<const TP*>(tr.v0);
   for (int e = tid; e < 6 * T3; e += 256) {
     const int p = e / T3, r = e % T3;
     const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
@@ -84,9 +92,29 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       case 4: q0 = TC; q1 = TA; q2 = TB; break;
       default: q0 = TC; q1 = TB; q2 = TA; break;
     }
-    const long long g = (((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2);
-    sc[p][r] = w0[g];
-    sv[p][r] = v0[g];
+    sc[p][r] = w0[(((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2)];
+  }
+  // --- CM_d / CN_d on the ordered tile pairs: block [d][pair mu][which][8x8] ---
+  {
+    const long long VV = (long long)V * V;
+    for (int e = tid; e < NCM; e += 256) {
+      const int xy = e % (T * T);
+      const int which = (e / (T * T)) % 2;
+      const int mu = (e / (2 * T * T)) % 6;
+      const int d = e / (12 * T * T);
+      const int x = sel3(c_perm[mu][0], TA, TB, TC) * T + xy / T;
+      const int y = sel3(c_perm[mu][1], TA, TB, TC) * T + xy % T;
+      cmh[e] = tr.cm[(2 * d + which) * VV + (long long)x * V + y];
+    }
+  }
+  for (int e = tid; e < 2 * 3 * 3 * T; e += 256) {
+    const int which = e / (9 * T), r = e % (9 * T);
+    const int d = r / (3 * T), tp = (r / T) % 3, z = sel3(tp, TA, TB, TC) * T + r % T;
+    const int td = sel3(d, tr.i, tr.j, tr.k);
+    double val = 0.0;
+    if (which == 0) val = z < v ? args.t1[(long long)td * v + z] : 0.0;
+    else val = args.fvoP[(long long)td * V + z];
+    vec[which][d][tp][r % T] = val;
   }
   __syncthreads();
 
@@ -109,7 +137,15 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     for (int p = 0; p < 6; p++) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
       W[p] = (double)sc[p][(x0 * T + x1) * T + x2];
-      U[p] = W[p] + (double)sv[p][(x0 * T + x1) * T + x2];
+      // v0 at the orbit point pi(s): position m of the point holds s_{pi_m}
+      double vv = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const int m1 = c_perm[p][c_oth[d][0]], m2 = c_perm[p][c_oth[d][1]], md = c_perm[p][d];
+        const int blk = ((d * 6 + perm_index(m1, m2)) * 2) * T * T + al[m1] * T + al[m2];
+        vv += cmh[blk] * vec[0][d][md][al[md]] + cmh[blk + T * T] * vec[1][d][md][al[md]];
+      }
+      U[p] = W[p] + 0.5 * vv;
     }
     double sU[2] = {0.0, 0.0}, sW[2] = {0.0, 0.0};
 #pragma unroll
@@ -166,7 +202,8 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     part[2 + tid] = s;
   }
 
-  // G and H into LDS, over the w0 and v0 sub-cubes
+  // G and H into LDS, over the w0 sub-cubes and the CM/CN area
+  TP* hl = reinterpret_cast<TP*>(cmh);
   npt = 0;
   for (int pt = tid; pt < T3; pt += 256, npt++) {
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
@@ -176,7 +213,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
       const int r = (x0 * T + x1) * T + x2;
       sc[p][r] = (TP)gval[npt][p];
-      sv[p][r] = (TP)hval[npt][p];
+      hl[p * T3 + r] = (TP)hval[npt][p];
     }
   }
   __syncthreads();
@@ -212,7 +249,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       if (!write) continue;
       const long long off = ((b0 + x[0]) * V + (b1 + x[1])) * V + (b2 + x[2]);
       G[off] = sc[p][r];
-      H[off] = sv[p][r];
+      H[off] = hl[p * T3 + r];
     }
   }
 }
@@ -221,68 +258,83 @@ __device__ __forceinline__ int cube_id(int A, int B, int C) {
   return A * (A + 1) * (A + 2) / 6 + B * (B + 1) / 2 + C;
 }
 
+// Per-batch reduction of the epilogue partials, every sum in a fixed order.
+// Block `ntile` : E += sum_t 2 f_t sum_cube Q;  eps_bar[occ of t] -= sum_cube Rsum.
+// Block X < ntile: eps_bar[o + 8X + l] += the vir partials of every cube that
+//                  contains tile X (backward only).
 __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
   __shared__ double red[256];
+  __shared__ double red8[256][EPI_T + 1];
   const int tid = threadIdx.x;
-  if (blockIdx.x == 0) {
-    // energy
-    double s = 0.0;
-    for (int t = 0; t < a.ntriple; t++) {
-      const EpiTriple& tr = a.triples[t];
-      double q = 0.0;
-      for (int c = tid; c < a.ncube; c += 256) q += tr.part[(long long)c * EPI_PSTRIDE];
-      s += 2.0 * tr.fo * q;
-    }
-    red[tid] = s;
+  auto tree = [&](double x) -> double {
+    red[tid] = x;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
       if (tid < w) red[tid] += red[tid + w];
       __syncthreads();
     }
-    if (tid == 0) *a.energy += red[0];
-    if (!a.backward) return;
+    const double r = red[0];
     __syncthreads();
-    // occupied adjoint
+    return r;
+  };
+  if ((int)blockIdx.x == a.ntile) {
+    double e = 0.0;
+    for (int t = 0; t < a.ntriple; t++) {
+      const EpiTriple& tr = a.triples[t];
+      double q = 0.0;
+      for (int c = tid; c < a.ncube; c += 256) q += tr.part[(long long)c * EPI_PSTRIDE];
+      e += 2.0 * tr.fo * tree(q);
+    }
+    if (tid == 0) *a.energy += e;
+    if (!a.backward) return;
     for (int t = 0; t < a.ntriple; t++) {
       const EpiTriple& tr = a.triples[t];
       double r = 0.0;
       for (int c = tid; c < a.ncube; c += 256) r += tr.part[(long long)c * EPI_PSTRIDE + 1];
-      red[tid] = r;
-      __syncthreads();
-      for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) red[tid] += red[tid + w];
-        __syncthreads();
-      }
+      r = tree(r);
       if (tid == 0) {
-        a.eps_bar[tr.i] -= red[0];
-        a.eps_bar[tr.j] -= red[0];
-        a.eps_bar[tr.k] -= red[0];
+        a.eps_bar[tr.i] -= r;
+        a.eps_bar[tr.j] -= r;
+        a.eps_bar[tr.k] -= r;
       }
-      __syncthreads();
     }
     return;
   }
   if (!a.backward) return;
-  // virtual adjoint: one thread per padded virtual index
-  const int x = (blockIdx.x - 1) * 256 + tid;
-  if (x >= a.V) return;
-  const int TX = x / EPI_T, lx = x % EPI_T;
-  double s = 0.0;
+  const int TX = blockIdx.x;
+  const int npair = a.ntile * (a.ntile + 1) / 2;
+  double total[EPI_T];
+  for (int l = 0; l < EPI_T; l++) total[l] = 0.0;
   for (int t = 0; t < a.ntriple; t++) {
     const double* part = a.triples[t].part;
-    for (int X = 0; X < a.ntile; X++)
-      for (int Y = 0; Y <= X; Y++) {
-        int u0 = TX, u1 = X, u2 = Y;   // sort descending
-        if (u1 > u0) { int tmp = u0; u0 = u1; u1 = tmp; }
-        if (u2 > u1) { int tmp = u1; u1 = u2; u2 = tmp; }
-        if (u1 > u0) { int tmp = u0; u0 = u1; u1 = tmp; }
-        const double* pc = part + (long long)cube_id(u0, u1, u2) * EPI_PSTRIDE;
-        if (u0 == TX) s += pc[2 + lx];
-        if (u1 == TX) s += pc[2 + EPI_T + lx];
-        if (u2 == TX) s += pc[2 + 2 * EPI_T + lx];
+    double acc[EPI_T];
+    for (int l = 0; l < EPI_T; l++) acc[l] = 0.0;
+    for (int pr = tid; pr < npair; pr += 256) {
+      int X = 0;
+      while ((X + 1) * (X + 2) / 2 <= pr) X++;
+      const int Y = pr - X * (X + 1) / 2;
+      int u0 = TX, u1 = X, u2 = Y;   // sort descending
+      if (u1 > u0) { int tmp = u0; u0 = u1; u1 = tmp; }
+      if (u2 > u1) { int tmp = u1; u1 = u2; u2 = tmp; }
+      if (u1 > u0) { int tmp = u0; u0 = u1; u1 = tmp; }
+      const double* pc = part + (long long)cube_id(u0, u1, u2) * EPI_PSTRIDE;
+      for (int l = 0; l < EPI_T; l++) {
+        if (u0 == TX) acc[l] += pc[2 + l];
+        if (u1 == TX) acc[l] += pc[2 + EPI_T + l];
+        if (u2 == TX) acc[l] += pc[2 + 2 * EPI_T + l];
       }
+    }
+    for (int l = 0; l < EPI_T; l++) red8[tid][l] = acc[l];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w)
+        for (int l = 0; l < EPI_T; l++) red8[tid][l] += red8[tid + w][l];
+      __syncthreads();
+    }
+    for (int l = 0; l < EPI_T; l++) total[l] += red8[0][l];
+    __syncthreads();
   }
-  a.eps_bar[a.nocc + x] += s;
+  if (tid < EPI_T) a.eps_bar[a.nocc + TX * EPI_T + tid] += total[tid];
 }
 
 }  // namespace t3
@@ -304,7 +356,7 @@ extern "C" hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, con
 }
 
 extern "C" hipError_t t3_launch_epi_reduce(const EpiReduceArgs* a, hipStream_t stream) {
-  const int nb = a->backward ? 1 + (a->V + 255) / 256 : 1;
+  const int nb = a->ntile + 1;
   hipLaunchKernelGGL(epi_reduce, dim3(nb), dim3(256), 0, stream, *a);
   return hipGetLastError();
 }

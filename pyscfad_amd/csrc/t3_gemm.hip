@@ -400,9 +400,12 @@ reduce_partials(const ReduceProb* __restrict__ probs, int nprob, long long total
 
 using namespace t3;
 
-// tile config ids: 0 -> 64x64 (2x2 waves), 1 -> 128x128 (2x2), 2 -> 64x16 (4x1)
-extern "C" int t3_gemm_tile_m(int cfg) { return cfg == 1 ? 128 : 64; }
-extern "C" int t3_gemm_tile_n(int cfg) { return cfg == 1 ? 128 : (cfg == 2 ? 16 : 64); }
+// tile config ids: 0 -> 64x64 (2x2 waves), 1 -> 128x128 (2x2), 2 -> 64x16 (4x1),
+//                  3 -> 64x32 (2x2), 4 -> 32x64 (2x2)
+extern "C" int t3_gemm_tile_m(int cfg) { return cfg == 1 ? 128 : (cfg == 4 ? 32 : 64); }
+extern "C" int t3_gemm_tile_n(int cfg) {
+  return cfg == 1 ? 128 : (cfg == 2 ? 16 : (cfg == 3 ? 32 : 64));
+}
 extern "C" int t3_gemm_chunk(int prec) { return prec == PREC_F64 ? 16 : 32; }
 
 extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob,
@@ -412,6 +415,8 @@ extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, i
 #define T3_DISPATCH(PR)                                                                                    \
   if (cfg == 0) hipLaunchKernelGGL((gemm_grouped<PR, 64, 64, 2>), grid, block, 0, stream, probs, nprob, segs); \
   else if (cfg == 1) hipLaunchKernelGGL((gemm_grouped<PR, 128, 128, 2>), grid, block, 0, stream, probs, nprob, segs); \
+  else if (cfg == 3) hipLaunchKernelGGL((gemm_grouped<PR, 64, 32, 2>), grid, block, 0, stream, probs, nprob, segs); \
+  else if (cfg == 4) hipLaunchKernelGGL((gemm_grouped<PR, 32, 64, 2>), grid, block, 0, stream, probs, nprob, segs); \
   else hipLaunchKernelGGL((gemm_grouped<PR, 64, 16, 4>), grid, block, 0, stream, probs, nprob, segs);
   if (prec == PREC_F64) { T3_DISPATCH(PREC_F64) }
   else if (prec == PREC_F32) { T3_DISPATCH(PREC_F32) }

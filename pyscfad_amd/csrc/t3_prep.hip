@@ -74,8 +74,13 @@ __global__ void prep_ovoo(TP* __restrict__ out, const double* __restrict__ ovoo,
 }
 
 __global__ void prep_eps(double* __restrict__ eps_occ, double* __restrict__ eps_vir,
-                         const double* __restrict__ mo_energy, int o, int v, int V) {
+                         double* __restrict__ fvoP, const double* __restrict__ mo_energy,
+                         const double* __restrict__ fock, int o, int v, int V) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < o * V) {
+    const int r = e / V, z = e % V;
+    fvoP[e] = z < v ? fock[(long long)(o + z) * (o + v) + r] : 0.0;
+  }
   if (e < o) eps_occ[e] = mo_energy[e];
   // padded virtuals get a large orbital energy: every padded term has w0 = v0 = 0,
   // this only keeps its denominator finite and away from zero
@@ -91,159 +96,115 @@ __global__ void prep_eps(double* __restrict__ eps_occ, double* __restrict__ eps_
 //   M_{pq}[x][y] = ovov[p,x,q,y]  and  N_{pq}[x][y] = t2[q,p,y,x].
 // Output per triple: 12 V x V FP64 arrays: [S0 F0 S1 F1 S2 F2 PM0 PN0 PM1 PN1 PM2 PN2].
 
+// Three passes over H per triple, one per contracted position, each with
+// coalesced reads of H:
+//   pass 0 (contract a): thread per (b,c), loop a      -> S0, F0
+//   pass 1 (contract b): thread per (a,c), loop b      -> S1, F1, PM2, PN2
+//   pass 2 (contract c): wave per (a,b), lanes over c  -> S2, F2, PM1, PN1, PM0, PN0
+// PM_d / PN_d are partial sums (over the position that is neither d nor the
+// row) reduced in vpart_scatter.  blockIdx.y = npass * triple + (pass - pass0).
 template <class TP>
 __global__ void __launch_bounds__(256)
 vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ t1,
-               const double* __restrict__ fock, const double* __restrict__ ovov,
-               const double* __restrict__ t2) {
-  const VpTriple tr = trs[blockIdx.y / 6];
-  const int task = blockIdx.y % 6;      // 0..5
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (long long)V * V) return;
-  const int u = (int)(e / V), w = (int)(e % V);   // output (row, col)
+               const double* __restrict__ fvoP, int npass, int pass0) {
+  const VpTriple tr = trs[blockIdx.y / npass];
+  const int pass = pass0 + (int)(blockIdx.y % npass);
   const TP* H = reinterpret_cast<const TP*>(tr.H);
   const int occ[3] = {tr.i, tr.j, tr.k};
   const long long VV = (long long)V * V;
-  const int n = o + v;
-  (void)ovov;
-  (void)t2;
-  if (task < 3) {
-    const int d = task;
-    // H index with s_d = x and the other two = (u, w) in increasing position order
-    double s = 0.0, f = 0.0;
-    const int td = occ[d];
-    for (int x = 0; x < v; x++) {
-      long long h;
-      if (d == 0) h = (long long)x * VV + (long long)u * V + w;
-      else if (d == 1) h = (long long)u * VV + (long long)x * V + w;
-      else h = (long long)u * VV + (long long)w * V + x;
-      const double hv = (double)H[h];
-      s += hv * t1[(long long)td * v + x];
-      f += hv * fock[(long long)(o + x) * n + td];
+  double* out = tr.out;
+  if (pass < 2) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= VV) return;
+    const int u = (int)(e / V), w = (int)(e % V);
+    const int td = occ[pass];
+    double s = 0.0, f = 0.0, pm = 0.0, pn = 0.0;
+    if (pass == 0) {        // e = (b, c)
+      for (int x = 0; x < v; x++) {
+        const double hv = (double)H[(long long)x * VV + e];
+        s += hv * t1[(long long)td * v + x];
+        f += hv * fvoP[(long long)td * V + x];
+      }
+      out[0 * VV + e] = s;
+      out[1 * VV + e] = f;
+    } else {                // e = (a, c)
+      const double* CM2 = out + (VP_CM + 4) * VV + (long long)u * V;
+      const double* CN2 = out + (VP_CM + 5) * VV + (long long)u * V;
+      for (int x = 0; x < v; x++) {
+        const double hv = (double)H[(long long)u * VV + (long long)x * V + w];
+        s += hv * t1[(long long)td * v + x];
+        f += hv * fvoP[(long long)td * V + x];
+        pm += hv * CM2[x];
+        pn += hv * CN2[x];
+      }
+      out[2 * VV + e] = s;
+      out[3 * VV + e] = f;
+      out[(6 + 4) * VV + e] = pm;   // PM2[a][c]
+      out[(7 + 4) * VV + e] = pn;   // PN2[a][c]
     }
-    tr.out[(2 * d) * VV + e] = s;
-    tr.out[(2 * d + 1) * VV + e] = f;
     return;
   }
-  // partial sums of the two-axis contractions
-  const int d = task - 3;
-  int e1, e2;
-  if (d == 0) { e1 = 1; e2 = 2; } else if (d == 1) { e1 = 0; e2 = 2; } else { e1 = 0; e2 = 1; }
-  const double* CM = tr.out + (VP_CM + 2 * d) * VV;
-  const double* CN = tr.out + (VP_CM + 2 * d + 1) * VV;
-  auto cm = [&](int x, int y, double& mval, double& nval) {
-    mval = CM[(long long)x * V + y];
-    nval = CN[(long long)x * V + y];
-  };
-  double sm = 0.0, sn = 0.0;
-  if (d == 2) {
-    // P2[a][c] = sum_b H[a,b,c] CM2[a][b]      (u = a, w = c)
-    for (int b = 0; b < v; b++) {
-      double mv, nv; cm(u, b, mv, nv);
-      const double hv = (double)H[(long long)u * VV + (long long)b * V + w];
-      sm += hv * mv; sn += hv * nv;
-    }
-  } else if (d == 1) {
-    // P1[a][b] = sum_c H[a,b,c] CM1[a][c]      (u = a, w = b)
-    for (int c = 0; c < v; c++) {
-      double mv, nv; cm(u, c, mv, nv);
-      const double hv = (double)H[(long long)u * VV + (long long)w * V + c];
-      sm += hv * mv; sn += hv * nv;
-    }
-  } else {
-    // P0[a][b] = sum_c H[a,b,c] CM0[b][c]      (u = a, w = b)
-    for (int c = 0; c < v; c++) {
-      double mv, nv; cm(w, c, mv, nv);
-      const double hv = (double)H[(long long)u * VV + (long long)w * V + c];
-      sm += hv * mv; sn += hv * nv;
-    }
+  // pass 2: one wave per (a, b) row
+  const int lane = threadIdx.x & 63;
+  const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= VV) return;
+  const int a = (int)(row / V), b = (int)(row % V);
+  const int t2o = occ[2];
+  const TP* h = H + row * V;
+  const double* CM1 = out + (VP_CM + 2) * VV + (long long)a * V;
+  const double* CN1 = out + (VP_CM + 3) * VV + (long long)a * V;
+  const double* CM0 = out + (VP_CM + 0) * VV + (long long)b * V;
+  const double* CN0 = out + (VP_CM + 1) * VV + (long long)b * V;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = lane; c < v; c += 64) {
+    const double hv = (double)h[c];
+    acc[0] += hv * t1[(long long)t2o * v + c];
+    acc[1] += hv * fvoP[(long long)t2o * V + c];
+    acc[2] += hv * CM1[c];
+    acc[3] += hv * CN1[c];
+    acc[4] += hv * CM0[c];
+    acc[5] += hv * CN0[c];
   }
-  tr.out[(6 + 2 * d) * VV + e] = sm;
-  tr.out[(7 + 2 * d) * VV + e] = sn;
+  // fixed butterfly: every lane ends with the same, order-independent-of-scheduling sum
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
+  if (lane == 0) {
+    out[4 * VV + row] = acc[0];            // S2[a][b]
+    out[5 * VV + row] = acc[1];            // F2[a][b]
+    out[(6 + 2) * VV + row] = acc[2];      // PM1[a][b]
+    out[(7 + 2) * VV + row] = acc[3];      // PN1[a][b]
+    out[(6 + 0) * VV + row] = acc[4];      // PM0[a][b]
+    out[(7 + 0) * VV + row] = acc[5];      // PN0[a][b]
+  }
 }
 
-// Compact per-triple copies of the V-term operands (FP64):
-//   mn6[rho][0] = M_pq[x][y] = ovov[p,x,q,y],  mn6[rho][1] = N_pq[x][y] = t2[q,p,y,x]
-//   for the six ordered pairs (p,q) = (t_rho0, t_rho1) of the triple, and
-//   CM_d[x][y] = M_{p1 p2}[x][y] + M_{p2 p1}[y][x] (CN_d likewise with N),
-//   (p1,p2) = (t_e1, t_e2), e1 < e2 the positions other than d.
-// Tasks 0..11: mn6 (rho = task/2, M/N = task%2); tasks 12..17: CM/CN.
-
+// Pair matrices of the V term of one triple (FP64), task = 2 d + (M/N):
+//   CM_d[x][y] = M_{p1 p2}[x][y] + M_{p2 p1}[y][x],  M_pq[x][y] = ovov[p,x,q,y]
+//   CN_d[x][y] = N_{p1 p2}[x][y] + N_{p2 p1}[y][x],  N_pq[x][y] = t2[q,p,y,x]
+// (p1, p2) = (t_e1, t_e2), e1 < e2 the positions other than d.  They sum the
+// two orderings with the same contracted position in ccsd_t.c:149-150 and
+// ccsd_t_vjp.c:171-204, 228-245.
 __global__ void __launch_bounds__(256)
-vmn_build(const VpTriple* __restrict__ trs, int o, int v, int V, int ntask,
-          const double* __restrict__ ovov, const double* __restrict__ t2) {
-  const VpTriple tr = trs[blockIdx.y / 30];
-  const int task = blockIdx.y % 30;
-  if (ntask < 0 && task >= 12 && task < 18) return;   // forward: no CM/CN
+vcm_build(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ ovov,
+          const double* __restrict__ t2) {
+  const VpTriple tr = trs[blockIdx.y / 6];
+  const int task = blockIdx.y % 6;
   const long long VV = (long long)V * V;
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
   if (e >= VV) return;
   const int x = (int)(e / V), y = (int)(e % V);
+  const int d = task / 2, which = task % 2;
   const int occ[3] = {tr.i, tr.j, tr.k};
-  auto M = [&](int p, int q, int a, int b) -> double {
-    return (a < v && b < v) ? ovov[(((long long)p * v + a) * o + q) * v + b] : 0.0;
-  };
-  auto N = [&](int p, int q, int a, int b) -> double {
-    return (a < v && b < v) ? t2[(((long long)q * o + p) * v + b) * v + a] : 0.0;
-  };
-  if (task < 12 || task >= 18) {
-    const bool tr_out = task >= 18;
-    const int tk = tr_out ? task - 18 : task;
-    const int rho = tk / 2, which = tk % 2;
-    // ordered pair (rho0, rho1) of the permutation rho in the order (01),(02),(10),(12),(20),(21)
-    const int r0 = rho / 2, r1 = (rho % 2 == 0) ? (r0 == 0 ? 1 : 0) : (r0 == 2 ? 1 : 2);
-    const int p = occ[r0], q = occ[r1];
-    if (!tr_out) tr.out[(VP_MN6 + tk) * VV + e] = which == 0 ? M(p, q, x, y) : N(p, q, x, y);
-    else tr.out[(VP_MN6T + tk) * VV + e] = which == 0 ? M(p, q, y, x) : N(p, q, y, x);
-    return;
+  const int p1 = occ[d == 0 ? 1 : 0], p2 = occ[d == 2 ? 1 : 2];
+  double val = 0.0;
+  if (x < v && y < v) {
+    if (which == 0)
+      val = ovov[(((long long)p1 * v + x) * o + p2) * v + y] + ovov[(((long long)p2 * v + y) * o + p1) * v + x];
+    else
+      val = t2[(((long long)p2 * o + p1) * v + y) * v + x] + t2[(((long long)p1 * o + p2) * v + x) * v + y];
   }
-  const int d = (task - 12) / 2, which = (task - 12) % 2;
-  int e1, e2;
-  if (d == 0) { e1 = 1; e2 = 2; } else if (d == 1) { e1 = 0; e2 = 2; } else { e1 = 0; e2 = 1; }
-  const int p1 = occ[e1], p2 = occ[e2];
-  tr.out[(VP_CM + task - 12) * VV + e] =
-      which == 0 ? M(p1, p2, x, y) + M(p2, p1, y, x) : N(p1, p2, x, y) + N(p2, p1, y, x);
-}
-
-// The V term of a triple as a V^3 cube (computed in FP64, stored as TP):
-//   v0[s] = 1/2 sum_rho ( M_rho[s_rho0][s_rho1] t1[t_rho2, s_rho2]
-//                       + N_rho[s_rho0][s_rho1] fock[o + s_rho2, t_rho2] )
-// with M_rho = M_{t_rho0 t_rho1} (ccsd_t.c:149-150 summed over the six orderings).
-// Thread e = (a V + b) V + c; the mn6 / mn6T copies keep every read either
-// broadcast or contiguous in c.
-template <class TP>
-__global__ void __launch_bounds__(256)
-v0_build(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ t1,
-         const double* __restrict__ fock) {
-  const VpTriple tr = trs[blockIdx.y];
-  const long long VV = (long long)V * V;
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= VV * V) return;
-  const int s[3] = {(int)(e / VV), (int)((e / V) % V), (int)(e % V)};
-  const int t[3] = {tr.i, tr.j, tr.k};
-  const int n = o + v;
-  const double* mn6 = tr.out + VP_MN6 * VV;
-  const double* mn6T = tr.out + VP_MN6T * VV;
-  constexpr int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-  double acc = 0.0;
-#pragma unroll
-  for (int rho = 0; rho < 6; rho++) {
-    const int z = s[P[rho][2]];
-    if (z >= v) continue;
-    const int r = t[P[rho][2]];
-    double m, nn;
-    if (P[rho][0] == 2) {   // first index contiguous: read the transposed copy
-      const long long idx = (long long)s[P[rho][1]] * V + s[P[rho][0]];
-      m = mn6T[(2 * rho) * VV + idx];
-      nn = mn6T[(2 * rho + 1) * VV + idx];
-    } else {
-      const long long idx = (long long)s[P[rho][0]] * V + s[P[rho][1]];
-      m = mn6[(2 * rho) * VV + idx];
-      nn = mn6[(2 * rho + 1) * VV + idx];
-    }
-    acc += m * t1[(long long)r * v + z] + nn * fock[(long long)(o + z) * n + r];
-  }
-  reinterpret_cast<TP*>(tr.v0)[e] = (TP)(0.5 * acc);
+  tr.out[(VP_CM + task) * VV + e] = val;
 }
 
 // Scatter of the V-term adjoints of one batch into the FP64 accumulators.
@@ -398,9 +359,9 @@ static inline dim3 grid_for(long long n) {
 }
 
 extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void* ovooT,
-                                     double* eps_occ, double* eps_vir, const double* ovvv, int packed,
-                                     const double* t2, const double* ovoo, const double* mo_energy, int o,
-                                     int v, int Op, int V, hipStream_t s) {
+                                     double* eps_occ, double* eps_vir, double* fvoP, const double* ovvv,
+                                     int packed, const double* t2, const double* ovoo, const double* mo_energy,
+                                     const double* fock, int o, int v, int Op, int V, hipStream_t s) {
   const long long n_ovvv = (long long)o * V * V * V, n_t2 = (long long)Op * Op * V * V,
                   n_ovoo = (long long)o * o * V * Op;
   if (tp_is_f64) {
@@ -412,20 +373,26 @@ extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void
     hipLaunchKernelGGL(prep_t2<float>, grid_for(n_t2), dim3(256), 0, s, (float*)t2T, (float*)t2H, t2, o, v, Op, V);
     hipLaunchKernelGGL(prep_ovoo<float>, grid_for(n_ovoo), dim3(256), 0, s, (float*)ovooT, ovoo, o, v, V, Op);
   }
-  hipLaunchKernelGGL(prep_eps, grid_for(V > o ? V : o), dim3(256), 0, s, eps_occ, eps_vir, mo_energy, o, v, V);
+  const long long ne = (long long)o * V > V ? (long long)o * V : V;
+  hipLaunchKernelGGL(prep_eps, grid_for(ne), dim3(256), 0, s, eps_occ, eps_vir, fvoP, mo_energy, fock, o, v, V);
   return hipGetLastError();
 }
 
 extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs,
                                       int nslab, const VpContrib* contribs, int o, int v, int V,
-                                      const double* t1, const double* fock, const double* ovov,
-                                      const double* t2, double* ovov_acc, double* t2v_acc, double* t1_acc,
-                                      double* fvo_acc, hipStream_t s) {
+                                      const double* t1, const double* fvoP, double* ovov_acc, double* t2v_acc,
+                                      double* t1_acc, double* fvo_acc, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
-  dim3 g1((unsigned)((VV + 255) / 256), 6 * ntriple);
-  if (tp_is_f64) hipLaunchKernelGGL(vpart_contract<double>, g1, dim3(256), 0, s, trs, o, v, V, t1, fock, ovov, t2);
-  else hipLaunchKernelGGL(vpart_contract<float>, g1, dim3(256), 0, s, trs, o, v, V, t1, fock, ovov, t2);
+  // passes 0/1: one thread per output element; pass 2: one wave per row (4 rows per block)
+  dim3 g1((unsigned)((VV + 255) / 256), 2 * ntriple), g2p((unsigned)((VV + 3) / 4), ntriple);
+  if (tp_is_f64) {
+    hipLaunchKernelGGL(vpart_contract<double>, g1, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 2, 0);
+    hipLaunchKernelGGL(vpart_contract<double>, g2p, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 1, 2);
+  } else {
+    hipLaunchKernelGGL(vpart_contract<float>, g1, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 2, 0);
+    hipLaunchKernelGGL(vpart_contract<float>, g2p, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 1, 2);
+  }
   const long long nx = VV > (long long)o * V ? VV : (long long)o * V;
   dim3 g2((unsigned)((nx + 255) / 256), nslab + 1);
   hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, V, ovov_acc,
@@ -433,23 +400,12 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
   return hipGetLastError();
 }
 
-extern "C" hipError_t t3_launch_vmn(const VpTriple* trs, int ntriple, int o, int v, int V, int with_cm,
-                                    const double* ovov, const double* t2, hipStream_t s) {
+extern "C" hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
+                                    const double* t2, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
-  // tasks: 0..11 mn6, 12..17 CM/CN (backward only), 18..29 mn6 transposed
-  dim3 g((unsigned)((VV + 255) / 256), 30 * ntriple);
-  hipLaunchKernelGGL(vmn_build, g, dim3(256), 0, s, trs, o, v, V, with_cm ? 30 : -1, ovov, t2);
-  return hipGetLastError();
-}
-
-extern "C" hipError_t t3_launch_v0(int tp_is_f64, const VpTriple* trs, int ntriple, int o, int v, int V,
-                                   const double* t1, const double* fock, hipStream_t s) {
-  if (ntriple <= 0) return hipSuccess;
-  const long long n = (long long)V * V * V;
-  dim3 g((unsigned)((n + 255) / 256), ntriple);
-  if (tp_is_f64) hipLaunchKernelGGL(v0_build<double>, g, dim3(256), 0, s, trs, o, v, V, t1, fock);
-  else hipLaunchKernelGGL(v0_build<float>, g, dim3(256), 0, s, trs, o, v, V, t1, fock);
+  dim3 g((unsigned)((VV + 255) / 256), 6 * ntriple);
+  hipLaunchKernelGGL(vcm_build, g, dim3(256), 0, s, trs, o, v, V, ovov, t2);
   return hipGetLastError();
 }
 
