@@ -77,6 +77,33 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double *mo_energy, c
  * scratch-allocator pattern of pyscfadlib/pyscfadlib/ffi_helpers.h:40-50. */
 size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp);
 
+/*
+ * Work sharding over unique occupied triples (no counterpart in the
+ * reference, whose unit of work is the virtual-triple block of
+ * ccsd_t.py:145-177).  The unique triples i >= j >= k are enumerated with i
+ * outermost, k innermost; pyscfad_amd_ccsd_t_num_triples(nocc) =
+ * nocc(nocc+1)(nocc+2)/6 of them.  The _slice calls compute the share of
+ * E(T) and of every cotangent contributed by triples [triple_begin,
+ * triple_end); the shares of a partition of the list sum to the full result.
+ * This is the unit one GPU of a multi-GPU run takes, and what a benchmark
+ * times on a bounded sample of a large instance.
+ */
+long long pyscfad_amd_ccsd_t_num_triples(int nocc);
+int pyscfad_amd_ccsd_t_energy_slice(int nocc, int nvir, const double *mo_energy, const double *t1,
+                                    const double *t2, const double *fock, const double *ovoo,
+                                    const double *ovov, const double *ovvv, int ovvv_packed,
+                                    int precision, long long triple_begin, long long triple_end,
+                                    double *e_out, void *workspace, size_t workspace_bytes,
+                                    void *stream);
+int pyscfad_amd_ccsd_t_energy_vjp_slice(int nocc, int nvir, const double *mo_energy, const double *t1,
+                                        const double *t2, const double *fock, const double *ovoo,
+                                        const double *ovov, const double *ovvv, int ovvv_packed,
+                                        double et_bar, int precision, long long triple_begin,
+                                        long long triple_end, double *t1_bar, double *t2_bar,
+                                        double *fock_bar, double *mo_energy_bar, double *ovoo_bar,
+                                        double *ovov_bar, double *ovvv_bar, double *e_out,
+                                        void *workspace, size_t workspace_bytes, void *stream);
+
 /* Text of the last error of the calling thread ("" if none). */
 const char *pyscfad_amd_last_error(void);
 

@@ -18,6 +18,9 @@ EXPORTS = (
     "pyscfad_amd_ccsd_t_energy",
     "pyscfad_amd_ccsd_t_energy_vjp",
     "pyscfad_amd_ccsd_t_workspace_bytes",
+    "pyscfad_amd_ccsd_t_num_triples",
+    "pyscfad_amd_ccsd_t_energy_slice",
+    "pyscfad_amd_ccsd_t_energy_vjp_slice",
     "pyscfad_amd_last_error",
     "pyscfad_amd_set_timing",
     "pyscfad_amd_last_timing",
@@ -47,6 +50,14 @@ def load():
     lib.pyscfad_amd_ccsd_t_energy_vjp.argtypes = [i, i] + [dp] * 7 + [i, d, i] + [dp] * 8 + [vp, sz, vp]
     lib.pyscfad_amd_ccsd_t_workspace_bytes.restype = sz
     lib.pyscfad_amd_ccsd_t_workspace_bytes.argtypes = [i, i, i, i]
+    ll = ctypes.c_longlong
+    lib.pyscfad_amd_ccsd_t_num_triples.restype = ll
+    lib.pyscfad_amd_ccsd_t_num_triples.argtypes = [i]
+    lib.pyscfad_amd_ccsd_t_energy_slice.restype = i
+    lib.pyscfad_amd_ccsd_t_energy_slice.argtypes = [i, i] + [dp] * 7 + [i, i, ll, ll, dp, vp, sz, vp]
+    lib.pyscfad_amd_ccsd_t_energy_vjp_slice.restype = i
+    lib.pyscfad_amd_ccsd_t_energy_vjp_slice.argtypes = ([i, i] + [dp] * 7 + [i, d, i, ll, ll] + [dp] * 8
+                                                        + [vp, sz, vp])
     lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
     lib.pyscfad_amd_last_error.argtypes = []
     lib.pyscfad_amd_set_timing.restype = None

@@ -72,20 +72,29 @@ def _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv):
     return nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed
 
 
-def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISION, stream=None):
-    """E(T) as a 0-d FP64 device tensor (the forward of the custom_vjp)."""
+def num_triples(nocc):
+    """Number of unique occupied triples i >= j >= k (the unit of work sharding)."""
+    return _lib.load().pyscfad_amd_ccsd_t_num_triples(nocc)
+
+
+def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISION, stream=None,
+           triples=None):
+    """E(T) as a 0-d FP64 device tensor (the forward of the custom_vjp).
+    ``triples=(begin, end)`` restricts the sum to that slice of the unique
+    occupied triples (a shard of the work)."""
     nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
     out = torch.zeros((), dtype=torch.float64, device=t1d.device)
     s = stream if stream is not None else torch.cuda.current_stream()
     lib = _lib.load()
-    _lib.check(lib.pyscfad_amd_ccsd_t_energy(
+    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
+    _lib.check(lib.pyscfad_amd_ccsd_t_energy_slice(
         nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
-        int(packed), _prec(precision), _ptr(out), None, 0, s.cuda_stream))
+        int(packed), _prec(precision), int(t0), int(t1_), _ptr(out), None, 0, s.cuda_stream))
     return out
 
 
 def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=DEFAULT_PRECISION,
-               stream=None, return_energy=False):
+               stream=None, return_energy=False, triples=None):
     """Cotangents of E(T) as FP64 device tensors, in the order and layouts of
     the reference's _ccsd_t_energy_vjp: (t1, t2, fock, mo_energy, ovoo, ovov, ovvv)."""
     nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
@@ -96,10 +105,11 @@ def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=
     lib = _lib.load()
     if hasattr(et_bar, "item"):
         et_bar = et_bar.item()
-    _lib.check(lib.pyscfad_amd_ccsd_t_energy_vjp(
+    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
+    _lib.check(lib.pyscfad_amd_ccsd_t_energy_vjp_slice(
         nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
-        int(packed), float(et_bar), _prec(precision), *[_ptr(b) for b in bars], _ptr(out),
-        None, 0, s.cuda_stream))
+        int(packed), float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars],
+        _ptr(out), None, 0, s.cuda_stream))
     if return_energy:
         return tuple(bars), out
     return tuple(bars)

@@ -237,6 +237,7 @@ struct Ctx {
   int B = 1;
   bool backward = false;
   double et_bar = 1.0;
+  long long t_begin = 0, t_end = -1;   // slice of the unique-triple list (-1: to the end)
   // inputs (device, reference layouts)
   const double *mo_energy = nullptr, *t1 = nullptr, *t2 = nullptr, *fock = nullptr, *ovoo = nullptr,
                *ovov = nullptr, *ovvv = nullptr;
@@ -617,7 +618,11 @@ struct Ctx {
   }
 
   void build_plan() {
-    const std::vector<Triple> all = enumerate_triples(d.o);
+    std::vector<Triple> all = enumerate_triples(d.o);
+    const long long n = (long long)all.size();
+    const long long e = (t_end < 0 || t_end > n) ? n : t_end;
+    const long long b = std::min(std::max(0LL, t_begin), e);
+    all = std::vector<Triple>(all.begin() + b, all.begin() + e);
     for (size_t b0 = 0; b0 < all.size(); b0 += B) {
       std::vector<Triple> bt(all.begin() + b0, all.begin() + std::min(all.size(), b0 + B));
       emit_wbuild(bt);
@@ -704,11 +709,14 @@ int check_args(int nocc, int nvir, int precision) {
 }
 
 // Size the descriptor blob by building the plan against a null workspace.
-size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B) {
+size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B,
+                  long long t_begin = 0, long long t_end = -1) {
   Ctx c;
   c.d = make_dims(nocc, nvir, precision);
   c.B = choose_batch(c.d);
   c.backward = backward;
+  c.t_begin = t_begin;
+  c.t_end = t_end;
   c.r = plan_regions(c.d, c.B, backward);
   c.build_plan();
   c.r.desc_bytes = c.blob.data.size();
@@ -722,10 +730,14 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
         const double* ovoo, const double* ovov, const double* ovvv, int packed, bool backward, double et_bar,
         int precision, double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
         double* ovoo_bar, double* ovov_bar, double* ovvv_bar, double* e_out, void* workspace,
-        size_t workspace_bytes, void* stream) {
+        size_t workspace_bytes, void* stream, long long t_begin = 0, long long t_end = -1) {
   g_last_error.clear();
   int rc = check_args(nocc, nvir, precision);
   if (rc) return rc;
+  const long long ntrip = (long long)nocc * (nocc + 1) * (nocc + 2) / 6;
+  if (t_end < 0) t_end = ntrip;
+  if (t_begin < 0 || t_begin > t_end || t_end > ntrip)
+    return fail(PYSCFAD_AMD_ERR_ARGS, "triple range must satisfy 0 <= begin <= end <= nocc(nocc+1)(nocc+2)/6");
   if (!mo_energy || !t1 || !t2 || !fock || !ovoo || !ovov || !ovvv)
     return fail(PYSCFAD_AMD_ERR_ARGS, "null input pointer");
   if (packed != 0 && packed != 1) return fail(PYSCFAD_AMD_ERR_ARGS, "ovvv_packed must be 0 or 1");
@@ -738,7 +750,9 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   c.d = make_dims(nocc, nvir, precision);
   c.backward = backward;
   c.et_bar = et_bar;
-  const size_t need = plan_bytes(nocc, nvir, precision, backward, &c.r, &c.B);
+  c.t_begin = t_begin;
+  c.t_end = t_end;
+  const size_t need = plan_bytes(nocc, nvir, precision, backward, &c.r, &c.B, t_begin, t_end);
   bool own = false;
   if (workspace) {
     if (workspace_bytes < need)
@@ -822,6 +836,31 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double* mo_energy, c
   return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, precision, t1_bar,
              t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out, workspace, workspace_bytes,
              stream);
+}
+
+long long pyscfad_amd_ccsd_t_num_triples(int nocc) {
+  return nocc < 1 ? 0 : (long long)nocc * (nocc + 1) * (nocc + 2) / 6;
+}
+
+int pyscfad_amd_ccsd_t_energy_slice(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
+                                    const double* fock, const double* ovoo, const double* ovov, const double* ovvv,
+                                    int ovvv_packed, int precision, long long triple_begin, long long triple_end,
+                                    double* e_out, void* workspace, size_t workspace_bytes, void* stream) {
+  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, false, 1.0, precision, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out, workspace, workspace_bytes, stream,
+             triple_begin, triple_end);
+}
+
+int pyscfad_amd_ccsd_t_energy_vjp_slice(int nocc, int nvir, const double* mo_energy, const double* t1,
+                                        const double* t2, const double* fock, const double* ovoo,
+                                        const double* ovov, const double* ovvv, int ovvv_packed, double et_bar,
+                                        int precision, long long triple_begin, long long triple_end,
+                                        double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
+                                        double* ovoo_bar, double* ovov_bar, double* ovvv_bar, double* e_out,
+                                        void* workspace, size_t workspace_bytes, void* stream) {
+  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, precision, t1_bar,
+             t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out, workspace, workspace_bytes,
+             stream, triple_begin, triple_end);
 }
 
 size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp) {

@@ -111,6 +111,45 @@ struct Stage {
     }
   }
 
+  // K-contiguous tiles go to LDS as [row][k] (rows of LDK elements); all
+  // other tiles as [k][row] (rows of ROWS elements, XOR-swizzled by
+  // swz(k) on 16-element groups), so that neither store needs a transpose.
+  template <int LDK>
+  __device__ __forceinline__ void store_rk(T* __restrict__ lds) const {
+    const int t = threadIdx.x;
+    using V = typename VecT<T, VEC>::type;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      const int id = t + 256 * q;
+      const int kv = id % (BK / VEC), row = id / (BK / VEC);
+      *reinterpret_cast<V*>(&lds[row * LDK + kv * VEC]) = *reinterpret_cast<const V*>(&r[q * VEC]);
+    }
+  }
+  __device__ __forceinline__ static int swz(int k) {
+    // f32 [k][row] reads touch rows k = g*KG + ks for the four lane groups g;
+    // flipping column bit 4 on odd g spreads them over both bank halves
+    return (sizeof(T) == 4) ? (((k / (BK / 4)) & 1) << 4) : 0;
+  }
+  __device__ __forceinline__ void store_kr(int mode, T* __restrict__ lds) const {
+    const int t = threadIdx.x;
+    if (mode == LOAD_ROWCONT && CANVEC) {
+      using V = typename VecT<T, VEC>::type;
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        const int id = t + 256 * q;
+        const int k = id / (ROWS / VEC), rv = id % (ROWS / VEC);
+        *reinterpret_cast<V*>(&lds[k * ROWS + ((rv * VEC) ^ swz(k))]) = *reinterpret_cast<const V*>(&r[q * VEC]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NE; q++) {
+        const int id = t + 256 * q;
+        const int row = id % ROWS, k = id / ROWS;
+        lds[k * ROWS + (row ^ swz(k))] = r[q];
+      }
+    }
+  }
+
   // LDS layout [row][k] as split bf16 pairs (bf16x3 MFMA operands)
   template <int LDK>
   __device__ __forceinline__ void store_rowk_split(int mode, __bf16* __restrict__ hi,
@@ -258,43 +297,80 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   };
 
   if constexpr (PREC != PREC_BF16X3) {
-    constexpr int LDA = BM + (sizeof(T) == 4 ? 16 : 0);
-    constexpr int LDB = BN + (sizeof(T) == 4 ? 16 : 0);
-    __shared__ T lds_a[2][BK * LDA];
-    __shared__ T lds_b[2][BK * LDB];
+    // K is consumed in a permuted order: in step ks the lanes of group
+    // g = lane/16 multiply k = g*KG + ks (the sum over k is order-free), so a
+    // lane's KG values of a [row][k] tile are contiguous (two 16-byte reads).
+    constexpr int KG = BK / 4;
+    constexpr int LDK = BK + 16 / (int)sizeof(T);   // [row][k] row pitch, staggered by 16 B
+    constexpr int SZA = (BM * LDK > BK * BM) ? BM * LDK : BK * BM;
+    constexpr int SZB = (BN * LDK > BK * BN) ? BN * LDK : BK * BN;
+    __shared__ __attribute__((aligned(16))) T lds_a[2][SZA];
+    __shared__ __attribute__((aligned(16))) T lds_b[2][SZB];
+    using SA = Stage<T, BM, BK, VEC>;
+    using SB = Stage<T, BN, BK, VEC>;
     int amode = 0, bmode = 0;
+    bool rk_a[2] = {false, false}, rk_b[2] = {false, false};
+    auto store = [&](int buf) {
+      rk_a[buf] = (amode == LOAD_KCONT) && SA::CANVEC;
+      rk_b[buf] = (bmode == LOAD_KCONT) && SB::CANVEC;
+      if (rk_a[buf]) sa.template store_rk<LDK>(lds_a[buf]); else sa.store_kr(amode, lds_a[buf]);
+      if (rk_b[buf]) sb.template store_rk<LDK>(lds_b[buf]); else sb.store_kr(bmode, lds_b[buf]);
+    };
     if (c_begin < c_end) {
       load_chunk(c_begin, amode, bmode);
-      sa.template store_krow<LDA>(amode, lds_a[0]);
-      sb.template store_krow<LDB>(bmode, lds_b[0]);
+      store(0);
     }
     __syncthreads();
     int cur = 0;
+    const int g = lane >> 4, l16 = lane & 15;
     for (int c = c_begin; c < c_end; c++) {
       const bool more = c + 1 < c_end;
       if (more) load_chunk(c + 1, amode, bmode);
       const T* la = lds_a[cur];
       const T* lb = lds_b[cur];
+      T av[FM][KG], bv[FN][KG];
+      if (rk_a[cur]) {
 #pragma unroll
-      for (int ks = 0; ks < BK / 4; ks++) {
-        const int kr = ks * 4 + (lane >> 4);
-        T av[FM], bv[FN];
+        for (int i = 0; i < FM; i++) {
+          const T* src = la + (wm0 + i * 16 + l16) * LDK + g * KG;
 #pragma unroll
-        for (int i = 0; i < FM; i++) av[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
+          for (int ks = 0; ks < KG; ks++) av[i][ks] = src[ks];
+        }
+      } else {
 #pragma unroll
-        for (int j = 0; j < FN; j++) bv[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+          for (int ks = 0; ks < KG; ks++) {
+            const int k = g * KG + ks;
+            av[i][ks] = la[k * BM + ((wm0 + i * 16 + l16) ^ SA::swz(k))];
+          }
+      }
+      if (rk_b[cur]) {
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+          const T* src = lb + (wn0 + j * 16 + l16) * LDK + g * KG;
+#pragma unroll
+          for (int ks = 0; ks < KG; ks++) bv[j][ks] = src[ks];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+          for (int ks = 0; ks < KG; ks++) {
+            const int k = g * KG + ks;
+            bv[j][ks] = lb[k * BN + ((wn0 + j * 16 + l16) ^ SB::swz(k))];
+          }
+      }
+#pragma unroll
+      for (int ks = 0; ks < KG; ks++)
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
           for (int j = 0; j < FN; j++) {
-            if constexpr (PREC == PREC_F64) acc[i][j] = mfma_f64(av[i], bv[j], acc[i][j]);
-            else acc[i][j] = mfma_f32(av[i], bv[j], acc[i][j]);
+            if constexpr (PREC == PREC_F64) acc[i][j] = mfma_f64(av[i][ks], bv[j][ks], acc[i][j]);
+            else acc[i][j] = mfma_f32(av[i][ks], bv[j][ks], acc[i][j]);
           }
-      }
-      if (more) {
-        sa.template store_krow<LDA>(amode, lds_a[cur ^ 1]);
-        sb.template store_krow<LDB>(bmode, lds_b[cur ^ 1]);
-      }
+      if (more) store(cur ^ 1);
       __syncthreads();
       cur ^= 1;
     }

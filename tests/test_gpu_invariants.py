@@ -48,3 +48,25 @@ def test_f32_agrees_with_f64(gpu_lib):
     assert abs(e32 - e64) <= 1e-6 * abs(e64)
     for x, y in zip(b32, b64):
         assert np.abs(x - y).max() <= 1e-5 * np.abs(y).max()
+
+
+def test_triple_slices_partition(gpu_lib):
+    """The shares of a partition of the unique-triple list sum to the full
+    E(T) and cotangents (the unit of multi-GPU sharding)."""
+    from pyscfad_amd.cc import ccsd_t
+    d = make_instance(7, 45, 12, fvo_scale=0.02)
+    args = [d[k] for k in ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")]
+    n = ccsd_t.num_triples(7)
+    assert n == 7 * 8 * 9 // 6
+    full, e_full = ccsd_t.energy_vjp(*args, return_energy=True)
+    cuts = [0, 5, 37, n]
+    acc = None
+    e_sum = 0.0
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        part, e = ccsd_t.energy_vjp(*args, return_energy=True, triples=(a, b))
+        e_sum += float(e)
+        assert abs(float(ccsd_t.energy(*args, triples=(a, b))) - float(e)) <= 1e-12 * abs(float(e_full))
+        acc = [p.clone() for p in part] if acc is None else [x + p for x, p in zip(acc, part)]
+    assert abs(e_sum - float(e_full)) <= 1e-12 * abs(float(e_full))
+    for x, y in zip(acc, full):
+        assert torch.allclose(x, y, rtol=1e-11, atol=1e-14 * float(torch.max(torch.abs(y))))
