@@ -41,6 +41,12 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 accuracy reference run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--triples", default=None,
+                    help="BEGIN:END slice of the unique occupied triples (a bounded sample of a large "
+                         "instance, or a profiling run); default: all")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of this many MiB (reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -75,6 +81,20 @@ def cpu_baseline(inst, nocc, nvir, target_s):
                       f"(nocc={nocc}, nvir={nvir}) in {t:.1f} s, FP64, credited 3*F_E per job"}
 
 
+def slice_flops(nocc, nvir, t0, t1):
+    """Algorithmic F_E of the unique triples [t0, t1): each unique triple stands
+    for its distinct orderings (6, 3 or 1), each costing 2 v^3 (v + o)."""
+    per = 2.0 * nvir ** 3 * (nvir + nocc)
+    total, idx = 0.0, 0
+    for i in range(nocc):
+        for j in range(i + 1):
+            for k in range(j + 1):
+                if t0 <= idx < t1:
+                    total += per * (1 if i == k else (3 if (i == j or j == k) else 6))
+                idx += 1
+    return total
+
+
 def max_rel(a, b):
     import torch
     scale = torch.max(torch.abs(b)).item()
@@ -98,13 +118,25 @@ def main():
     torch.cuda.synchronize()
     keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")
     a = [inst[k] for k in keys]
-    fe = flops_energy(o, v)
+    ntrip = ccsd_t.num_triples(o)
+    if args.triples:
+        t0, t1 = (int(x) for x in args.triples.split(":"))
+    else:
+        t0, t1 = 0, ntrip
+    sl = (t0, t1)
+    fe = flops_energy(o, v) if (t0, t1) == (0, ntrip) else slice_flops(o, v, t0, t1)
+    if args.calibrate_copy:
+        src = torch.empty(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step(prec, timing=False):
         _lib.set_timing(timing)
-        e = ccsd_t.energy(*a, precision=prec)
+        e = ccsd_t.energy(*a, precision=prec, triples=sl)
         t_fwd = _lib.last_timing() if timing else None
-        bars = ccsd_t.energy_vjp(*a, et_bar=1.0, precision=prec)
+        bars = ccsd_t.energy_vjp(*a, et_bar=1.0, precision=prec, triples=sl)
         t_bwd = _lib.last_timing() if timing else None
         _lib.set_timing(False)
         return e, bars, t_fwd, t_bwd
@@ -160,7 +192,8 @@ def main():
         "dtype": {"f64": "f64", "f32": "f32 (FP64 denominators and sums)",
                   "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": f"single (T) instance nocc={o} nvir={v} seed={args.seed}",
+        "config": {"workload": f"single (T) instance nocc={o} nvir={v} seed={args.seed}"
+                               + ("" if (t0, t1) == (0, ntrip) else f", unique triples [{t0},{t1}) of {ntrip}"),
                    "precision": args.precision, "E_T": float(e)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
