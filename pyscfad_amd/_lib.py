@@ -80,7 +80,8 @@ def check(rc):
         raise T3Error(f"pyscfad_amd error {rc}: {msg}")
 
 
-FAMILIES = ("gemm", "epilogue", "vpart", "reduce")
+FAMILIES = ("gemm", "epilogue", "vpart", "reduce", "prep",
+            "gemm_w", "gemm_b1", "gemm_b2", "gemm_b3", "gemm_b4")
 
 
 def set_timing(enable):
@@ -89,9 +90,10 @@ def set_timing(enable):
 
 def last_timing():
     """{family: (ms, launches)} of the calling thread's last call with timing on."""
-    ms = (ctypes.c_double * 8)()
-    nl = (ctypes.c_longlong * 8)()
-    m = load().pyscfad_amd_last_timing(ms, nl, 8)
+    n = len(FAMILIES)
+    ms = (ctypes.c_double * n)()
+    nl = (ctypes.c_longlong * n)()
+    m = load().pyscfad_amd_last_timing(ms, nl, n)
     return {FAMILIES[f]: (ms[f], nl[f]) for f in range(min(m, len(FAMILIES)))}
 
 

@@ -9,6 +9,7 @@ namespace t3 {
 // Arithmetic of the W/V contractions.
 enum Prec : int { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
 
+typedef double d2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
@@ -18,6 +19,11 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 // Element addresses (in elements):
 //   A: b*a_sb + m*a_sm + k1*a_sk1 + k2*a_sk2
 //   B: b*b_sb + k1*b_sk1 + k2*b_sk2 + n*b_sn
+// Every operand is either row-contiguous (a_sm == 1 / b_sn == 1) or
+// k-contiguous (a_sk2 == 1 / b_sk2 == 1); the host sets the modes and checks
+// the vector alignment of every stride and base.
+enum LoadMode : int { LOAD_ROWCONT = 0, LOAD_KCONT = 1 };
+
 struct GemmSeg {
   const void* A;
   const void* B;
@@ -26,6 +32,7 @@ struct GemmSeg {
   int K1, K2;
   float scale;
   int chunk0;   // first K-chunk of this segment in the problem's concatenated K
+  int a_mode, b_mode;
 };
 
 enum GemmMode : int {

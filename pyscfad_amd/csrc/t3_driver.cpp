@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -33,6 +34,7 @@ using namespace t3;
 extern "C" {
 int t3_gemm_tile_m(int cfg);
 int t3_gemm_tile_n(int cfg);
+int t3_gemm_vec(int prec);
 int t3_gemm_chunk(int prec);
 hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob, const GemmSeg* segs,
                           long long nblocks, hipStream_t stream);
@@ -64,7 +66,12 @@ constexpr int MAX_KSPLIT = 16;
 
 thread_local std::string g_last_error;
 // per-family device time (ms) of the last call, recorded with hipEvents
-enum Family { FAM_GEMM = 0, FAM_EPILOGUE = 1, FAM_VPART = 2, FAM_REDUCE = 3, FAM_PREP = 4, FAM_COUNT = 5 };
+// families 5..9 split the GEMM total (family 0) by contraction: W build
+// (forward and backward recompute), then the four adjoint contractions
+enum Family {
+  FAM_GEMM = 0, FAM_EPILOGUE = 1, FAM_VPART = 2, FAM_REDUCE = 3, FAM_PREP = 4,
+  FAM_GEMM_W = 5, FAM_GEMM_B1 = 6, FAM_GEMM_B2 = 7, FAM_GEMM_B3 = 8, FAM_GEMM_B4 = 9, FAM_COUNT = 10
+};
 thread_local int g_timing = 0;
 thread_local double g_family_ms[FAM_COUNT];
 thread_local long long g_family_launches[FAM_COUNT];
@@ -120,11 +127,21 @@ struct Regions {
   size_t total;
 };
 
+// Triples per batch.  Larger batches merge the adjoint GEMM problems that
+// share an output slab (consecutive triples share two occupied indices), so
+// fewer read-modify-write passes over the FP64 accumulators, and they give
+// every launch more blocks.  Bounded by the per-slot memory (w0, G, H cubes).
+// PYSCFAD_AMD_BATCH (1..64) overrides the choice, for tuning.
 int choose_batch(const Dims& d) {
-  // enough W-build blocks (V x (V/64)^2 per triple) to keep 256 CUs busy
-  const long long tiles = (long long)d.V * ((d.V + 63) / 64) * ((d.V + 63) / 64);
-  long long b = (8192 + tiles - 1) / tiles;
-  return (int)std::max<long long>(1, std::min<long long>(b, 16));
+  const double cube = (double)d.V * d.V * d.V * (double)d.elt;
+  const double budget = 16.0e9;
+  long long cap = 2;
+  if (const char* env = std::getenv("PYSCFAD_AMD_BATCH")) {
+    const long long x = std::atoll(env);
+    if (x >= 1 && x <= 64) return (int)x;
+  }
+  long long b = (long long)(budget / (3.0 * cube));
+  return (int)std::max<long long>(1, std::min<long long>(b, cap));
 }
 
 Regions plan_regions(const Dims& d, int B, bool backward) {
@@ -178,6 +195,7 @@ struct Launch {
   int n_a = 0, n_b = 0, n_c = 0;
   long long work = 0;                        // GEMM blocks / reduce elements
   int aux = -1;                              // index of host-side argument struct
+  int sub = -1;                              // GEMM contraction family (FAM_GEMM_*)
 };
 
 struct Blob {
@@ -246,6 +264,7 @@ struct Ctx {
   std::vector<Launch> launches;
   std::vector<EpiArgs> epi_args;
   std::vector<EpiReduceArgs> epired_args;
+  std::string plan_error;   // set when a contraction does not fit the GEMM's operand modes
 
   template <class T = char>
   T* at(size_t off) const { return reinterpret_cast<T*>(ws + off); }
@@ -268,7 +287,7 @@ struct Ctx {
   }
 
   // Flatten a GemmBuild into a launch.
-  void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split) {
+  void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub) {
     if (gb.empty()) return;
     if (cfg < 0) cfg = pick_cfg(gb);
     const int BM = t3_gemm_tile_m(cfg), BN = t3_gemm_tile_n(cfg);
@@ -291,6 +310,25 @@ struct Ctx {
     if (allow_split) {
       ksplit = (int)std::max<long long>(1, (GEMM_TARGET_BLOCKS + tiles - 1) / tiles);
       ksplit = std::min(ksplit, std::min(MAX_KSPLIT, std::max(1, min_chunks)));
+    }
+    // operand load modes; every base and stride must keep VEC-element vectors aligned
+    const long long vec = t3_gemm_vec(d.prec);
+    const long long vbytes = vec * (long long)d.elt;
+    auto aligned = [&](const void* p) { return ((uintptr_t)p % (uintptr_t)vbytes) == 0; };
+    for (size_t p = 0; p < gb.probs.size(); p++) {
+      const GemmProb& P = gb.probs[p];
+      for (auto& s : gb.segs[p]) {
+        s.a_mode = s.a_sm == 1 ? LOAD_ROWCONT : LOAD_KCONT;
+        s.b_mode = s.b_sn == 1 ? LOAD_ROWCONT : LOAD_KCONT;
+        const long long a_row = s.a_mode == LOAD_ROWCONT ? s.a_sk2 : s.a_sm;
+        const long long b_row = s.b_mode == LOAD_ROWCONT ? s.b_sk2 : s.b_sn;
+        const bool ok = (s.a_mode == LOAD_ROWCONT ? P.M % vec == 0 : s.a_sk2 == 1) &&
+                        (s.b_mode == LOAD_ROWCONT ? P.N % vec == 0 : s.b_sk2 == 1) &&
+                        a_row % vec == 0 && b_row % vec == 0 && s.a_sb % vec == 0 && s.b_sb % vec == 0 &&
+                        s.a_sk1 % vec == 0 && s.b_sk1 % vec == 0 && s.K2 % gb.BK == 0 &&
+                        aligned(s.A) && aligned(s.B);
+        if (!ok && plan_error.empty()) plan_error = "GEMM operand is neither row- nor k-contiguous, or misaligned";
+      }
     }
     std::vector<GemmSeg> flat;
     std::vector<ReduceProb> red;
@@ -339,6 +377,7 @@ struct Ctx {
     L.off_b = blob.put(flat);
     L.n_b = (int)flat.size();
     L.work = nblocks;
+    L.sub = sub;
     launches.push_back(L);
     if (allow_split) {
       Launch R;
@@ -406,7 +445,7 @@ struct Ctx {
         for (int q = 0; q < np; q++) gb.add(p, particle_seg(bt[s], pass == 0 ? P1[q] : P2[q], ib, im));
         for (int q = 0; q < np; q++) gb.add(p, hole_seg(bt[s], pass == 0 ? H1[q] : H2[q], ib, im));
       }
-      emit_gemm(gb, -1, pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false);
+      emit_gemm(gb, -1, pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false, FAM_GEMM_W);
     }
   }
 
@@ -490,7 +529,7 @@ struct Ctx {
           sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
           gb.add(q, sg);
         }
-      emit_gemm(gb, -1, GEMM_ACCUM, 1, false);
+      emit_gemm(gb, -1, GEMM_ACCUM, 1, false, FAM_GEMM_B1);
     }
   }
 
@@ -519,7 +558,7 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, -1, GEMM_ACCUM, 1, true);
+    emit_gemm(gb, -1, GEMM_ACCUM, 1, true, FAM_GEMM_B2);
   }
 
   void emit_b3(const std::vector<Triple>& bt) {
@@ -549,7 +588,7 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, d.Op <= 32 ? 3 : 0, GEMM_ACCUM, 1, true);
+    emit_gemm(gb, d.Op <= 32 ? 3 : 0, GEMM_ACCUM, 1, true, FAM_GEMM_B3);
   }
 
   void emit_b4(const std::vector<Triple>& bt) {
@@ -576,7 +615,7 @@ struct Ctx {
         sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, d.Op <= 32 ? 4 : 0, GEMM_ACCUM, 1, false);
+    emit_gemm(gb, d.Op <= 32 ? 4 : 0, GEMM_ACCUM, 1, false, FAM_GEMM_B4);
   }
 
   void emit_vpart(const std::vector<Triple>& bt) {
@@ -641,7 +680,7 @@ struct Ctx {
     char* db = ws + r.desc;
     hipError_t e = hipSuccess;
     std::vector<hipEvent_t> ev;
-    std::vector<int> fam;
+    std::vector<int> fam, sub;
     for (const Launch& L : launches) {
       if (g_timing) {
         hipEvent_t a, b;
@@ -652,6 +691,7 @@ struct Ctx {
         fam.push_back(L.kind == L_GEMM ? FAM_GEMM
                       : (L.kind == L_EPI || L.kind == L_EPIRED) ? FAM_EPILOGUE
                       : (L.kind == L_VPART || L.kind == L_VMN) ? FAM_VPART : FAM_REDUCE);
+        sub.push_back(L.kind == L_GEMM ? L.sub : -1);
         if ((e = hipEventRecord(a, st)) != hipSuccess) return e;
       }
       switch (L.kind) {
@@ -693,6 +733,10 @@ struct Ctx {
         if ((e = hipEventElapsedTime(&ms, ev[2 * x], ev[2 * x + 1])) != hipSuccess) return e;
         g_family_ms[fam[x]] += ms;
         g_family_launches[fam[x]]++;
+        if (sub[x] >= 0) {
+          g_family_ms[sub[x]] += ms;
+          g_family_launches[sub[x]]++;
+        }
       }
       for (auto& x : ev) (void)hipEventDestroy(x);
     }
@@ -755,6 +799,8 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   const size_t need = plan_bytes(nocc, nvir, precision, backward, &c.r, &c.B, t_begin, t_end);
   bool own = false;
   if (workspace) {
+    if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+      return fail(PYSCFAD_AMD_ERR_ARGS, "workspace must be 256-byte aligned");
     if (workspace_bytes < need)
       return fail(PYSCFAD_AMD_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
     c.ws = reinterpret_cast<char*>(workspace);
@@ -768,6 +814,10 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   c.mo_energy = mo_energy; c.t1 = t1; c.t2 = t2; c.fock = fock; c.ovoo = ovoo; c.ovov = ovov; c.ovvv = ovvv;
   c.packed = packed;
   c.build_plan();
+  if (!c.plan_error.empty()) {
+    if (own) (void)hipFree(c.ws);
+    return fail(PYSCFAD_AMD_ERR_ARGS, "internal plan error: " + c.plan_error);
+  }
   auto cleanup = [&](int code) {
     if (own) {
       (void)hipStreamSynchronize(st);

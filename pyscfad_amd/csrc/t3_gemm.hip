@@ -13,177 +13,144 @@
 // every output element has exactly one writer: no atomics anywhere, and the
 // result does not depend on scheduling.
 //
-// Tiles are staged through LDS with double buffering (one barrier per K-chunk);
-// global loads of chunk c+1 are in flight while chunk c is multiplied.
+// Staging: a K-chunk of each operand tile is read from HBM with 16-byte
+// global loads into registers (rows past the edge are clamped to the last
+// valid row, so the loads carry no branches), and written to LDS after the
+// MFMAs of the previous chunk, scaled and with the clamped rows zeroed.  The
+// LDS layout is always [row][k] with a 16-byte row stagger; row-contiguous
+// tiles are transposed in registers (each thread holds a VEC-row block).  Two
+// LDS buffers, one barrier per chunk: the global loads of chunk c+1 are in
+// flight while chunk c is multiplied.
 #include "t3_common.h"
+
+#define T3_GLOBAL __attribute__((address_space(1)))
 
 namespace t3 {
 
-enum LoadMode : int { LOAD_ROWCONT = 0, LOAD_KCONT = 1, LOAD_SCALAR = 2 };
-
 template <int PREC> struct PrecTraits;
-template <> struct PrecTraits<PREC_F64> { using T = double; using Acc = double; static constexpr int BK = 16; static constexpr int VEC = 2; };
-template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; static constexpr int BK = 32; static constexpr int VEC = 4; };
-template <> struct PrecTraits<PREC_BF16X3> { using T = float; using Acc = float; static constexpr int BK = 32; static constexpr int VEC = 4; };
+template <> struct PrecTraits<PREC_F64> { using T = double; using Acc = double; using V = d2v; static constexpr int BK = 16; static constexpr int VEC = 2; };
+template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
+template <> struct PrecTraits<PREC_BF16X3> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 
-template <class T, int VEC> struct VecT;
-template <> struct VecT<double, 2> { using type = double2; };
-template <> struct VecT<float, 4> { using type = float4; };
+typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 
-// Register stage of one operand tile (ROWS x BK) loaded from global memory.
-template <class T, int ROWS, int BK, int VEC>
+// Register stage of one operand tile (ROWS x BK elements, NE per thread).
+//   KCONT  : thread holds NV vectors, each VEC consecutive k of one row
+//            (thread t: k-vector t % (BK/VEC), rows t / (BK/VEC) + q * RSTEP)
+//   ROWCONT: thread holds a VEC-row x NV-k block, one vector per k
+//            (thread t: row block t % RB, k block t / RB)
+template <class T, class V, int ROWS, int BK, int VEC>
 struct Stage {
-  static constexpr int NE = ROWS * BK / 256;   // elements per thread
-  static constexpr int NV = NE / VEC;          // vectors per thread
-  // tiles too small for one vector per thread are always loaded element-wise
-  static constexpr bool CANVEC = NE >= VEC && (NE % VEC) == 0;
-  static_assert(NE >= 1 && (ROWS * BK) % 256 == 0, "tile must give each thread an element");
-  T r[NE];
+  static constexpr int NE = ROWS * BK / 256;
+  static constexpr int NV = NE / VEC;
+  static constexpr int KV = BK / VEC;         // k-vectors per row (KCONT)
+  static constexpr int RSTEP = 256 / KV;      // rows per q step (KCONT)
+  static constexpr int RB = ROWS / VEC;       // row blocks (ROWCONT)
+  static_assert(NV >= 1 && NE % VEC == 0 && RB * (BK / NV) == 256, "tile must give each thread whole vectors");
+  V r[NV];
+  int mode, rows_valid;
+  float scale;
 
-  __device__ __forceinline__ void load(int mode, const T* __restrict__ base, long long s_row,
-                                       long long s_k, int rows_valid, float scale) {
+  __device__ __forceinline__ void load(int mode_, const T* base, long long s_row, long long s_k,
+                                       int rows_valid_, float scale_) {
+    mode = mode_;
+    rows_valid = rows_valid_;
+    scale = scale_;
+    const T3_GLOBAL T* g = (const T3_GLOBAL T*)base;
     const int t = threadIdx.x;
-    if (!CANVEC) mode = LOAD_SCALAR;
     if (mode == LOAD_ROWCONT) {
-      using V = typename VecT<T, VEC>::type;
+      const int rb = min(t % RB, (rows_valid - 1) / VEC), kb = t / RB;
+      const T3_GLOBAL T* p = g + (long long)rb * VEC + (long long)(kb * NV) * s_k;
 #pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int k = id / (ROWS / VEC), rv = id % (ROWS / VEC);
-        V x;
-        if (rv * VEC < rows_valid) x = *reinterpret_cast<const V*>(base + k * s_k + rv * VEC);
-        else x = V{};
-        const T* px = reinterpret_cast<const T*>(&x);
-#pragma unroll
-        for (int e = 0; e < VEC; e++) r[q * VEC + e] = px[e] * (T)scale;
-      }
-    } else if (mode == LOAD_KCONT) {
-      using V = typename VecT<T, VEC>::type;
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int kv = id % (BK / VEC), row = id / (BK / VEC);
-        V x;
-        if (row < rows_valid) x = *reinterpret_cast<const V*>(base + row * s_row + kv * VEC);
-        else x = V{};
-        const T* px = reinterpret_cast<const T*>(&x);
-#pragma unroll
-        for (int e = 0; e < VEC; e++) r[q * VEC + e] = px[e] * (T)scale;
-      }
+      for (int j = 0; j < NV; j++) r[j] = *(const T3_GLOBAL V*)(p + j * s_k);
     } else {
+      const int kv = t % KV, row0 = t / KV;
 #pragma unroll
-      for (int q = 0; q < NE; q++) {
-        const int id = t + 256 * q;
-        const int row = id % ROWS, k = id / ROWS;
-        r[q] = (row < rows_valid) ? base[row * s_row + k * s_k] * (T)scale : (T)0;
+      for (int q = 0; q < NV; q++) {
+        const int row = min(row0 + q * RSTEP, rows_valid - 1);
+        r[q] = *(const T3_GLOBAL V*)(g + (long long)row * s_row + kv * VEC);
       }
     }
   }
 
-  // LDS layout [k][row] (f64 / f32 MFMA operands)
-  template <int LD>
-  __device__ __forceinline__ void store_krow(int mode, T* __restrict__ lds) const {
-    const int t = threadIdx.x;
-    if (!CANVEC) mode = LOAD_SCALAR;
-    if (mode == LOAD_ROWCONT) {
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int k = id / (ROWS / VEC), rv = id % (ROWS / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; e++) lds[k * LD + rv * VEC + e] = r[q * VEC + e];
-      }
-    } else if (mode == LOAD_KCONT) {
-#pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int kv = id % (BK / VEC), row = id / (BK / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; e++) lds[(kv * VEC + e) * LD + row] = r[q * VEC + e];
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < NE; q++) {
-        const int id = t + 256 * q;
-        const int row = id % ROWS, k = id / ROWS;
-        lds[k * LD + row] = r[q];
-      }
-    }
-  }
-
-  // K-contiguous tiles go to LDS as [row][k] (rows of LDK elements); all
-  // other tiles as [k][row] (rows of ROWS elements, XOR-swizzled by
-  // swz(k) on 16-element groups), so that neither store needs a transpose.
+  // LDS [row][k], row pitch LDK elements
   template <int LDK>
-  __device__ __forceinline__ void store_rk(T* __restrict__ lds) const {
+  __device__ __forceinline__ void store(T* __restrict__ lds) const {
     const int t = threadIdx.x;
-    using V = typename VecT<T, VEC>::type;
+    const T s = (T)scale;
+    if (mode == LOAD_ROWCONT) {
+      const int rb = t % RB, kb = t / RB;
+      const bool ok = rb * VEC < rows_valid;
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-      const int id = t + 256 * q;
-      const int kv = id % (BK / VEC), row = id / (BK / VEC);
-      *reinterpret_cast<V*>(&lds[row * LDK + kv * VEC]) = *reinterpret_cast<const V*>(&r[q * VEC]);
-    }
-  }
-  __device__ __forceinline__ static int swz(int k) {
-    // f32 [k][row] reads touch rows k = g*KG + ks for the four lane groups g;
-    // flipping column bit 4 on odd g spreads them over both bank halves
-    return (sizeof(T) == 4) ? (((k / (BK / 4)) & 1) << 4) : 0;
-  }
-  __device__ __forceinline__ void store_kr(int mode, T* __restrict__ lds) const {
-    const int t = threadIdx.x;
-    if (mode == LOAD_ROWCONT && CANVEC) {
-      using V = typename VecT<T, VEC>::type;
+      for (int e = 0; e < VEC; e++) {
+        T* dst = lds + (rb * VEC + e) * LDK + kb * NV;
+        if constexpr (NV == 2 || (NV == 4 && sizeof(T) == 4)) {
+          typedef T tnv __attribute__((ext_vector_type(NV)));
+          tnv x;
 #pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int k = id / (ROWS / VEC), rv = id % (ROWS / VEC);
-        *reinterpret_cast<V*>(&lds[k * ROWS + ((rv * VEC) ^ swz(k))]) = *reinterpret_cast<const V*>(&r[q * VEC]);
+          for (int j = 0; j < NV; j++) x[j] = r[j][e];
+          *reinterpret_cast<tnv*>(dst) = ok ? x * s : tnv{};
+        } else {
+#pragma unroll
+          for (int j = 0; j < NV; j++) dst[j] = ok ? r[j][e] * s : (T)0;
+        }
       }
     } else {
+      const int kv = t % KV, row0 = t / KV;
 #pragma unroll
-      for (int q = 0; q < NE; q++) {
-        const int id = t + 256 * q;
-        const int row = id % ROWS, k = id / ROWS;
-        lds[k * ROWS + (row ^ swz(k))] = r[q];
+      for (int q = 0; q < NV; q++) {
+        const int row = row0 + q * RSTEP;
+        *reinterpret_cast<V*>(lds + row * LDK + kv * VEC) = (row < rows_valid) ? r[q] * s : V{};
       }
     }
   }
 
-  // LDS layout [row][k] as split bf16 pairs (bf16x3 MFMA operands)
+  // LDS [row][k] as split bf16 pairs x = hi + lo (FP32 operands only)
   template <int LDK>
-  __device__ __forceinline__ void store_rowk_split(int mode, __bf16* __restrict__ hi,
-                                                   __bf16* __restrict__ lo) const {
+  __device__ __forceinline__ void store_split(__bf16* __restrict__ hi, __bf16* __restrict__ lo) const {
     const int t = threadIdx.x;
-    if (!CANVEC) mode = LOAD_SCALAR;
-    auto put = [&](int row, int k, float x) {
-      const __bf16 h = (__bf16)x;
-      const __bf16 l = (__bf16)(x - (float)h);
-      hi[row * LDK + k] = h;
-      lo[row * LDK + k] = l;
-    };
+    const float s = scale;
     if (mode == LOAD_ROWCONT) {
+      const int rb = t % RB, kb = t / RB;
+      const bool ok = rb * VEC < rows_valid;
 #pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int k = id / (ROWS / VEC), rv = id % (ROWS / VEC);
+      for (int e = 0; e < VEC; e++) {
+        const int off = (rb * VEC + e) * LDK + kb * NV;
+        if constexpr (NV == 4) {
+          f4v x = {r[0][e], r[1][e], r[2][e], r[3][e]};
+          x = ok ? x * s : f4v{0, 0, 0, 0};
+          bf4v h, l;
 #pragma unroll
-        for (int e = 0; e < VEC; e++) put(rv * VEC + e, k, (float)r[q * VEC + e]);
-      }
-    } else if (mode == LOAD_KCONT) {
+          for (int j = 0; j < 4; j++) {
+            h[j] = (__bf16)x[j];
+            l[j] = (__bf16)(x[j] - (float)h[j]);
+          }
+          *reinterpret_cast<bf4v*>(hi + off) = h;
+          *reinterpret_cast<bf4v*>(lo + off) = l;
+        } else {
 #pragma unroll
-      for (int q = 0; q < NV; q++) {
-        const int id = t + 256 * q;
-        const int kv = id % (BK / VEC), row = id / (BK / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; e++) put(row, kv * VEC + e, (float)r[q * VEC + e]);
+          for (int j = 0; j < NV; j++) {
+            const float x = ok ? (float)r[j][e] * s : 0.f;
+            const __bf16 h = (__bf16)x;
+            hi[off + j] = h;
+            lo[off + j] = (__bf16)(x - (float)h);
+          }
+        }
       }
     } else {
+      const int kv = t % KV, row0 = t / KV;
 #pragma unroll
-      for (int q = 0; q < NE; q++) {
-        const int id = t + 256 * q;
-        const int row = id % ROWS, k = id / ROWS;
-        put(row, k, (float)r[q]);
+      for (int q = 0; q < NV; q++) {
+        const int row = row0 + q * RSTEP;
+        const f4v x = (row < rows_valid) ? r[q] * s : f4v{0, 0, 0, 0};
+        bf4v h, l;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          h[j] = (__bf16)x[j];
+          l[j] = (__bf16)(x[j] - (float)h[j]);
+        }
+        *reinterpret_cast<bf4v*>(hi + row * LDK + kv * VEC) = h;
+        *reinterpret_cast<bf4v*>(lo + row * LDK + kv * VEC) = l;
       }
     }
   }
@@ -221,17 +188,17 @@ __device__ __forceinline__ void write_out(const GemmProb& P, const TileCoord& tc
                                           Acc v) {
   if (m >= P.M || n >= P.N) return;
   if (P.mode == GEMM_PARTIAL) {
-    Acc* out = reinterpret_cast<Acc*>(P.P);
+    T3_GLOBAL Acc* out = (T3_GLOBAL Acc*)P.P;
     out[((long long)(tc.split * P.nb + tc.b) * P.M + m) * P.N + n] = v;
     return;
   }
   const long long off = (long long)tc.b * P.c_sb + (long long)m * P.c_sm + (long long)n * P.c_sn;
   const double x = P.alpha * (double)v;
   if (P.c_is_f64) {
-    double* c = reinterpret_cast<double*>(P.C) + off;
+    T3_GLOBAL double* c = (T3_GLOBAL double*)P.C + off;
     *c = (P.mode == GEMM_ACCUM) ? (*c + x) : x;
   } else {
-    float* c = reinterpret_cast<float*>(P.C) + off;
+    T3_GLOBAL float* c = (T3_GLOBAL float*)P.C + off;
     *c = (P.mode == GEMM_ACCUM) ? (float)((double)*c + x) : (float)x;
   }
 }
@@ -243,6 +210,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   using TR = PrecTraits<PREC>;
   using T = typename TR::T;
   using Acc = typename TR::Acc;
+  using V = typename TR::V;
   using Frag = typename AccFrag<Acc>::type;
   constexpr int BK = TR::BK, VEC = TR::VEC;
   constexpr int WGN = 4 / WGM;
@@ -267,6 +235,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm0 = (wave % WGM) * WM, wn0 = (wave / WGM) * WN;
+  const int g = lane >> 4, l16 = lane & 15;
 
   Frag acc[FM][FN];
 #pragma unroll
@@ -274,132 +243,93 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
 #pragma unroll
     for (int j = 0; j < FN; j++) acc[i][j] = Frag{0, 0, 0, 0};
 
-  Stage<T, BM, BK, VEC> sa;
-  Stage<T, BN, BK, VEC> sb;
+  Stage<T, V, BM, BK, VEC> sa;
+  Stage<T, V, BN, BK, VEC> sb;
 
   // segment cursor
   int sidx = P.seg0;
-  auto seg_of = [&](int c) -> const GemmSeg& {
+  auto load_chunk = [&](int c) {
     while (sidx + 1 < P.seg0 + P.nseg && segs[sidx + 1].chunk0 <= c) sidx++;
-    return segs[sidx];
-  };
-  auto load_chunk = [&](int c, int& amode, int& bmode) {
-    const GemmSeg& S = seg_of(c);
+    const GemmSeg& S = segs[sidx];
     const long long kk = (long long)(c - S.chunk0) * BK;
     const long long k1 = kk / S.K2, k2 = kk % S.K2;
     const T* A = reinterpret_cast<const T*>(S.A) + tc.b * S.a_sb + m0 * S.a_sm + k1 * S.a_sk1 + k2 * S.a_sk2;
     const T* B = reinterpret_cast<const T*>(S.B) + tc.b * S.b_sb + n0 * S.b_sn + k1 * S.b_sk1 + k2 * S.b_sk2;
-    amode = S.a_sm == 1 ? LOAD_ROWCONT : (S.a_sk2 == 1 ? LOAD_KCONT : LOAD_SCALAR);
-    bmode = (S.b_sn == 1 && (P.N % VEC) == 0) ? LOAD_ROWCONT : (S.b_sk2 == 1 ? LOAD_KCONT : LOAD_SCALAR);
-    if (amode == LOAD_ROWCONT && (P.M % VEC) != 0) amode = (S.a_sk2 == 1 ? LOAD_KCONT : LOAD_SCALAR);
-    sa.load(amode, A, S.a_sm, S.a_sk2, rows_a, S.scale);
-    sb.load(bmode, B, S.b_sn, S.b_sk2, rows_b, 1.0f);
+    sa.load(S.a_mode, A, S.a_sm, S.a_sk2, rows_a, S.scale);
+    sb.load(S.b_mode, B, S.b_sn, S.b_sk2, rows_b, 1.0f);
   };
 
   if constexpr (PREC != PREC_BF16X3) {
     // K is consumed in a permuted order: in step ks the lanes of group
     // g = lane/16 multiply k = g*KG + ks (the sum over k is order-free), so a
-    // lane's KG values of a [row][k] tile are contiguous (two 16-byte reads).
+    // lane's KG values of a row are contiguous in LDS (two 16-byte reads).
     constexpr int KG = BK / 4;
-    constexpr int LDK = BK + 16 / (int)sizeof(T);   // [row][k] row pitch, staggered by 16 B
-    constexpr int SZA = (BM * LDK > BK * BM) ? BM * LDK : BK * BM;
-    constexpr int SZB = (BN * LDK > BK * BN) ? BN * LDK : BK * BN;
-    __shared__ __attribute__((aligned(16))) T lds_a[2][SZA];
-    __shared__ __attribute__((aligned(16))) T lds_b[2][SZB];
-    using SA = Stage<T, BM, BK, VEC>;
-    using SB = Stage<T, BN, BK, VEC>;
-    int amode = 0, bmode = 0;
-    bool rk_a[2] = {false, false}, rk_b[2] = {false, false};
-    auto store = [&](int buf) {
-      rk_a[buf] = (amode == LOAD_KCONT) && SA::CANVEC;
-      rk_b[buf] = (bmode == LOAD_KCONT) && SB::CANVEC;
-      if (rk_a[buf]) sa.template store_rk<LDK>(lds_a[buf]); else sa.store_kr(amode, lds_a[buf]);
-      if (rk_b[buf]) sb.template store_rk<LDK>(lds_b[buf]); else sb.store_kr(bmode, lds_b[buf]);
-    };
+    constexpr int LDK = BK + 16 / (int)sizeof(T);   // row pitch, staggered by 16 B
+    __shared__ __attribute__((aligned(16))) T lds_a[2][BM * LDK];
+    __shared__ __attribute__((aligned(16))) T lds_b[2][BN * LDK];
     if (c_begin < c_end) {
-      load_chunk(c_begin, amode, bmode);
-      store(0);
+      load_chunk(c_begin);
+      sa.template store<LDK>(lds_a[0]);
+      sb.template store<LDK>(lds_b[0]);
     }
     __syncthreads();
     int cur = 0;
-    const int g = lane >> 4, l16 = lane & 15;
     for (int c = c_begin; c < c_end; c++) {
       const bool more = c + 1 < c_end;
-      if (more) load_chunk(c + 1, amode, bmode);
-      const T* la = lds_a[cur];
-      const T* lb = lds_b[cur];
-      T av[FM][KG], bv[FN][KG];
-      if (rk_a[cur]) {
+      if (more) load_chunk(c + 1);
+      const T* la = lds_a[cur] + (wm0 + l16) * LDK + g * KG;
+      const T* lb = lds_b[cur] + (wn0 + l16) * LDK + g * KG;
+      V av[FM][KG / VEC], bv[FN][KG / VEC];
 #pragma unroll
-        for (int i = 0; i < FM; i++) {
-          const T* src = la + (wm0 + i * 16 + l16) * LDK + g * KG;
+      for (int i = 0; i < FM; i++)
 #pragma unroll
-          for (int ks = 0; ks < KG; ks++) av[i][ks] = src[ks];
-        }
-      } else {
+        for (int u = 0; u < KG / VEC; u++) av[i][u] = *reinterpret_cast<const V*>(la + i * 16 * LDK + u * VEC);
 #pragma unroll
-        for (int i = 0; i < FM; i++)
+      for (int j = 0; j < FN; j++)
 #pragma unroll
-          for (int ks = 0; ks < KG; ks++) {
-            const int k = g * KG + ks;
-            av[i][ks] = la[k * BM + ((wm0 + i * 16 + l16) ^ SA::swz(k))];
-          }
-      }
-      if (rk_b[cur]) {
-#pragma unroll
-        for (int j = 0; j < FN; j++) {
-          const T* src = lb + (wn0 + j * 16 + l16) * LDK + g * KG;
-#pragma unroll
-          for (int ks = 0; ks < KG; ks++) bv[j][ks] = src[ks];
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < FN; j++)
-#pragma unroll
-          for (int ks = 0; ks < KG; ks++) {
-            const int k = g * KG + ks;
-            bv[j][ks] = lb[k * BN + ((wn0 + j * 16 + l16) ^ SB::swz(k))];
-          }
-      }
+        for (int u = 0; u < KG / VEC; u++) bv[j][u] = *reinterpret_cast<const V*>(lb + j * 16 * LDK + u * VEC);
 #pragma unroll
       for (int ks = 0; ks < KG; ks++)
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
           for (int j = 0; j < FN; j++) {
-            if constexpr (PREC == PREC_F64) acc[i][j] = mfma_f64(av[i][ks], bv[j][ks], acc[i][j]);
-            else acc[i][j] = mfma_f32(av[i][ks], bv[j][ks], acc[i][j]);
+            const T a = av[i][ks / VEC][ks % VEC], b = bv[j][ks / VEC][ks % VEC];
+            if constexpr (PREC == PREC_F64) acc[i][j] = mfma_f64(a, b, acc[i][j]);
+            else acc[i][j] = mfma_f32(a, b, acc[i][j]);
           }
-      if (more) store(cur ^ 1);
+      if (more) {
+        sa.template store<LDK>(lds_a[cur ^ 1]);
+        sb.template store<LDK>(lds_b[cur ^ 1]);
+      }
       __syncthreads();
       cur ^= 1;
     }
   } else {
-    constexpr int LDK = BK + 8;
-    __shared__ __bf16 lah[2][BM * LDK], lal[2][BM * LDK];
-    __shared__ __bf16 lbh[2][BN * LDK], lbl[2][BN * LDK];
-    int amode = 0, bmode = 0;
+    constexpr int LDK = BK + 8;   // bf16 elements; 80-byte rows keep 16-byte reads aligned
+    __shared__ __attribute__((aligned(16))) __bf16 lah[2][BM * LDK], lal[2][BM * LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 lbh[2][BN * LDK], lbl[2][BN * LDK];
     if (c_begin < c_end) {
-      load_chunk(c_begin, amode, bmode);
-      sa.template store_rowk_split<LDK>(amode, lah[0], lal[0]);
-      sb.template store_rowk_split<LDK>(bmode, lbh[0], lbl[0]);
+      load_chunk(c_begin);
+      sa.template store_split<LDK>(lah[0], lal[0]);
+      sb.template store_split<LDK>(lbh[0], lbl[0]);
     }
     __syncthreads();
     int cur = 0;
+    const int ko = g * 8;
     for (int c = c_begin; c < c_end; c++) {
       const bool more = c + 1 < c_end;
-      if (more) load_chunk(c + 1, amode, bmode);
-      const int ko = (lane >> 4) * 8;
+      if (more) load_chunk(c + 1);
       bf8v ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) {
-        const int row = wm0 + i * 16 + (lane & 15);
+        const int row = wm0 + i * 16 + l16;
         ah[i] = *reinterpret_cast<const bf8v*>(&lah[cur][row * LDK + ko]);
         al[i] = *reinterpret_cast<const bf8v*>(&lal[cur][row * LDK + ko]);
       }
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int row = wn0 + j * 16 + (lane & 15);
+        const int row = wn0 + j * 16 + l16;
         bh[j] = *reinterpret_cast<const bf8v*>(&lbh[cur][row * LDK + ko]);
         bl[j] = *reinterpret_cast<const bf8v*>(&lbl[cur][row * LDK + ko]);
       }
@@ -412,8 +342,8 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
           acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         }
       if (more) {
-        sa.template store_rowk_split<LDK>(amode, lah[cur ^ 1], lal[cur ^ 1]);
-        sb.template store_rowk_split<LDK>(bmode, lbh[cur ^ 1], lbl[cur ^ 1]);
+        sa.template store_split<LDK>(lah[cur ^ 1], lal[cur ^ 1]);
+        sb.template store_split<LDK>(lbh[cur ^ 1], lbl[cur ^ 1]);
       }
       __syncthreads();
       cur ^= 1;
@@ -429,9 +359,9 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     for (int j = 0; j < FN; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = (PREC == PREC_F64) ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
+        const int row = (PREC == PREC_F64) ? (g + 4 * r) : (4 * g + r);
         const int m = m0 + wm0 + i * 16 + row;
-        const int n = n0 + wn0 + j * 16 + (lane & 15);
+        const int n = n0 + wn0 + j * 16 + l16;
         write_out<Acc>(P, tc, m, n, acc[i][j][r]);
       }
 }
@@ -454,46 +384,40 @@ reduce_partials(const ReduceProb* __restrict__ probs, int nprob, long long total
   const int n = (int)(local % R.N);
   const int m = (int)((local / R.N) % R.M);
   const int b = (int)(local / ((long long)R.N * R.M));
-  const Acc* P = reinterpret_cast<const Acc*>(R.P);
+  const T3_GLOBAL Acc* P = (const T3_GLOBAL Acc*)R.P;
   double s = 0.0;
   for (int sp = 0; sp < R.ksplit; sp++) s += (double)P[sp * per + local];
   s *= R.alpha;
   const long long off = (long long)b * R.c_sb + (long long)m * R.c_sm + (long long)n * R.c_sn;
   if (R.c_is_f64) {
-    double* c = reinterpret_cast<double*>(R.C) + off;
+    T3_GLOBAL double* c = (T3_GLOBAL double*)R.C + off;
     *c = R.accumulate ? *c + s : s;
   } else {
-    float* c = reinterpret_cast<float*>(R.C) + off;
+    T3_GLOBAL float* c = (T3_GLOBAL float*)R.C + off;
     *c = R.accumulate ? (float)((double)*c + s) : (float)s;
   }
 }
-
-// Host-side launch helpers (C linkage inside the library only).
-#define T3_GEMM_INST(PREC, BM, BN, WGM)                                                        \
-  template __global__ void gemm_grouped<PREC, BM, BN, WGM>(const GemmProb*, int, const GemmSeg*);
 
 }  // namespace t3
 
 using namespace t3;
 
-// tile config ids: 0 -> 64x64 (2x2 waves), 1 -> 128x128 (2x2), 2 -> 64x16 (4x1),
-//                  3 -> 64x32 (2x2), 4 -> 32x64 (2x2)
+// tile config ids: 0 -> 64x64 (2x2 waves), 1 -> 128x128 (2x2), 3 -> 64x32 (2x2), 4 -> 32x64 (2x2)
 extern "C" int t3_gemm_tile_m(int cfg) { return cfg == 1 ? 128 : (cfg == 4 ? 32 : 64); }
-extern "C" int t3_gemm_tile_n(int cfg) {
-  return cfg == 1 ? 128 : (cfg == 2 ? 16 : (cfg == 3 ? 32 : 64));
-}
+extern "C" int t3_gemm_tile_n(int cfg) { return cfg == 1 ? 128 : (cfg == 3 ? 32 : 64); }
 extern "C" int t3_gemm_chunk(int prec) { return prec == PREC_F64 ? 16 : 32; }
+extern "C" int t3_gemm_vec(int prec) { return prec == PREC_F64 ? 2 : 4; }
 
 extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob,
                                      const GemmSeg* segs, long long nblocks, hipStream_t stream) {
   if (nblocks <= 0) return hipSuccess;
+  if (cfg != 0 && cfg != 1 && cfg != 3 && cfg != 4) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblocks), block(256);
 #define T3_DISPATCH(PR)                                                                                    \
   if (cfg == 0) hipLaunchKernelGGL((gemm_grouped<PR, 64, 64, 2>), grid, block, 0, stream, probs, nprob, segs); \
   else if (cfg == 1) hipLaunchKernelGGL((gemm_grouped<PR, 128, 128, 2>), grid, block, 0, stream, probs, nprob, segs); \
   else if (cfg == 3) hipLaunchKernelGGL((gemm_grouped<PR, 64, 32, 2>), grid, block, 0, stream, probs, nprob, segs); \
-  else if (cfg == 4) hipLaunchKernelGGL((gemm_grouped<PR, 32, 64, 2>), grid, block, 0, stream, probs, nprob, segs); \
-  else hipLaunchKernelGGL((gemm_grouped<PR, 64, 16, 4>), grid, block, 0, stream, probs, nprob, segs);
+  else hipLaunchKernelGGL((gemm_grouped<PR, 32, 64, 2>), grid, block, 0, stream, probs, nprob, segs);
   if (prec == PREC_F64) { T3_DISPATCH(PREC_F64) }
   else if (prec == PREC_F32) { T3_DISPATCH(PREC_F32) }
   else { T3_DISPATCH(PREC_BF16X3) }
