@@ -326,7 +326,10 @@ struct Ctx {
                         (s.b_mode == LOAD_ROWCONT ? P.N % vec == 0 : s.b_sk2 == 1) &&
                         a_row % vec == 0 && b_row % vec == 0 && s.a_sb % vec == 0 && s.b_sb % vec == 0 &&
                         s.a_sk1 % vec == 0 && s.b_sk1 % vec == 0 && s.K2 % gb.BK == 0 &&
-                        aligned(s.A) && aligned(s.B);
+                        aligned(s.A) && aligned(s.B) &&
+                        // per-thread tile offsets are 32-bit byte offsets
+                        ((BM - 1) * s.a_sm + (gb.BK - 1) * s.a_sk2 + vec) * (long long)d.elt < (1LL << 32) &&
+                        ((BN - 1) * s.b_sn + (gb.BK - 1) * s.b_sk2 + vec) * (long long)d.elt < (1LL << 32);
         if (!ok && plan_error.empty()) plan_error = "GEMM operand is neither row- nor k-contiguous, or misaligned";
       }
     }

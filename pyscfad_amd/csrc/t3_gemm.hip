@@ -19,8 +19,10 @@
 // MFMAs of the previous chunk, scaled and with the clamped rows zeroed.  The
 // LDS layout is always [row][k] with a 16-byte row stagger; row-contiguous
 // tiles are transposed in registers (each thread holds a VEC-row block).  Two
-// LDS buffers, one barrier per chunk: the global loads of chunk c+1 are in
-// flight while chunk c is multiplied.
+// LDS buffers and two register stages, one barrier per chunk: the global
+// loads of chunk c+2 are in flight while chunk c is multiplied.
+#include <type_traits>
+
 #include "t3_common.h"
 
 #define T3_GLOBAL __attribute__((address_space(1)))
@@ -51,26 +53,35 @@ struct Stage {
   int mode, rows_valid;
   float scale;
 
-  __device__ __forceinline__ void load(int mode_, const T* base, long long s_row, long long s_k,
-                                       int rows_valid_, float scale_) {
-    mode = mode_;
-    rows_valid = rows_valid_;
-    scale = scale_;
-    const T3_GLOBAL T* g = (const T3_GLOBAL T*)base;
+  // Byte offsets of this thread's NV vectors from the tile origin (rows past
+  // the edge clamped to the last valid row).  They depend only on the
+  // segment's strides, so they are computed once per segment.
+  static __device__ __forceinline__ void offsets(int mode, long long s_row, long long s_k, int rows_valid,
+                                                 uint32_t* off) {
     const int t = threadIdx.x;
     if (mode == LOAD_ROWCONT) {
       const int rb = min(t % RB, (rows_valid - 1) / VEC), kb = t / RB;
-      const T3_GLOBAL T* p = g + (long long)rb * VEC + (long long)(kb * NV) * s_k;
 #pragma unroll
-      for (int j = 0; j < NV; j++) r[j] = *(const T3_GLOBAL V*)(p + j * s_k);
+      for (int j = 0; j < NV; j++)
+        off[j] = (uint32_t)(((long long)rb * VEC + (long long)(kb * NV + j) * s_k) * (long long)sizeof(T));
     } else {
       const int kv = t % KV, row0 = t / KV;
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         const int row = min(row0 + q * RSTEP, rows_valid - 1);
-        r[q] = *(const T3_GLOBAL V*)(g + (long long)row * s_row + kv * VEC);
+        off[q] = (uint32_t)(((long long)row * s_row + kv * VEC) * (long long)sizeof(T));
       }
     }
+  }
+
+  __device__ __forceinline__ void load(int mode_, const char* base, const uint32_t* off, int rows_valid_,
+                                       float scale_) {
+    mode = mode_;
+    rows_valid = rows_valid_;
+    scale = scale_;
+    const T3_GLOBAL char* g = (const T3_GLOBAL char*)base;
+#pragma unroll
+    for (int q = 0; q < NV; q++) r[q] = *(const T3_GLOBAL V*)(g + off[q]);
   }
 
   // LDS [row][k], row pitch LDK elements
@@ -203,9 +214,10 @@ __device__ __forceinline__ void write_out(const GemmProb& P, const TileCoord& tc
   }
 }
 
-// BM x BN tile, 4 waves arranged WGM x (4/WGM).
+// BM x BN tile, 4 waves arranged WGM x (4/WGM); register budget for two
+// waves per SIMD (two blocks per CU).
 template <int PREC, int BM, int BN, int WGM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __restrict__ segs) {
   using TR = PrecTraits<PREC>;
   using T = typename TR::T;
@@ -243,21 +255,69 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
 #pragma unroll
     for (int j = 0; j < FN; j++) acc[i][j] = Frag{0, 0, 0, 0};
 
-  Stage<T, V, BM, BK, VEC> sa;
-  Stage<T, V, BN, BK, VEC> sb;
+  // Two register stages: the global loads of chunk c+2 are issued while
+  // chunk c is multiplied, and chunk c+1 is written to LDS after it.
+  using SA = Stage<T, V, BM, BK, VEC>;
+  using SB = Stage<T, V, BN, BK, VEC>;
+  SA sa[2];
+  SB sb[2];
 
-  // segment cursor
-  int sidx = P.seg0;
-  auto load_chunk = [&](int c) {
-    while (sidx + 1 < P.seg0 + P.nseg && segs[sidx + 1].chunk0 <= c) sidx++;
-    const GemmSeg& S = segs[sidx];
+  // Segment walker: the chunks of this block are loaded in increasing order;
+  // the tile origins advance by a fixed step per chunk (and a carry when k2
+  // wraps into the next k1), the per-thread offsets change only between
+  // segments.
+  int si = P.seg0;
+  while (si + 1 < P.seg0 + P.nseg && segs[si + 1].chunk0 <= c_begin) si++;
+  const char *pa = nullptr, *pb = nullptr;
+  long long da2 = 0, db2 = 0, da1 = 0, db1 = 0;
+  int k2 = 0, K2 = 0, seg_end = 0, amode = 0, bmode = 0;
+  float scale = 1.0f;
+  uint32_t oa[SA::NV], ob[SB::NV];
+  auto seg_init = [&](int c) {
+    const GemmSeg& S = segs[si];
     const long long kk = (long long)(c - S.chunk0) * BK;
-    const long long k1 = kk / S.K2, k2 = kk % S.K2;
-    const T* A = reinterpret_cast<const T*>(S.A) + tc.b * S.a_sb + m0 * S.a_sm + k1 * S.a_sk1 + k2 * S.a_sk2;
-    const T* B = reinterpret_cast<const T*>(S.B) + tc.b * S.b_sb + n0 * S.b_sn + k1 * S.b_sk1 + k2 * S.b_sk2;
-    sa.load(S.a_mode, A, S.a_sm, S.a_sk2, rows_a, S.scale);
-    sb.load(S.b_mode, B, S.b_sn, S.b_sk2, rows_b, 1.0f);
+    const long long k1 = kk / S.K2;
+    k2 = (int)(kk % S.K2);
+    K2 = S.K2;
+    constexpr long long E = sizeof(T);
+    pa = reinterpret_cast<const char*>(S.A) + (tc.b * S.a_sb + m0 * S.a_sm + k1 * S.a_sk1 + k2 * S.a_sk2) * E;
+    pb = reinterpret_cast<const char*>(S.B) + (tc.b * S.b_sb + n0 * S.b_sn + k1 * S.b_sk1 + k2 * S.b_sk2) * E;
+    da2 = BK * S.a_sk2 * E;
+    db2 = BK * S.b_sk2 * E;
+    da1 = (S.a_sk1 - (long long)S.K2 * S.a_sk2) * E;
+    db1 = (S.b_sk1 - (long long)S.K2 * S.b_sk2) * E;
+    amode = S.a_mode;
+    bmode = S.b_mode;
+    scale = S.scale;
+    seg_end = (si + 1 < P.seg0 + P.nseg) ? segs[si + 1].chunk0 : P.nchunks;
+    SA::offsets(amode, S.a_sm, S.a_sk2, rows_a, oa);
+    SB::offsets(bmode, S.b_sn, S.b_sk2, rows_b, ob);
   };
+  int lc = c_begin;   // chunk the walker points at
+  if (c_begin < c_end) seg_init(c_begin);
+  // load the walker's chunk, then step to the next one (the last chunk is
+  // loaded again at the tail, which keeps the loop free of branches)
+  auto load_next = [&](SA& xa, SB& xb) {
+    xa.load(amode, pa, oa, rows_a, scale);
+    xb.load(bmode, pb, ob, rows_b, 1.0f);
+    if (lc + 1 < c_end) {
+      lc++;
+      if (lc == seg_end) {
+        si++;
+        seg_init(lc);
+      } else {
+        k2 += BK;
+        pa += da2;
+        pb += db2;
+        if (k2 == K2) {
+          k2 = 0;
+          pa += da1;
+          pb += db1;
+        }
+      }
+    }
+  };
+  const int nch = c_end - c_begin;
 
   if constexpr (PREC != PREC_BF16X3) {
     // K is consumed in a permuted order: in step ks the lanes of group
@@ -267,18 +327,19 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     constexpr int LDK = BK + 16 / (int)sizeof(T);   // row pitch, staggered by 16 B
     __shared__ __attribute__((aligned(16))) T lds_a[2][BM * LDK];
     __shared__ __attribute__((aligned(16))) T lds_b[2][BN * LDK];
-    if (c_begin < c_end) {
-      load_chunk(c_begin);
-      sa.template store<LDK>(lds_a[0]);
-      sb.template store<LDK>(lds_b[0]);
+    if (nch > 0) {
+      load_next(sa[0], sb[0]);
+      load_next(sa[1], sb[1]);
+      sa[0].template store<LDK>(lds_a[0]);
+      sb[0].template store<LDK>(lds_b[0]);
     }
     __syncthreads();
-    int cur = 0;
-    for (int c = c_begin; c < c_end; c++) {
-      const bool more = c + 1 < c_end;
-      if (more) load_chunk(c + 1);
-      const T* la = lds_a[cur] + (wm0 + l16) * LDK + g * KG;
-      const T* lb = lds_b[cur] + (wn0 + l16) * LDK + g * KG;
+    // chunk c_begin + it sits in LDS buffer and register stage it & 1
+    auto step = [&](int it, auto par) {
+      constexpr int p = decltype(par)::value;
+      load_next(sa[p], sb[p]);
+      const T* la = lds_a[p] + (wm0 + l16) * LDK + g * KG;
+      const T* lb = lds_b[p] + (wn0 + l16) * LDK + g * KG;
       V av[FM][KG / VEC], bv[FN][KG / VEC];
 #pragma unroll
       for (int i = 0; i < FM; i++)
@@ -298,40 +359,46 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
             if constexpr (PREC == PREC_F64) acc[i][j] = mfma_f64(a, b, acc[i][j]);
             else acc[i][j] = mfma_f32(a, b, acc[i][j]);
           }
-      if (more) {
-        sa.template store<LDK>(lds_a[cur ^ 1]);
-        sb.template store<LDK>(lds_b[cur ^ 1]);
+      if (it + 1 < nch) {
+        sa[p ^ 1].template store<LDK>(lds_a[p ^ 1]);
+        sb[p ^ 1].template store<LDK>(lds_b[p ^ 1]);
       }
       __syncthreads();
-      cur ^= 1;
+    };
+    // straight-line pairs keep the register stages in place across the loop
+    int it = 0;
+    for (; it + 1 < nch; it += 2) {
+      step(it, std::integral_constant<int, 0>{});
+      step(it + 1, std::integral_constant<int, 1>{});
     }
+    if (it < nch) step(it, std::integral_constant<int, 0>{});
   } else {
     constexpr int LDK = BK + 8;   // bf16 elements; 80-byte rows keep 16-byte reads aligned
     __shared__ __attribute__((aligned(16))) __bf16 lah[2][BM * LDK], lal[2][BM * LDK];
     __shared__ __attribute__((aligned(16))) __bf16 lbh[2][BN * LDK], lbl[2][BN * LDK];
-    if (c_begin < c_end) {
-      load_chunk(c_begin);
-      sa.template store_split<LDK>(lah[0], lal[0]);
-      sb.template store_split<LDK>(lbh[0], lbl[0]);
+    if (nch > 0) {
+      load_next(sa[0], sb[0]);
+      load_next(sa[1], sb[1]);
+      sa[0].template store_split<LDK>(lah[0], lal[0]);
+      sb[0].template store_split<LDK>(lbh[0], lbl[0]);
     }
     __syncthreads();
-    int cur = 0;
     const int ko = g * 8;
-    for (int c = c_begin; c < c_end; c++) {
-      const bool more = c + 1 < c_end;
-      if (more) load_chunk(c + 1);
+    auto step = [&](int it, auto par) {
+      constexpr int p = decltype(par)::value;
+      load_next(sa[p], sb[p]);
       bf8v ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) {
         const int row = wm0 + i * 16 + l16;
-        ah[i] = *reinterpret_cast<const bf8v*>(&lah[cur][row * LDK + ko]);
-        al[i] = *reinterpret_cast<const bf8v*>(&lal[cur][row * LDK + ko]);
+        ah[i] = *reinterpret_cast<const bf8v*>(&lah[p][row * LDK + ko]);
+        al[i] = *reinterpret_cast<const bf8v*>(&lal[p][row * LDK + ko]);
       }
 #pragma unroll
       for (int j = 0; j < FN; j++) {
         const int row = wn0 + j * 16 + l16;
-        bh[j] = *reinterpret_cast<const bf8v*>(&lbh[cur][row * LDK + ko]);
-        bl[j] = *reinterpret_cast<const bf8v*>(&lbl[cur][row * LDK + ko]);
+        bh[j] = *reinterpret_cast<const bf8v*>(&lbh[p][row * LDK + ko]);
+        bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][row * LDK + ko]);
       }
 #pragma unroll
       for (int i = 0; i < FM; i++)
@@ -341,13 +408,19 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
           acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
           acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         }
-      if (more) {
-        sa.template store_split<LDK>(lah[cur ^ 1], lal[cur ^ 1]);
-        sb.template store_split<LDK>(lbh[cur ^ 1], lbl[cur ^ 1]);
+      if (it + 1 < nch) {
+        sa[p ^ 1].template store_split<LDK>(lah[p ^ 1], lal[p ^ 1]);
+        sb[p ^ 1].template store_split<LDK>(lbh[p ^ 1], lbl[p ^ 1]);
       }
       __syncthreads();
-      cur ^= 1;
+    };
+    // straight-line pairs keep the register stages in place across the loop
+    int it = 0;
+    for (; it + 1 < nch; it += 2) {
+      step(it, std::integral_constant<int, 0>{});
+      step(it + 1, std::integral_constant<int, 1>{});
     }
+    if (it < nch) step(it, std::integral_constant<int, 0>{});
   }
 
   // epilogue: 16x16 fragment layout of gfx950 MFMA results
