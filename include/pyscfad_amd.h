@@ -113,7 +113,8 @@ const char *pyscfad_amd_last_error(void);
  * that stream before returning.  pyscfad_amd_last_timing fills up to n
  * entries of device milliseconds and launch counts of the last call, by
  * kernel family: 0 grouped MFMA GEMM, 1 orbit epilogue (+ its reduction),
- * 2 V-term adjoint kernels, 3 split-K reductions, 4 (unused), and family 0
+ * 2 V-term adjoint kernels, 3 split-K reductions, 4 per-call setup (accumulator
+ * clears, input re-layout, cotangent write-back), and family 0
  * split by contraction: 5 W build, 6 ovvv adjoint, 7 t2 adjoint (particle
  * term), 8 ovoo adjoint, 9 t2 adjoint (hole term); returns the count filled. */
 void pyscfad_amd_set_timing(int enable);

@@ -77,11 +77,20 @@ def num_triples(nocc):
     return _lib.load().pyscfad_amd_ccsd_t_num_triples(nocc)
 
 
+def _ws(workspace):
+    """(pointer, bytes) of a caller-provided device workspace (a torch tensor), or (None, 0)."""
+    if workspace is None:
+        return None, 0
+    return workspace.data_ptr(), workspace.numel() * workspace.element_size()
+
+
 def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISION, stream=None,
-           triples=None):
+           triples=None, workspace=None):
     """E(T) as a 0-d FP64 device tensor (the forward of the custom_vjp).
     ``triples=(begin, end)`` restricts the sum to that slice of the unique
-    occupied triples (a shard of the work)."""
+    occupied triples (a shard of the work).  ``workspace``: an optional device
+    tensor of at least ``workspace_bytes(..., with_vjp=False)`` bytes, 256-byte
+    aligned; without it the library allocates and frees its own."""
     nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
     out = torch.zeros((), dtype=torch.float64, device=t1d.device)
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -89,12 +98,12 @@ def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISIO
     t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
     _lib.check(lib.pyscfad_amd_ccsd_t_energy_slice(
         nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
-        int(packed), _prec(precision), int(t0), int(t1_), _ptr(out), None, 0, s.cuda_stream))
+        int(packed), _prec(precision), int(t0), int(t1_), _ptr(out), *_ws(workspace), s.cuda_stream))
     return out
 
 
 def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=DEFAULT_PRECISION,
-               stream=None, return_energy=False, triples=None):
+               stream=None, return_energy=False, triples=None, workspace=None):
     """Cotangents of E(T) as FP64 device tensors, in the order and layouts of
     the reference's _ccsd_t_energy_vjp: (t1, t2, fock, mo_energy, ovoo, ovov, ovvv)."""
     nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
@@ -109,7 +118,7 @@ def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=
     _lib.check(lib.pyscfad_amd_ccsd_t_energy_vjp_slice(
         nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
         int(packed), float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars],
-        _ptr(out), None, 0, s.cuda_stream))
+        _ptr(out), *_ws(workspace), s.cuda_stream))
     if return_energy:
         return tuple(bars), out
     return tuple(bars)

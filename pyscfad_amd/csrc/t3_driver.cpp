@@ -47,13 +47,14 @@ hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void
                           const double* ovoo, const double* mo_energy, const double* fock, int o, int v, int Op,
                           int V, hipStream_t s);
 hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs, int nslab,
-                           const VpContrib* contribs, int o, int v, int V, const double* t1, const double* fvoP,
-                           double* ovov_acc, double* t2v_acc, double* t1_acc, double* fvo_acc, hipStream_t s);
+                           const VpContrib* contribs, int o, int v, int Op, int V, const double* t1,
+                           const double* fvoP, double* ovov_acc, double* t2h_acc, double* t1_acc, double* fvo_acc,
+                           hipStream_t s);
 hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
                          const double* t2, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
-                       const double* t2hacc, const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
+                       const double* t2hacc, const double* ovoo_acc, const double* ovov_acc,
                        const double* t1_acc, const double* fvo_acc, const double* eps_bar, int o, int v, int Op,
                        int V, hipStream_t s);
 }
@@ -119,7 +120,7 @@ struct Layout {
 
 struct Regions {
   size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, fvoP, energy;
-  size_t ovvv_acc, t2acc, t2hacc, t2vacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
+  size_t ovvv_acc, t2acc, t2hacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
   size_t slots, slot_bytes;
   size_t slot_w0, slot_G, slot_H, slot_part, slot_vp;
   size_t split, split_bytes;
@@ -160,7 +161,6 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
     r.ovvv_acc = L.take(o * V * V * V * 8);
     r.t2acc = L.take(Op * Op * V * V * 8);
     r.t2hacc = L.take(Op * Op * V * V * 8);
-    r.t2vacc = L.take(o * o * V * V * 8);
     r.ovov_acc = L.take(o * o * V * V * 8);
     r.ovoo_acc = L.take(o * o * V * Op * 8);
     r.t1_acc = L.take(o * V * 8);
@@ -721,8 +721,8 @@ struct Ctx {
         case L_VPART:
           e = t3_launch_vpart(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a,
                               reinterpret_cast<const VpSlab*>(db + L.off_b), L.n_b,
-                              reinterpret_cast<const VpContrib*>(db + L.off_c), d.o, d.v, d.V, t1,
-                              at<double>(r.fvoP), at<double>(r.ovov_acc), at<double>(r.t2vacc),
+                              reinterpret_cast<const VpContrib*>(db + L.off_c), d.o, d.v, d.Op, d.V, t1,
+                              at<double>(r.fvoP), at<double>(r.ovov_acc), at<double>(r.t2hacc),
                               at<double>(r.t1_acc), at<double>(r.fvo_acc), st);
           break;
       }
@@ -838,6 +838,14 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   // the upload reads host memory that must stay alive until it completes
   e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+  // with timing on, the per-call setup (accumulator clears, input re-layout)
+  // and the cotangent write-back are recorded as family 4
+  hipEvent_t sev[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (g_timing)
+    for (auto& x : sev)
+      if ((e = hipEventCreate(&x)) != hipSuccess) return hip_fail(e, "event");
+  auto mark = [&](int i) { return g_timing ? hipEventRecord(sev[i], st) : hipSuccess; };
+  if ((e = mark(0)) != hipSuccess) return hip_fail(e, "event");
   e = hipMemsetAsync(c.ws + c.r.energy, 0, 8, st);
   if (e != hipSuccess) return hip_fail(e, "memset");
   if (backward) {
@@ -851,12 +859,14 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
                      c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), c.at<double>(c.r.fvoP), ovvv, packed,
                      t2, ovoo, mo_energy, fock, c.d.o, c.d.v, c.d.Op, c.d.V, st);
   if (e != hipSuccess) return hip_fail(e, "prep");
+  if ((e = mark(1)) != hipSuccess) return hip_fail(e, "event");
   e = c.execute(st);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  if ((e = mark(2)) != hipSuccess) return hip_fail(e, "event");
   if (backward) {
     e = t3_finalize(t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, packed,
                     c.at<double>(c.r.ovvv_acc), c.at<double>(c.r.t2acc), c.at<double>(c.r.t2hacc),
-                    c.at<double>(c.r.t2vacc), c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc),
+                    c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc),
                     c.at<double>(c.r.t1_acc), c.at<double>(c.r.fvo_acc), c.at<double>(c.r.eps_bar), c.d.o, c.d.v,
                     c.d.Op, c.d.V, st);
     if (e != hipSuccess) return hip_fail(e, "finalize");
@@ -864,6 +874,16 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   if (e_out) {
     e = hipMemcpyAsync(e_out, c.ws + c.r.energy, 8, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "energy copy");
+  }
+  if (g_timing) {
+    if ((e = mark(3)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "timing");
+    float a = 0.f, b = 0.f;
+    if ((e = hipEventElapsedTime(&a, sev[0], sev[1])) != hipSuccess) return hip_fail(e, "timing");
+    if ((e = hipEventElapsedTime(&b, sev[2], sev[3])) != hipSuccess) return hip_fail(e, "timing");
+    g_family_ms[FAM_PREP] += (double)a + (double)b;
+    g_family_launches[FAM_PREP] += 1;
+    for (auto& x : sev) (void)hipEventDestroy(x);
   }
   return cleanup(PYSCFAD_AMD_OK);
 }

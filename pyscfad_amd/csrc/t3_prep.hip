@@ -208,14 +208,15 @@ vcm_build(const VpTriple* __restrict__ trs, int o, int v, int V, const double* _
 }
 
 // Scatter of the V-term adjoints of one batch into the FP64 accumulators.
-// Slab tasks (blockIdx.y < nslab): element (x,y) of ovov_acc[p][q] and
-// t2v_acc[p][q] += 1/2 of every (triple, d, transposed) contribution, in list
+// Slab tasks (blockIdx.y < nslab): element (x,y) of ovov_acc[p][q] and of
+// t2h_acc[q][p] (the [i][j][b][a] accumulator of t2_bar, shared with the hole
+// adjoint) += 1/2 of every (triple, d, transposed) contribution, in list
 // order.  Vector tasks (blockIdx.y == nslab): t1_acc[r][z], fvo_acc[r][z].
 
 __global__ void __launch_bounds__(256)
 vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __restrict__ contribs,
-              const VpTriple* __restrict__ trs, int ntriple, int o, int V, double* __restrict__ ovov_acc,
-              double* __restrict__ t2v_acc, double* __restrict__ t1_acc, double* __restrict__ fvo_acc) {
+              const VpTriple* __restrict__ trs, int ntriple, int o, int Op, int V, double* __restrict__ ovov_acc,
+              double* __restrict__ t2h_acc, double* __restrict__ t1_acc, double* __restrict__ fvo_acc) {
   const long long VV = (long long)V * V;
   if ((int)blockIdx.y < nslab) {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -230,9 +231,8 @@ vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __re
       am += out[(2 * ct.d) * VV + idx];
       af += out[(2 * ct.d + 1) * VV + idx];
     }
-    const long long off = ((long long)sl.p * o + sl.q) * VV + e;
-    ovov_acc[off] += 0.5 * am;
-    t2v_acc[off] += 0.5 * af;
+    ovov_acc[((long long)sl.p * o + sl.q) * VV + e] += 0.5 * am;
+    t2h_acc[((long long)sl.q * Op + sl.p) * VV + e] += 0.5 * af;
     return;
   }
   // vectors: thread per (r, z)
@@ -287,19 +287,16 @@ __global__ void fin_ovvv(double* __restrict__ out, const double* __restrict__ ac
   }
 }
 
-// t2_bar[i,j,a,b] = t2acc[i][j][a][b] + t2hacc[i][j][b][a] + t2vacc[j][i][b][a]
+// t2_bar[i,j,a,b] = t2acc[i][j][a][b] + t2hacc[i][j][b][a]
 __global__ void fin_t2(double* __restrict__ out, const double* __restrict__ t2acc,
-                       const double* __restrict__ t2hacc, const double* __restrict__ t2vacc, int o, int v,
-                       int Op, int V) {
+                       const double* __restrict__ t2hacc, int o, int v, int Op, int V) {
   const long long total = (long long)o * o * v * v;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     const int b = (int)(e % v), a = (int)((e / v) % v);
     const int j = (int)((e / ((long long)v * v)) % o), i = (int)(e / ((long long)v * v * o));
     const long long pij = ((long long)i * Op + j) * V * V;
-    const long long vji = ((long long)j * o + i) * V * V;
-    out[e] = t2acc[pij + (long long)a * V + b] + t2hacc[pij + (long long)b * V + a] +
-             t2vacc[vji + (long long)b * V + a];
+    out[e] = t2acc[pij + (long long)a * V + b] + t2hacc[pij + (long long)b * V + a];
   }
 }
 
@@ -379,8 +376,8 @@ extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void
 }
 
 extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs,
-                                      int nslab, const VpContrib* contribs, int o, int v, int V,
-                                      const double* t1, const double* fvoP, double* ovov_acc, double* t2v_acc,
+                                      int nslab, const VpContrib* contribs, int o, int v, int Op, int V,
+                                      const double* t1, const double* fvoP, double* ovov_acc, double* t2h_acc,
                                       double* t1_acc, double* fvo_acc, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
@@ -395,8 +392,8 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
   }
   const long long nx = VV > (long long)o * V ? VV : (long long)o * V;
   dim3 g2((unsigned)((nx + 255) / 256), nslab + 1);
-  hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, V, ovov_acc,
-                     t2v_acc, t1_acc, fvo_acc);
+  hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, Op, V,
+                     ovov_acc, t2h_acc, t1_acc, fvo_acc);
   return hipGetLastError();
 }
 
@@ -412,14 +409,14 @@ extern "C" hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int
 extern "C" hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
                                   double* ovoo_bar, double* ovov_bar, double* ovvv_bar, int packed,
                                   const double* ovvv_acc, const double* t2acc, const double* t2hacc,
-                                  const double* t2vacc, const double* ovoo_acc, const double* ovov_acc,
+                                  const double* ovoo_acc, const double* ovov_acc,
                                   const double* t1_acc, const double* fvo_acc, const double* eps_bar, int o,
                                   int v, int Op, int V, hipStream_t s) {
   const long long npair = (long long)v * (v + 1) / 2;
   const long long n_ovvv = packed ? (long long)o * v * npair : (long long)o * v * v * v;
   hipLaunchKernelGGL(fin_ovvv, grid_for(n_ovvv), dim3(256), 0, s, ovvv_bar, ovvv_acc, o, v, V, packed);
-  hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2acc, t2hacc, t2vacc,
-                     o, v, Op, V);
+  hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2acc, t2hacc, o, v,
+                     Op, V);
   const long long n = o + v;
   const long long nsmall = (long long)o * v * o * o + (long long)o * v * o * v + (long long)o * v + n * n + n;
   hipLaunchKernelGGL(fin_small, grid_for(nsmall), dim3(256), 0, s, ovoo_bar, ovov_bar, t1_bar, fock_bar,
