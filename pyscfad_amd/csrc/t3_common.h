@@ -109,9 +109,9 @@ struct EpiReduceArgs {
 struct VpTriple {
   int i, j, k;
   const void* H;
-  double* out;     // 18 V^2: [0,12) contractions, [12,18) CM_d / CN_d
+  double* out;     // 19 V^2: [0,12) contractions, [12,18) CM_d / CN_d, [18] 6 vector sums
 };
-constexpr int VP_OUT = 0, VP_CM = 12, VP_TOTAL = 18;   // offsets in V^2 units
+constexpr int VP_OUT = 0, VP_CM = 12, VP_VSUM = 18, VP_TOTAL = 19;   // offsets in V^2 units
 
 struct VpSlab {
   int p, q;

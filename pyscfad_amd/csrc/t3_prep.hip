@@ -98,11 +98,12 @@ __global__ void prep_eps(double* __restrict__ eps_occ, double* __restrict__ eps_
 
 // Three passes over H per triple, one per contracted position, each with
 // coalesced reads of H:
-//   pass 0 (contract a): thread per (b,c), loop a      -> S0, F0
-//   pass 1 (contract b): thread per (a,c), loop b      -> S1, F1, PM2, PN2
-//   pass 2 (contract c): wave per (a,b), lanes over c  -> S2, F2, PM1, PN1, PM0, PN0
-// PM_d / PN_d are partial sums (over the position that is neither d nor the
-// row) reduced in vpart_scatter.  blockIdx.y = npass * triple + (pass - pass0).
+//   pass 0 (contract a): 64 (b,c) per block, 4 waves split a  -> S0, F0
+//   pass 1 (contract b): 64 (a,c) per block, 4 waves split b  -> S1, F1, PM2, PN2
+//   pass 2 (contract c): wave per (a,b), lanes over c          -> S2, F2, PM1, PN1, PM0, PN0
+// The wave partials of passes 0/1 are added in wave order.  PM_d / PN_d are
+// partial sums (over the position that is neither d nor the row), summed to
+// vectors by vpart_vecsum.  blockIdx.y = npass * triple + (pass - pass0).
 template <class TP>
 __global__ void __launch_bounds__(256)
 vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ t1,
@@ -114,33 +115,48 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
   const long long VV = (long long)V * V;
   double* out = tr.out;
   if (pass < 2) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= VV) return;
-    const int u = (int)(e / V), w = (int)(e % V);
+    __shared__ double red[4][4][64];
+    const int el = threadIdx.x & 63, sp = threadIdx.x >> 6;
+    const long long e = (long long)blockIdx.x * 64 + el;
+    const bool live = e < VV;
+    const long long ec = live ? e : 0;
+    const int u = (int)(ec / V), w = (int)(ec % V);
     const int td = occ[pass];
     double s = 0.0, f = 0.0, pm = 0.0, pn = 0.0;
     if (pass == 0) {        // e = (b, c)
-      for (int x = 0; x < v; x++) {
-        const double hv = (double)H[(long long)x * VV + e];
+      for (int x = sp; x < v; x += 4) {
+        const double hv = (double)H[(long long)x * VV + ec];
         s += hv * t1[(long long)td * v + x];
         f += hv * fvoP[(long long)td * V + x];
       }
-      out[0 * VV + e] = s;
-      out[1 * VV + e] = f;
     } else {                // e = (a, c)
       const double* CM2 = out + (VP_CM + 4) * VV + (long long)u * V;
       const double* CN2 = out + (VP_CM + 5) * VV + (long long)u * V;
-      for (int x = 0; x < v; x++) {
+      for (int x = sp; x < v; x += 4) {
         const double hv = (double)H[(long long)u * VV + (long long)x * V + w];
         s += hv * t1[(long long)td * v + x];
         f += hv * fvoP[(long long)td * V + x];
         pm += hv * CM2[x];
         pn += hv * CN2[x];
       }
-      out[2 * VV + e] = s;
-      out[3 * VV + e] = f;
-      out[(6 + 4) * VV + e] = pm;   // PM2[a][c]
-      out[(7 + 4) * VV + e] = pn;   // PN2[a][c]
+    }
+    red[0][sp][el] = s;
+    red[1][sp][el] = f;
+    red[2][sp][el] = pm;
+    red[3][sp][el] = pn;
+    __syncthreads();
+    if (sp != 0 || !live) return;
+    double r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) r[q] = ((red[q][0][el] + red[q][1][el]) + red[q][2][el]) + red[q][3][el];
+    if (pass == 0) {
+      out[0 * VV + e] = r[0];
+      out[1 * VV + e] = r[1];
+    } else {
+      out[2 * VV + e] = r[0];
+      out[3 * VV + e] = r[1];
+      out[(6 + 4) * VV + e] = r[2];   // PM2[a][c]
+      out[(7 + 4) * VV + e] = r[3];   // PN2[a][c]
     }
     return;
   }
@@ -177,6 +193,41 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
     out[(6 + 0) * VV + row] = acc[4];      // PM0[a][b]
     out[(7 + 0) * VV + row] = acc[5];      // PN0[a][b]
   }
+}
+
+// Vector sums of the partial matrices, per triple and task = 2 d + (M/N):
+//   d = 0: row sums    vsum[z] = sum_b P0[z][b]
+//   d = 1: column sums vsum[z] = sum_a P1[a][z]
+//   d = 2: column sums vsum[z] = sum_a P2[a][z]
+// into out[VP_VSUM][task][z]; every sum in a fixed order.
+__global__ void __launch_bounds__(256)
+vpart_vecsum(const VpTriple* __restrict__ trs, int V) {
+  __shared__ double red[4][64];
+  const VpTriple tr = trs[blockIdx.y / 6];
+  const int task = blockIdx.y % 6, d = task / 2;
+  const long long VV = (long long)V * V;
+  const double* P = tr.out + (6 + task) * VV;
+  double* vs = tr.out + (long long)VP_VSUM * VV + (long long)task * V;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (d == 0) {
+    // 16 rows per wave, lanes over the row
+    for (int rr = 0; rr < 16; rr++) {
+      const int z = blockIdx.x * 64 + wv * 16 + rr;
+      if (z >= V) break;
+      double sacc = 0.0;
+      for (int b = lane; b < V; b += 64) sacc += P[(long long)z * V + b];
+      for (int off = 32; off > 0; off >>= 1) sacc += __shfl_xor(sacc, off, 64);
+      if (lane == 0) vs[z] = sacc;
+    }
+    return;
+  }
+  const int z = blockIdx.x * 64 + lane;
+  double sacc = 0.0;
+  if (z < V)
+    for (int a = wv; a < V; a += 4) sacc += P[(long long)a * V + z];
+  red[wv][lane] = sacc;
+  __syncthreads();
+  if (wv == 0 && z < V) vs[z] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // Pair matrices of the V term of one triple (FP64), task = 2 d + (M/N):
@@ -235,7 +286,7 @@ vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __re
     t2h_acc[((long long)sl.q * Op + sl.p) * VV + e] += 0.5 * af;
     return;
   }
-  // vectors: thread per (r, z)
+  // vectors: thread per (r, z), the batch's vector sums in triple and d order
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
   if (e >= (long long)o * V) return;
   const int r = (int)(e / V), z = (int)(e % V);
@@ -243,17 +294,11 @@ vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __re
   for (int t = 0; t < ntriple; t++) {
     const VpTriple& tr = trs[t];
     const int occ[3] = {tr.i, tr.j, tr.k};
+    const double* vs = tr.out + (long long)VP_VSUM * VV;
     for (int d = 0; d < 3; d++) {
       if (occ[d] != r) continue;
-      const double* PM = tr.out + (6 + 2 * d) * VV;
-      const double* PN = tr.out + (7 + 2 * d) * VV;
-      if (d == 2) {          // sum over a of P2[a][z]
-        for (int a = 0; a < V; a++) { sm += PM[(long long)a * V + z]; sn += PN[(long long)a * V + z]; }
-      } else if (d == 1) {   // sum over a of P1[a][z]
-        for (int a = 0; a < V; a++) { sm += PM[(long long)a * V + z]; sn += PN[(long long)a * V + z]; }
-      } else {               // sum over b of P0[z][b]
-        for (int b = 0; b < V; b++) { sm += PM[(long long)z * V + b]; sn += PN[(long long)z * V + b]; }
-      }
+      sm += vs[(2 * d) * V + z];
+      sn += vs[(2 * d + 1) * V + z];
     }
   }
   t1_acc[e] += 0.5 * sm;
@@ -382,7 +427,7 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
   // passes 0/1: one thread per output element; pass 2: one wave per row (4 rows per block)
-  dim3 g1((unsigned)((VV + 255) / 256), 2 * ntriple), g2p((unsigned)((VV + 3) / 4), ntriple);
+  dim3 g1((unsigned)((VV + 63) / 64), 2 * ntriple), g2p((unsigned)((VV + 3) / 4), ntriple);
   if (tp_is_f64) {
     hipLaunchKernelGGL(vpart_contract<double>, g1, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 2, 0);
     hipLaunchKernelGGL(vpart_contract<double>, g2p, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 1, 2);
@@ -390,6 +435,7 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
     hipLaunchKernelGGL(vpart_contract<float>, g1, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 2, 0);
     hipLaunchKernelGGL(vpart_contract<float>, g2p, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 1, 2);
   }
+  hipLaunchKernelGGL(vpart_vecsum, dim3((unsigned)((V + 63) / 64), 6 * ntriple), dim3(256), 0, s, trs, V);
   const long long nx = VV > (long long)o * V ? VV : (long long)o * V;
   dim3 g2((unsigned)((nx + 255) / 256), nslab + 1);
   hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, Op, V,
