@@ -58,30 +58,49 @@ __device__ __forceinline__ void decode_cube(int m, int& A, int& B, int& C) {
   A = a; B = b; C = m;
 }
 
+// Sum over the 64 lanes of a wave with a fixed butterfly (every lane gets the
+// same, schedule-independent result).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+// Sum over the lanes that differ only in the bits of `mask` (fixed order).
+template <int MASK>
+__device__ __forceinline__ double lane_group_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    if (MASK & off) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
 template <class TP, bool BW>
 __global__ void __launch_bounds__(256)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   constexpr int T = EPI_T, T3 = T * T * T;
+  static_assert(T == 8 && T3 == 512, "two points per thread, lane bits = point bits");
   constexpr int NCM = 3 * 6 * 2 * T * T;                   // CM/CN blocks
   constexpr int NH = (6 * T3 * (int)sizeof(TP) + 7) / 8;   // H sub-cubes, in doubles
   constexpr int NU = NCM > NH ? NCM : NH;
+  constexpr int VW = 16 / (int)sizeof(TP);                 // elements per 16-byte load
+  typedef TP tpv __attribute__((ext_vector_type(VW)));
+  typedef double dv2 __attribute__((ext_vector_type(2)));
   const EpiTriple tr = triples[blockIdx.y];
   const int cube = blockIdx.x;
   int TA, TB, TC;
   decode_cube(cube, TA, TB, TC);
   const int V = args.V, v = args.nvir;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-  __shared__ TP sc[6][T3];             // w0 sub-cubes, later G
-  __shared__ double cmh[NU];           // CM/CN tile-pair blocks [d][pair][M/N][8x8], later H
-  __shared__ double vec[2][3][3][T];   // [t1/fvo][d][tile pos][8]
-  __shared__ double rbuf[2][T3];       // per canonical point: Q and mo_energy contributions
-  __shared__ double red[256];
+  __shared__ __attribute__((aligned(16))) TP sc[6][T3];   // w0 sub-cubes, later G
+  __shared__ __attribute__((aligned(16))) double cmh[NU]; // CM/CN tile-pair blocks [d][pair][M/N][8x8], later H
+  __shared__ double vec[2][3][3][T];                      // [t1/fvo][d][tile pos][8]
+  __shared__ double wred[4][20];                          // per-wave partial sums
 
-  // --- the six permuted sub-cubes of w0 ---
+  // --- the six permuted sub-cubes of w0, 16-byte loads along the rows ---
   const TP* w0 = reinterpret_cast<const TP*>(tr.w0);
-  for (int e = tid; e < 6 * T3; e += 256) {
-    const int p = e / T3, r = e % T3;
+  for (int e = tid; e < 6 * T3 / VW; e += 256) {
+    const int p = e / (T3 / VW), r = (e % (T3 / VW)) * VW;
     const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
     int q0, q1, q2;
     switch (p) {
@@ -92,19 +111,21 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       case 4: q0 = TC; q1 = TA; q2 = TB; break;
       default: q0 = TC; q1 = TB; q2 = TA; break;
     }
-    sc[p][r] = w0[(((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2)];
+    *reinterpret_cast<tpv*>(&sc[p][r]) =
+        *reinterpret_cast<const tpv*>(&w0[(((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2)]);
   }
   // --- CM_d / CN_d on the ordered tile pairs: block [d][pair mu][which][8x8] ---
   {
     const long long VV = (long long)V * V;
-    for (int e = tid; e < NCM; e += 256) {
+    for (int e2 = tid; e2 < NCM / 2; e2 += 256) {
+      const int e = 2 * e2;
       const int xy = e % (T * T);
       const int which = (e / (T * T)) % 2;
       const int mu = (e / (2 * T * T)) % 6;
       const int d = e / (12 * T * T);
       const int x = sel3(c_perm[mu][0], TA, TB, TC) * T + xy / T;
       const int y = sel3(c_perm[mu][1], TA, TB, TC) * T + xy % T;
-      cmh[e] = tr.cm[(2 * d + which) * VV + (long long)x * V + y];
+      *reinterpret_cast<dv2*>(&cmh[e]) = *reinterpret_cast<const dv2*>(&tr.cm[(2 * d + which) * VV + (long long)x * V + y]);
     }
   }
   for (int e = tid; e < 2 * 3 * 3 * T; e += 256) {
@@ -122,12 +143,17 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   const double s_bw = 2.0 * tr.fo * args.et_bar;
   const bool eqAB = (TA == TB), eqBC = (TB == TC);
 
-  double gval[2][6], hval[2][6];
-  int npt = 0;
-  for (int pt = tid; pt < T3; pt += 256, npt++) {
+  // two points per thread: pt = tid and tid + 256, i.e. al0 = wave (+4), al1 = lane bits 3-5,
+  // al2 = lane bits 0-2
+  double gval[2][6], hval[2][6], qpt[2], rpt[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int pt = tid + 256 * h;
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
-    rbuf[0][pt] = 0.0;
-    rbuf[1][pt] = 0.0;
+    qpt[h] = 0.0;
+    rpt[h] = 0.0;
+#pragma unroll
+    for (int p = 0; p < 6; p++) { gval[h][p] = 0.0; hval[h][p] = 0.0; }
     if ((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2])) continue;
     const int s0 = TA * T + al[0], s1 = TB * T + al[1], s2 = TC * T + al[2];
     const double fp = (s0 == s2) ? (1.0 / 6.0) : ((s0 == s1 || s1 == s2) ? 0.5 : 1.0);
@@ -159,61 +185,59 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       const double CW = 3.0 * W[p] + sW[par] - 2.0 * sW[par ^ 1];
       wz += W[p] * Z;
       if constexpr (BW) {
-        gval[npt][p] = s_bw * (Z + CW) * invD;
-        hval[npt][p] = s_bw * CW * invD;
+        gval[h][p] = s_bw * (Z + CW) * invD;
+        hval[h][p] = s_bw * CW * invD;
       }
     }
-    rbuf[0][pt] = fp * wz * invD;                    // Q contribution
-    rbuf[1][pt] = s_bw * fp * wz * invD * invD;      // mo_energy adjoint weight
+    qpt[h] = fp * wz * invD;                    // Q contribution
+    rpt[h] = s_bw * fp * wz * invD * invD;      // mo_energy adjoint weight
   }
-  __syncthreads();
+
+  // wave partials, fixed shuffle order:
+  //   [0] Q, [1] R, [2,3] vir pos 0 (x = wave, wave + 4), [4,12) pos 1, [12,20) pos 2
+  {
+    const double q = wave_sum(qpt[0] + qpt[1]);
+    if (lane == 0) wred[wv][0] = q;
+  }
+  if constexpr (BW) {
+    const double ra = wave_sum(rpt[0]), rb = wave_sum(rpt[1]);
+    const double r1 = lane_group_sum<7>(rpt[0] + rpt[1]);    // same al1 = lane bits 3-5
+    const double r2 = lane_group_sum<56>(rpt[0] + rpt[1]);   // same al2 = lane bits 0-2
+    if (lane == 0) {
+      wred[wv][1] = ra + rb;
+      wred[wv][2] = ra;
+      wred[wv][3] = rb;
+    }
+    if ((lane & 7) == 0) wred[wv][4 + (lane >> 3)] = r1;
+    if (lane < 8) wred[wv][12 + lane] = r2;
+  }
+  __syncthreads();   // every read of sc / cmh is done; wave partials are in place
 
   double* part = tr.part + (long long)cube * EPI_PSTRIDE;
-  // Q: fixed-order reduction (thread t sums points t, t+256; then a tree)
-  red[tid] = rbuf[0][tid] + rbuf[0][tid + 256];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w) red[tid] += red[tid + w];
-    __syncthreads();
-  }
-  if (tid == 0) part[0] = red[0];
+  if (tid == 0) part[0] = ((wred[0][0] + wred[1][0]) + wred[2][0]) + wred[3][0];
   if constexpr (!BW) return;
-
-  // mo_energy adjoint partials
-  __syncthreads();
-  red[tid] = rbuf[1][tid] + rbuf[1][tid + 256];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w) red[tid] += red[tid + w];
-    __syncthreads();
-  }
-  if (tid == 0) part[1] = red[0];
-  if (tid < 3 * T) {
-    const int pos = tid / T, x = tid % T;
-    double s = 0.0;
-    for (int u = 0; u < T; u++)
-      for (int w2 = 0; w2 < T; w2++) {
-        int idx;
-        if (pos == 0) idx = (x * T + u) * T + w2;
-        else if (pos == 1) idx = (u * T + x) * T + w2;
-        else idx = (u * T + w2) * T + x;
-        s += rbuf[1][idx];
-      }
-    part[2 + tid] = s;
+  if (tid == 1) part[1] = ((wred[0][1] + wred[1][1]) + wred[2][1]) + wred[3][1];
+  if (tid >= 2 && tid < 2 + T) {
+    const int x = tid - 2;
+    part[2 + x] = wred[x % 4][2 + x / 4];
+  } else if (tid >= 2 + T && tid < 2 + 3 * T) {
+    const int k = tid - 2 - T;   // 0..15: pos 1 then pos 2
+    part[2 + T + k] = ((wred[0][4 + k] + wred[1][4 + k]) + wred[2][4 + k]) + wred[3][4 + k];
   }
 
   // G and H into LDS, over the w0 sub-cubes and the CM/CN area
   TP* hl = reinterpret_cast<TP*>(cmh);
-  npt = 0;
-  for (int pt = tid; pt < T3; pt += 256, npt++) {
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int pt = tid + 256 * h;
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
     if ((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2])) continue;
 #pragma unroll
     for (int p = 0; p < 6; p++) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
       const int r = (x0 * T + x1) * T + x2;
-      sc[p][r] = (TP)gval[npt][p];
-      hl[p * T3 + r] = (TP)hval[npt][p];
+      sc[p][r] = (TP)gval[h][p];
+      hl[p * T3 + r] = (TP)hval[h][p];
     }
   }
   __syncthreads();
@@ -224,10 +248,21 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   TP* G = reinterpret_cast<TP*>(tr.G);
   TP* H = reinterpret_cast<TP*>(tr.H);
   const int tt[3] = {TA, TB, TC};
+  const bool distinct = !eqAB && !eqBC;
 #pragma unroll
   for (int p = 0; p < 6; p++) {
     const long long b0 = (long long)tt[c_perm[p][0]] * T, b1 = (long long)tt[c_perm[p][1]] * T,
                     b2 = (long long)tt[c_perm[p][2]] * T;
+    if (distinct) {   // every sub-cube is a different global block and all its points are canonical
+      for (int e = tid; e < T3 / VW; e += 256) {
+        const int r = e * VW;
+        const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
+        const long long off = ((b0 + x0) * V + (b1 + x1)) * V + (b2 + x2);
+        *reinterpret_cast<tpv*>(&G[off]) = *reinterpret_cast<const tpv*>(&sc[p][r]);
+        *reinterpret_cast<tpv*>(&H[off]) = *reinterpret_cast<const tpv*>(&hl[p * T3 + r]);
+      }
+      continue;
+    }
     for (int r = tid; r < T3; r += 256) {
       const int x[3] = {r / (T * T), (r / T) % T, r % T};
       bool write = false, owned = false;
