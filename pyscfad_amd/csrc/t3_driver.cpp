@@ -63,7 +63,7 @@ namespace {
 
 constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 constexpr int GEMM_TARGET_BLOCKS = 2048;
-constexpr int MAX_KSPLIT = 16;
+constexpr int MAX_KSPLIT = 64;
 
 thread_local std::string g_last_error;
 // per-family device time (ms) of the last call, recorded with hipEvents
@@ -287,7 +287,8 @@ struct Ctx {
   }
 
   // Flatten a GemmBuild into a launch.
-  void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub) {
+  void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub,
+                 int max_ksplit = MAX_KSPLIT) {
     if (gb.empty()) return;
     if (cfg < 0) cfg = pick_cfg(gb);
     const int BM = t3_gemm_tile_m(cfg), BN = t3_gemm_tile_n(cfg);
@@ -309,7 +310,7 @@ struct Ctx {
     int ksplit = 1;
     if (allow_split) {
       ksplit = (int)std::max<long long>(1, (GEMM_TARGET_BLOCKS + tiles - 1) / tiles);
-      ksplit = std::min(ksplit, std::min(MAX_KSPLIT, std::max(1, min_chunks)));
+      ksplit = std::min(ksplit, std::min(max_ksplit, std::max(1, min_chunks)));
     }
     // operand load modes; every base and stride must keep VEC-element vectors aligned
     const long long vec = t3_gemm_vec(d.prec);
@@ -561,7 +562,8 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, -1, GEMM_ACCUM, 1, true, FAM_GEMM_B2);
+    // FP64 partials of 64 splits cost more in the reduction than the extra blocks gain
+    emit_gemm(gb, -1, GEMM_ACCUM, 1, true, FAM_GEMM_B2, d.prec == PREC_F64 ? 16 : MAX_KSPLIT);
   }
 
   void emit_b3(const std::vector<Triple>& bt) {
