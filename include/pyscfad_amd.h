@@ -78,6 +78,35 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double *mo_energy, c
 size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp);
 
 /*
+ * Many independent instances of uneven size in one call.  The reference has
+ * no batched form: its callers loop over molecules and call
+ * pyscfad/cc/ccsd_t.py:28 (kernel) and ccsd_t.py:82 (_ccsd_t_energy_vjp) once
+ * per instance; each field below carries the meaning of the single-instance
+ * arguments above (device pointers, reference layouts).  With with_vjp = 0
+ * the cotangent pointers are ignored and e_out is required.
+ */
+typedef struct PyscfadAmdInstance {
+  int nocc, nvir, ovvv_packed;
+  double et_bar;
+  const double *mo_energy, *t1, *t2, *fock, *ovoo, *ovov, *ovvv;
+  double *t1_bar, *t2_bar, *fock_bar, *mo_energy_bar, *ovoo_bar, *ovov_bar, *ovvv_bar;
+  double *e_out; /* may be NULL when with_vjp = 1 */
+} PyscfadAmdInstance;
+
+/* Workspace of pyscfad_amd_ccsd_t_batched for these instances; 0 if invalid. */
+size_t pyscfad_amd_ccsd_t_batched_workspace_bytes(int n, const PyscfadAmdInstance *instances,
+                                                  int precision, int with_vjp);
+
+/* Runs every instance, in order, on `stream` as one captured hipGraph (one
+ * hipGraphLaunch); the graph is cached and replayed when the same call (same
+ * instance array contents, precision and workspace) comes again, so new
+ * values in the same buffers cost a single launch.  workspace is required
+ * (256-byte aligned, at least the size above) and must not be used by
+ * anything else while the call runs. */
+int pyscfad_amd_ccsd_t_batched(int n, const PyscfadAmdInstance *instances, int precision,
+                               int with_vjp, void *workspace, size_t workspace_bytes, void *stream);
+
+/*
  * Work sharding over unique occupied triples (no counterpart in the
  * reference, whose unit of work is the virtual-triple block of
  * ccsd_t.py:145-177).  The unique triples i >= j >= k are enumerated with i

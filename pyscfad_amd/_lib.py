@@ -21,6 +21,8 @@ EXPORTS = (
     "pyscfad_amd_ccsd_t_num_triples",
     "pyscfad_amd_ccsd_t_energy_slice",
     "pyscfad_amd_ccsd_t_energy_vjp_slice",
+    "pyscfad_amd_ccsd_t_batched_workspace_bytes",
+    "pyscfad_amd_ccsd_t_batched",
     "pyscfad_amd_last_error",
     "pyscfad_amd_set_timing",
     "pyscfad_amd_last_timing",
@@ -30,6 +32,15 @@ EXPORTS = (
 )
 
 _lib = None
+
+
+class Instance(ctypes.Structure):
+    """PyscfadAmdInstance of include/pyscfad_amd.h (one problem of a batched call)."""
+    _fields_ = ([("nocc", ctypes.c_int), ("nvir", ctypes.c_int), ("ovvv_packed", ctypes.c_int),
+                 ("et_bar", ctypes.c_double)]
+                + [(n, ctypes.c_void_p) for n in ("mo_energy", "t1", "t2", "fock", "ovoo", "ovov", "ovvv",
+                                                  "t1_bar", "t2_bar", "fock_bar", "mo_energy_bar", "ovoo_bar",
+                                                  "ovov_bar", "ovvv_bar", "e_out")])
 
 
 class T3Error(RuntimeError):
@@ -58,6 +69,11 @@ def load():
     lib.pyscfad_amd_ccsd_t_energy_vjp_slice.restype = i
     lib.pyscfad_amd_ccsd_t_energy_vjp_slice.argtypes = ([i, i] + [dp] * 7 + [i, d, i, ll, ll] + [dp] * 8
                                                         + [vp, sz, vp])
+    ip = ctypes.POINTER(Instance)
+    lib.pyscfad_amd_ccsd_t_batched_workspace_bytes.restype = sz
+    lib.pyscfad_amd_ccsd_t_batched_workspace_bytes.argtypes = [i, ip, i, i]
+    lib.pyscfad_amd_ccsd_t_batched.restype = i
+    lib.pyscfad_amd_ccsd_t_batched.argtypes = [i, ip, i, i, vp, sz, vp]
     lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
     lib.pyscfad_amd_last_error.argtypes = []
     lib.pyscfad_amd_set_timing.restype = None

@@ -124,6 +124,68 @@ def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=
     return tuple(bars)
 
 
+def _instances(problems, precision, with_vjp, outputs=None):
+    """ctypes array of PyscfadAmdInstance for a list of problems (dicts with the
+    seven inputs and optionally et_bar), plus the device tensors it points at.
+    ``outputs``: the result list of an earlier call, whose tensors are reused."""
+    import ctypes
+    arr = (_lib.Instance * len(problems))()
+    keep = []
+    for x, pr in enumerate(problems):
+        nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(
+            pr["t1"], pr["t2"], pr["mo_energy"], pr["fock"], pr["ovoo"], pr["ovov"], pr["ovvv"])
+        if outputs is not None:
+            out = outputs[x][0]
+            bars = list(outputs[x][1]) if with_vjp else [None] * 7
+        else:
+            out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+            bars = [torch.empty_like(a) for a in (t1d, t2d, fk, me, oo, ov, vv)] if with_vjp else [None] * 7
+        et_bar = pr.get("et_bar", 1.0)
+        if hasattr(et_bar, "item"):
+            et_bar = et_bar.item()
+        I = arr[x]
+        I.nocc, I.nvir, I.ovvv_packed, I.et_bar = nocc, nvir, int(packed), float(et_bar)
+        for name, t in zip(("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv"), (t1d, t2d, me, fk, oo, ov, vv)):
+            setattr(I, name, t.data_ptr())
+        for name, t in zip(("t1_bar", "t2_bar", "fock_bar", "mo_energy_bar", "ovoo_bar", "ovov_bar", "ovvv_bar"),
+                           bars):
+            setattr(I, name, t.data_ptr() if t is not None else None)
+        I.e_out = out.data_ptr()
+        keep.append((t1d, t2d, me, fk, oo, ov, vv, out, bars))
+    return arr, keep
+
+
+def batched_workspace_bytes(problems, precision=DEFAULT_PRECISION, with_vjp=True):
+    arr, keep = _instances(problems, precision, with_vjp)
+    return _lib.load().pyscfad_amd_ccsd_t_batched_workspace_bytes(len(problems), arr, _prec(precision),
+                                                                  int(with_vjp))
+
+
+def batched(problems, precision=DEFAULT_PRECISION, with_vjp=True, stream=None, workspace=None, outputs=None):
+    """E(T) (and, with with_vjp, the cotangents) of many independent instances
+    of uneven size in one call: the instances run in order as one captured
+    hipGraph, replayed when the same call comes again.  ``problems`` is a list
+    of dicts with t1, t2, mo_energy, fock, ovoo, ovov, ovvv (and optionally
+    et_bar).  Returns a list of (E, bars) per instance, bars in the order of
+    energy_vjp (None without with_vjp).  The arrays are used in place when they
+    are contiguous FP64 device tensors; a repeated call with the same input
+    tensors (new values allowed), the same ``workspace`` and ``outputs`` = the
+    previous result replays the cached graph."""
+    lib = _lib.load()
+    arr, keep = _instances(problems, precision, with_vjp, outputs)
+    need = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(problems), arr, _prec(precision), int(with_vjp))
+    if need == 0:
+        raise _lib.T3Error(lib.pyscfad_amd_last_error().decode())
+    if workspace is None:
+        workspace = torch.empty(need, dtype=torch.uint8, device=keep[0][0].device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _lib.check(lib.pyscfad_amd_ccsd_t_batched(len(problems), arr, _prec(precision), int(with_vjp),
+                                              *_ws(workspace), s.cuda_stream))
+    # the workspace must outlive the enqueued work
+    s.synchronize()
+    return [(k[7], tuple(k[8]) if with_vjp else None) for k in keep]
+
+
 def workspace_bytes(nocc, nvir, precision=DEFAULT_PRECISION, with_vjp=True):
     return _lib.load().pyscfad_amd_ccsd_t_workspace_bytes(nocc, nvir, _prec(precision), int(with_vjp))
 

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,8 @@ hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, cons
                            hipStream_t s);
 hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
                          const double* t2, hipStream_t s);
+hipError_t t3_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t t3_copy_f64(double* dst, const double* src, int n, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
                        const double* t2hacc, const double* ovoo_acc, const double* ovov_acc,
@@ -129,20 +132,20 @@ struct Regions {
 };
 
 // Triples per batch.  Larger batches merge the adjoint GEMM problems that
-// share an output slab (consecutive triples share two occupied indices), so
-// fewer read-modify-write passes over the FP64 accumulators, and they give
-// every launch more blocks.  Bounded by the per-slot memory (w0, G, H cubes).
+// share an output slab (consecutive triples share two occupied indices) and
+// give every launch more blocks; smaller ones keep the batch's cubes (w0, G,
+// H) resident in the 256 MB last-level cache while the adjoint contractions
+// re-read them.  The batch's cubes are held to about 96 MB: one triple at
+// v = 256, dozens for small instances.
 // PYSCFAD_AMD_BATCH (1..64) overrides the choice, for tuning.
 int choose_batch(const Dims& d) {
   const double cube = (double)d.V * d.V * d.V * (double)d.elt;
-  const double budget = 16.0e9;
-  long long cap = 2;
   if (const char* env = std::getenv("PYSCFAD_AMD_BATCH")) {
     const long long x = std::atoll(env);
     if (x >= 1 && x <= 64) return (int)x;
   }
-  long long b = (long long)(budget / (3.0 * cube));
-  return (int)std::max<long long>(1, std::min<long long>(b, cap));
+  const long long b = (long long)(96.0e6 / cube);
+  return (int)std::max<long long>(1, std::min<long long>(b, 64));
 }
 
 Regions plan_regions(const Dims& d, int B, bool backward) {
@@ -681,12 +684,18 @@ struct Ctx {
     }
   }
 
-  hipError_t execute(hipStream_t st) {
+  // kernels enqueued by one Launch (the V-term launch runs four)
+  static int kernels_of(const Launch& L) { return L.kind == L_VPART ? 4 : 1; }
+
+  // Enqueue launches [from, to) of the plan (all of it by default).
+  hipError_t execute(hipStream_t st, size_t from = 0, size_t to = (size_t)-1) {
     char* db = ws + r.desc;
     hipError_t e = hipSuccess;
     std::vector<hipEvent_t> ev;
     std::vector<int> fam, sub;
-    for (const Launch& L : launches) {
+    to = std::min(to, launches.size());
+    for (size_t li = from; li < to; li++) {
+      const Launch& L = launches[li];
       if (g_timing) {
         hipEvent_t a, b;
         if ((e = hipEventCreate(&a)) != hipSuccess) return e;
@@ -775,33 +784,134 @@ size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out
   return c.r.total;
 }
 
-int run(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2, const double* fock,
-        const double* ovoo, const double* ovov, const double* ovvv, int packed, bool backward, double et_bar,
-        int precision, double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
-        double* ovoo_bar, double* ovov_bar, double* ovvv_bar, double* e_out, void* workspace,
-        size_t workspace_bytes, void* stream, long long t_begin = 0, long long t_end = -1) {
-  g_last_error.clear();
-  int rc = check_args(nocc, nvir, precision);
+// Inputs and outputs of one (T) problem, in the reference layouts.
+struct Problem {
+  int nocc = 0, nvir = 0, packed = 1;
+  bool backward = false;
+  double et_bar = 1.0;
+  const double *mo_energy = nullptr, *t1 = nullptr, *t2 = nullptr, *fock = nullptr, *ovoo = nullptr,
+               *ovov = nullptr, *ovvv = nullptr;
+  double *t1_bar = nullptr, *t2_bar = nullptr, *fock_bar = nullptr, *mo_energy_bar = nullptr,
+         *ovoo_bar = nullptr, *ovov_bar = nullptr, *ovvv_bar = nullptr, *e_out = nullptr;
+};
+
+int check_problem(const Problem& p, int precision) {
+  int rc = check_args(p.nocc, p.nvir, precision);
   if (rc) return rc;
-  const long long ntrip = (long long)nocc * (nocc + 1) * (nocc + 2) / 6;
+  if (!p.mo_energy || !p.t1 || !p.t2 || !p.fock || !p.ovoo || !p.ovov || !p.ovvv)
+    return fail(PYSCFAD_AMD_ERR_ARGS, "null input pointer");
+  if (p.packed != 0 && p.packed != 1) return fail(PYSCFAD_AMD_ERR_ARGS, "ovvv_packed must be 0 or 1");
+  if (p.backward && (!p.t1_bar || !p.t2_bar || !p.fock_bar || !p.mo_energy_bar || !p.ovoo_bar || !p.ovov_bar ||
+                     !p.ovvv_bar))
+    return fail(PYSCFAD_AMD_ERR_ARGS, "null output pointer");
+  if (!p.backward && !p.e_out) return fail(PYSCFAD_AMD_ERR_ARGS, "null e_out");
+  return PYSCFAD_AMD_OK;
+}
+
+void bind(Ctx& c, const Problem& p) {
+  c.mo_energy = p.mo_energy; c.t1 = p.t1; c.t2 = p.t2; c.fock = p.fock; c.ovoo = p.ovoo; c.ovov = p.ovov;
+  c.ovvv = p.ovvv;
+  c.packed = p.packed;
+  c.backward = p.backward;
+  c.et_bar = p.et_bar;
+}
+
+// All device work of a planned problem whose descriptor blob is in place at
+// ws + r.desc: accumulator clears and input re-layout (setup), the launch
+// plan, the cotangent write-back and the energy copy (finish).  Kernels only
+// (no memset, copy, allocation or synchronisation), so the sequence can be
+// captured into a graph.  sev: 4 events bracketing setup / plan / finish, or
+// null.
+// kernels enqueued by enqueue_setup / enqueue_finish (upper bounds)
+constexpr int SETUP_KERNELS = 6, FINISH_KERNELS = 4;
+
+hipError_t enqueue_setup(Ctx& c, const Problem& p, hipStream_t st) {
+  hipError_t e;
+  if ((e = t3_zero(c.ws + c.r.energy, 16, st)) != hipSuccess) return e;
+  if (p.backward) {
+    // G and H cubes are fully rewritten per triple; the split scratch is fully
+    // rewritten before each reduction
+    const size_t acc0 = c.r.ovvv_acc, acc1 = c.r.eps_bar + (c.d.o + c.d.V) * 8;
+    if ((e = t3_zero(c.ws + acc0, (size_t)round_up((long long)(acc1 - acc0), 16), st)) != hipSuccess) return e;
+  }
+  e = t3_prep_inputs(c.d.tp_is_f64, c.ws + c.r.ovvvT, c.ws + c.r.t2T, c.ws + c.r.t2H, c.ws + c.r.ovooT,
+                     c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), c.at<double>(c.r.fvoP), p.ovvv,
+                     p.packed, p.t2, p.ovoo, p.mo_energy, p.fock, c.d.o, c.d.v, c.d.Op, c.d.V, st);
+  return e;
+}
+
+hipError_t enqueue_finish(Ctx& c, const Problem& p, hipStream_t st) {
+  hipError_t e;
+  if (p.backward) {
+    e = t3_finalize(p.t1_bar, p.t2_bar, p.fock_bar, p.mo_energy_bar, p.ovoo_bar, p.ovov_bar, p.ovvv_bar, p.packed,
+                    c.at<double>(c.r.ovvv_acc), c.at<double>(c.r.t2acc), c.at<double>(c.r.t2hacc),
+                    c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc), c.at<double>(c.r.t1_acc),
+                    c.at<double>(c.r.fvo_acc), c.at<double>(c.r.eps_bar), c.d.o, c.d.v, c.d.Op, c.d.V, st);
+    if (e != hipSuccess) return e;
+  }
+  if (p.e_out && (e = t3_copy_f64(p.e_out, c.at<double>(c.r.energy), 1, st)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+hipError_t enqueue(Ctx& c, const Problem& p, hipStream_t st, hipEvent_t* sev) {
+  hipError_t e;
+  auto mark = [&](int i) { return sev ? hipEventRecord(sev[i], st) : hipSuccess; };
+  if ((e = mark(0)) != hipSuccess) return e;
+  if ((e = enqueue_setup(c, p, st)) != hipSuccess) return e;
+  if ((e = mark(1)) != hipSuccess) return e;
+  if ((e = c.execute(st)) != hipSuccess) return e;
+  if ((e = mark(2)) != hipSuccess) return e;
+  if ((e = enqueue_finish(c, p, st)) != hipSuccess) return e;
+  return mark(3);
+}
+
+void reset_timing() {
+  for (int f = 0; f < FAM_COUNT; f++) { g_family_ms[f] = 0.0; g_family_launches[f] = 0; }
+}
+
+// Add the setup and write-back intervals of one enqueue() to family 4.
+hipError_t add_setup_time(hipEvent_t* sev) {
+  float a = 0.f, b = 0.f;
+  hipError_t e;
+  if ((e = hipEventElapsedTime(&a, sev[0], sev[1])) != hipSuccess) return e;
+  if ((e = hipEventElapsedTime(&b, sev[2], sev[3])) != hipSuccess) return e;
+  g_family_ms[FAM_PREP] += (double)a + (double)b;
+  g_family_launches[FAM_PREP] += 1;
+  return hipSuccess;
+}
+
+struct Events4 {
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipError_t create() {
+    for (auto& x : ev) {
+      hipError_t e = hipEventCreate(&x);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  ~Events4() {
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes, void* stream,
+        long long t_begin = 0, long long t_end = -1) {
+  g_last_error.clear();
+  int rc = check_problem(p, precision);
+  if (rc) return rc;
+  const long long ntrip = (long long)p.nocc * (p.nocc + 1) * (p.nocc + 2) / 6;
   if (t_end < 0) t_end = ntrip;
   if (t_begin < 0 || t_begin > t_end || t_end > ntrip)
     return fail(PYSCFAD_AMD_ERR_ARGS, "triple range must satisfy 0 <= begin <= end <= nocc(nocc+1)(nocc+2)/6");
-  if (!mo_energy || !t1 || !t2 || !fock || !ovoo || !ovov || !ovvv)
-    return fail(PYSCFAD_AMD_ERR_ARGS, "null input pointer");
-  if (packed != 0 && packed != 1) return fail(PYSCFAD_AMD_ERR_ARGS, "ovvv_packed must be 0 or 1");
-  if (backward && (!t1_bar || !t2_bar || !fock_bar || !mo_energy_bar || !ovoo_bar || !ovov_bar || !ovvv_bar))
-    return fail(PYSCFAD_AMD_ERR_ARGS, "null output pointer");
-  if (!backward && !e_out) return fail(PYSCFAD_AMD_ERR_ARGS, "null e_out");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 
   Ctx c;
-  c.d = make_dims(nocc, nvir, precision);
-  c.backward = backward;
-  c.et_bar = et_bar;
+  c.d = make_dims(p.nocc, p.nvir, precision);
   c.t_begin = t_begin;
   c.t_end = t_end;
-  const size_t need = plan_bytes(nocc, nvir, precision, backward, &c.r, &c.B, t_begin, t_end);
+  bind(c, p);
+  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end);
   bool own = false;
   if (workspace) {
     if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
@@ -816,8 +926,6 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
     }
     own = true;
   }
-  c.mo_energy = mo_energy; c.t1 = t1; c.t2 = t2; c.fock = fock; c.ovoo = ovoo; c.ovov = ovov; c.ovvv = ovvv;
-  c.packed = packed;
   c.build_plan();
   if (!c.plan_error.empty()) {
     if (own) (void)hipFree(c.ws);
@@ -833,7 +941,7 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   auto hip_fail = [&](hipError_t e, const char* where) {
     return cleanup(fail(PYSCFAD_AMD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e)));
   };
-  for (int f = 0; f < FAM_COUNT; f++) { g_family_ms[f] = 0.0; g_family_launches[f] = 0; }
+  reset_timing();
   hipError_t e;
   e = hipMemcpyAsync(c.ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
@@ -842,64 +950,265 @@ int run(int nocc, int nvir, const double* mo_energy, const double* t1, const dou
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
   // with timing on, the per-call setup (accumulator clears, input re-layout)
   // and the cotangent write-back are recorded as family 4
-  hipEvent_t sev[4] = {nullptr, nullptr, nullptr, nullptr};
-  if (g_timing)
-    for (auto& x : sev)
-      if ((e = hipEventCreate(&x)) != hipSuccess) return hip_fail(e, "event");
-  auto mark = [&](int i) { return g_timing ? hipEventRecord(sev[i], st) : hipSuccess; };
-  if ((e = mark(0)) != hipSuccess) return hip_fail(e, "event");
-  e = hipMemsetAsync(c.ws + c.r.energy, 0, 8, st);
-  if (e != hipSuccess) return hip_fail(e, "memset");
-  if (backward) {
-    const size_t acc0 = c.r.ovvv_acc, acc1 = c.r.eps_bar + (c.d.o + c.d.V) * 8;
-    e = hipMemsetAsync(c.ws + acc0, 0, acc1 - acc0, st);
-    if (e != hipSuccess) return hip_fail(e, "memset");
-    // G and H cubes are fully rewritten per triple; the split scratch is fully
-    // rewritten before each reduction
-  }
-  e = t3_prep_inputs(c.d.tp_is_f64, c.ws + c.r.ovvvT, c.ws + c.r.t2T, c.ws + c.r.t2H, c.ws + c.r.ovooT,
-                     c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), c.at<double>(c.r.fvoP), ovvv, packed,
-                     t2, ovoo, mo_energy, fock, c.d.o, c.d.v, c.d.Op, c.d.V, st);
-  if (e != hipSuccess) return hip_fail(e, "prep");
-  if ((e = mark(1)) != hipSuccess) return hip_fail(e, "event");
-  e = c.execute(st);
+  Events4 ev;
+  if (g_timing && (e = ev.create()) != hipSuccess) return hip_fail(e, "event");
+  e = enqueue(c, p, st, g_timing ? ev.ev : nullptr);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
-  if ((e = mark(2)) != hipSuccess) return hip_fail(e, "event");
-  if (backward) {
-    e = t3_finalize(t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, packed,
-                    c.at<double>(c.r.ovvv_acc), c.at<double>(c.r.t2acc), c.at<double>(c.r.t2hacc),
-                    c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc),
-                    c.at<double>(c.r.t1_acc), c.at<double>(c.r.fvo_acc), c.at<double>(c.r.eps_bar), c.d.o, c.d.v,
-                    c.d.Op, c.d.V, st);
-    if (e != hipSuccess) return hip_fail(e, "finalize");
-  }
-  if (e_out) {
-    e = hipMemcpyAsync(e_out, c.ws + c.r.energy, 8, hipMemcpyDeviceToDevice, st);
-    if (e != hipSuccess) return hip_fail(e, "energy copy");
-  }
   if (g_timing) {
-    if ((e = mark(3)) != hipSuccess) return hip_fail(e, "event");
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "timing");
-    float a = 0.f, b = 0.f;
-    if ((e = hipEventElapsedTime(&a, sev[0], sev[1])) != hipSuccess) return hip_fail(e, "timing");
-    if ((e = hipEventElapsedTime(&b, sev[2], sev[3])) != hipSuccess) return hip_fail(e, "timing");
-    g_family_ms[FAM_PREP] += (double)a + (double)b;
-    g_family_launches[FAM_PREP] += 1;
-    for (auto& x : sev) (void)hipEventDestroy(x);
+    if ((e = add_setup_time(ev.ev)) != hipSuccess) return hip_fail(e, "timing");
   }
   return cleanup(PYSCFAD_AMD_OK);
+}
+
+// ---------------------------------------------------------------------------
+// Batched problems: the instances run one after another in stream order and
+// share one workspace (its region part sized for the largest instance); every
+// instance's descriptor blob has its own place after that.  The whole
+// sequence -- descriptor upload from pinned host memory, then every
+// instance's kernels -- is captured once into a hipGraph and replayed with a
+// single hipGraphLaunch; graphs are cached by the exact call (shapes,
+// pointers, precision, workspace).
+
+struct BatchPlan {
+  std::vector<Problem> probs;
+  std::vector<Ctx> ctx;
+  size_t region_bytes = 0, blob_bytes = 0;
+  size_t need() const { return region_bytes + blob_bytes; }
+};
+
+int plan_batch(int n, const PyscfadAmdInstance* inst, int precision, int with_vjp, BatchPlan& bp) {
+  if (n < 1 || !inst) return fail(PYSCFAD_AMD_ERR_ARGS, "need at least one instance");
+  bp.probs.resize(n);
+  bp.ctx.resize(n);
+  std::vector<size_t> blob(n);
+  for (int x = 0; x < n; x++) {
+    const PyscfadAmdInstance& I = inst[x];
+    Problem& p = bp.probs[x];
+    p.nocc = I.nocc; p.nvir = I.nvir; p.packed = I.ovvv_packed; p.backward = with_vjp != 0; p.et_bar = I.et_bar;
+    p.mo_energy = I.mo_energy; p.t1 = I.t1; p.t2 = I.t2; p.fock = I.fock; p.ovoo = I.ovoo; p.ovov = I.ovov;
+    p.ovvv = I.ovvv;
+    p.t1_bar = I.t1_bar; p.t2_bar = I.t2_bar; p.fock_bar = I.fock_bar; p.mo_energy_bar = I.mo_energy_bar;
+    p.ovoo_bar = I.ovoo_bar; p.ovov_bar = I.ovov_bar; p.ovvv_bar = I.ovvv_bar; p.e_out = I.e_out;
+    int rc = check_problem(p, precision);
+    if (rc) return fail(rc, "instance " + std::to_string(x) + ": " + g_last_error);
+    Regions r;
+    int B = 1;
+    plan_bytes(p.nocc, p.nvir, precision, p.backward, &r, &B);
+    bp.region_bytes = std::max(bp.region_bytes, r.desc);
+    blob[x] = (size_t)round_up((long long)r.desc_bytes, 256);
+    Ctx& c = bp.ctx[x];
+    c.d = make_dims(p.nocc, p.nvir, precision);
+    c.B = B;
+    c.r = r;
+    bind(c, p);
+  }
+  size_t off = bp.region_bytes;
+  for (int x = 0; x < n; x++) {
+    bp.ctx[x].r.desc = off;
+    off += blob[x];
+  }
+  bp.blob_bytes = off - bp.region_bytes;
+  return PYSCFAD_AMD_OK;
+}
+
+// A captured call: a chain of graphs, each of at most MAX_GRAPH_NODES kernel
+// nodes (a single graph for small batches), launched back to back.
+struct GraphEntry {
+  std::vector<char> key;
+  std::vector<hipGraph_t> graphs;
+  std::vector<hipGraphExec_t> execs;
+  void* host_blobs = nullptr;   // pinned; read by the first graph's upload node
+};
+constexpr int MAX_GRAPH_NODES = 2048;
+
+std::mutex g_graph_mu;
+std::vector<GraphEntry> g_graphs;   // least recently used first
+constexpr size_t MAX_GRAPHS = 4;
+
+void destroy_entry(GraphEntry& g) {
+  for (auto x : g.execs) (void)hipGraphExecDestroy(x);
+  for (auto x : g.graphs) (void)hipGraphDestroy(x);
+  if (g.host_blobs) (void)hipHostFree(g.host_blobs);
+  g = GraphEntry{};
+}
+
+hipError_t launch_entry(const GraphEntry& g, hipStream_t st) {
+  for (auto x : g.execs) {
+    const hipError_t e = hipGraphLaunch(x, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_vjp, void* workspace,
+                size_t workspace_bytes, void* stream) {
+  g_last_error.clear();
+  BatchPlan bp;
+  int rc = plan_batch(n, inst, precision, with_vjp, bp);
+  if (rc) return rc;
+  if (!workspace) return fail(PYSCFAD_AMD_ERR_ARGS, "the batched entry point needs a caller workspace");
+  if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+    return fail(PYSCFAD_AMD_ERR_ARGS, "workspace must be 256-byte aligned");
+  if (workspace_bytes < bp.need())
+    return fail(PYSCFAD_AMD_ERR_WORKSPACE, "workspace too small: need " + std::to_string(bp.need()) + " bytes");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* ws = reinterpret_cast<char*>(workspace);
+  reset_timing();
+
+  std::vector<char> key(sizeof(int) * 3 + sizeof(void*) + sizeof(size_t) + sizeof(PyscfadAmdInstance) * n);
+  {
+    char* k = key.data();
+    const int head[3] = {n, precision, with_vjp};
+    std::memcpy(k, head, sizeof(head)); k += sizeof(head);
+    std::memcpy(k, &workspace, sizeof(void*)); k += sizeof(void*);
+    std::memcpy(k, &workspace_bytes, sizeof(size_t)); k += sizeof(size_t);
+    std::memcpy(k, inst, sizeof(PyscfadAmdInstance) * n);
+  }
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  if (!g_timing) {
+    for (size_t x = 0; x < g_graphs.size(); x++) {
+      if (g_graphs[x].key != key) continue;
+      GraphEntry hit = g_graphs[x];
+      g_graphs.erase(g_graphs.begin() + x);
+      g_graphs.push_back(hit);
+      const hipError_t e = launch_entry(hit, st);
+      if (e != hipSuccess) return fail(PYSCFAD_AMD_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+      return PYSCFAD_AMD_OK;
+    }
+  }
+
+  // plans against the real workspace, blobs into one pinned buffer
+  GraphEntry g;
+  g.key = key;
+  if (hipHostMalloc(&g.host_blobs, std::max<size_t>(bp.blob_bytes, 256), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PYSCFAD_AMD_ERR_OOM, "pinned host buffer for the descriptors");
+  }
+  for (int x = 0; x < n; x++) {
+    Ctx& c = bp.ctx[x];
+    c.ws = ws;
+    c.build_plan();
+    if (!c.plan_error.empty()) {
+      destroy_entry(g);
+      return fail(PYSCFAD_AMD_ERR_ARGS, "internal plan error: " + c.plan_error);
+    }
+    std::memcpy(static_cast<char*>(g.host_blobs) + (c.r.desc - bp.region_bytes), c.blob.data.data(),
+                c.blob.data.size());
+  }
+  auto hip_fail = [&](hipError_t e, const char* where) {
+    destroy_entry(g);
+    return fail(PYSCFAD_AMD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if (g_timing) {
+    // eager, with per-launch events, for measurement
+    e = hipMemcpyAsync(ws + bp.region_bytes, g.host_blobs, bp.blob_bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+    for (int x = 0; x < n; x++) {
+      Events4 ev;
+      if ((e = ev.create()) != hipSuccess) return hip_fail(e, "event");
+      if ((e = enqueue(bp.ctx[x], bp.probs[x], st, ev.ev)) != hipSuccess) return hip_fail(e, "kernel launch");
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "timing");
+      if ((e = add_setup_time(ev.ev)) != hipSuccess) return hip_fail(e, "timing");
+    }
+    destroy_entry(g);
+    return PYSCFAD_AMD_OK;
+  }
+  // The call as a list of pieces (instance setup, runs of plan launches,
+  // instance finish), cut into graphs of at most MAX_GRAPH_NODES kernels:
+  // the graph runtime walks a graph's dependency chain, and one chain per
+  // call (hundreds of thousands of nodes for 64 instances) is kept out of it.
+  struct Piece { int inst, kind; size_t a, b; int nodes; };   // kind 0 setup, 1 launches [a,b), 2 finish
+  std::vector<std::vector<Piece>> groups(1);
+  int in_group = 1;   // the descriptor upload
+  auto add = [&](const Piece& pc) {
+    if (in_group + pc.nodes > MAX_GRAPH_NODES && !groups.back().empty()) {
+      groups.emplace_back();
+      in_group = 0;
+    }
+    groups.back().push_back(pc);
+    in_group += pc.nodes;
+  };
+  for (int x = 0; x < n; x++) {
+    add(Piece{x, 0, 0, 0, SETUP_KERNELS});
+    const auto& ls = bp.ctx[x].launches;
+    size_t a = 0;
+    while (a < ls.size()) {
+      size_t b = a;
+      int k = 0;
+      const int room = std::max(MAX_GRAPH_NODES - in_group, 0);
+      const int cap = room > 0 ? room : MAX_GRAPH_NODES;
+      while (b < ls.size() && k + Ctx::kernels_of(ls[b]) <= cap) k += Ctx::kernels_of(ls[b++]);
+      if (b == a) k += Ctx::kernels_of(ls[b++]);
+      add(Piece{x, 1, a, b, k});
+      a = b;
+    }
+    add(Piece{x, 2, 0, 0, FINISH_KERNELS});
+  }
+  // capture on a private stream (the caller's may be the null stream, which
+  // cannot be captured); the graphs are launched on the caller's stream
+  hipStream_t cs = nullptr;
+  if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
+  for (size_t gi = 0; gi < groups.size() && e == hipSuccess; gi++) {
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) break;
+    if (gi == 0) e = hipMemcpyAsync(ws + bp.region_bytes, g.host_blobs, bp.blob_bytes, hipMemcpyHostToDevice, cs);
+    for (const Piece& pc : groups[gi]) {
+      if (e != hipSuccess) break;
+      Ctx& c = bp.ctx[pc.inst];
+      const Problem& p = bp.probs[pc.inst];
+      if (pc.kind == 0) e = enqueue_setup(c, p, cs);
+      else if (pc.kind == 1) e = c.execute(cs, pc.a, pc.b);
+      else e = enqueue_finish(c, p, cs);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    if (e == hipSuccess) e = e2;
+    if (graph) g.graphs.push_back(graph);
+    if (e != hipSuccess) break;
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) g.execs.push_back(exec);
+  }
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess) return hip_fail(e, "graph capture");
+  if ((e = launch_entry(g, st)) != hipSuccess) return hip_fail(e, "graph launch");
+  if (g_graphs.size() >= MAX_GRAPHS) {
+    // the evicted graph may still be running: its pinned upload source must outlive it
+    (void)hipDeviceSynchronize();
+    destroy_entry(g_graphs.front());
+    g_graphs.erase(g_graphs.begin());
+  }
+  g_graphs.push_back(g);
+  return PYSCFAD_AMD_OK;
 }
 
 }  // namespace
 
 extern "C" {
 
+namespace {
+Problem make_problem(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
+                     const double* fock, const double* ovoo, const double* ovov, const double* ovvv, int packed,
+                     bool backward, double et_bar, double* t1_bar, double* t2_bar, double* fock_bar,
+                     double* mo_energy_bar, double* ovoo_bar, double* ovov_bar, double* ovvv_bar, double* e_out) {
+  Problem p;
+  p.nocc = nocc; p.nvir = nvir; p.packed = packed; p.backward = backward; p.et_bar = et_bar;
+  p.mo_energy = mo_energy; p.t1 = t1; p.t2 = t2; p.fock = fock; p.ovoo = ovoo; p.ovov = ovov; p.ovvv = ovvv;
+  p.t1_bar = t1_bar; p.t2_bar = t2_bar; p.fock_bar = fock_bar; p.mo_energy_bar = mo_energy_bar;
+  p.ovoo_bar = ovoo_bar; p.ovov_bar = ovov_bar; p.ovvv_bar = ovvv_bar; p.e_out = e_out;
+  return p;
+}
+}  // namespace
+
 int pyscfad_amd_ccsd_t_energy(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
                               const double* fock, const double* ovoo, const double* ovov, const double* ovvv,
                               int ovvv_packed, int precision, double* e_out, void* workspace,
                               size_t workspace_bytes, void* stream) {
-  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, false, 1.0, precision, nullptr,
-             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out, workspace, workspace_bytes, stream);
+  return run(make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, false, 1.0, nullptr,
+                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out),
+             precision, workspace, workspace_bytes, stream);
 }
 
 int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
@@ -908,9 +1217,21 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double* mo_energy, c
                                   double* fock_bar, double* mo_energy_bar, double* ovoo_bar, double* ovov_bar,
                                   double* ovvv_bar, double* e_out, void* workspace, size_t workspace_bytes,
                                   void* stream) {
-  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, precision, t1_bar,
-             t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out, workspace, workspace_bytes,
-             stream);
+  return run(make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, t1_bar,
+                          t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out),
+             precision, workspace, workspace_bytes, stream);
+}
+
+size_t pyscfad_amd_ccsd_t_batched_workspace_bytes(int n, const PyscfadAmdInstance* instances, int precision,
+                                                  int with_vjp) {
+  BatchPlan bp;
+  if (plan_batch(n, instances, precision, with_vjp, bp)) return 0;
+  return bp.need();
+}
+
+int pyscfad_amd_ccsd_t_batched(int n, const PyscfadAmdInstance* instances, int precision, int with_vjp,
+                               void* workspace, size_t workspace_bytes, void* stream) {
+  return run_batched(n, instances, precision, with_vjp, workspace, workspace_bytes, stream);
 }
 
 long long pyscfad_amd_ccsd_t_num_triples(int nocc) {
@@ -921,9 +1242,9 @@ int pyscfad_amd_ccsd_t_energy_slice(int nocc, int nvir, const double* mo_energy,
                                     const double* fock, const double* ovoo, const double* ovov, const double* ovvv,
                                     int ovvv_packed, int precision, long long triple_begin, long long triple_end,
                                     double* e_out, void* workspace, size_t workspace_bytes, void* stream) {
-  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, false, 1.0, precision, nullptr,
-             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out, workspace, workspace_bytes, stream,
-             triple_begin, triple_end);
+  return run(make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, false, 1.0, nullptr,
+                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out),
+             precision, workspace, workspace_bytes, stream, triple_begin, triple_end);
 }
 
 int pyscfad_amd_ccsd_t_energy_vjp_slice(int nocc, int nvir, const double* mo_energy, const double* t1,
@@ -933,9 +1254,9 @@ int pyscfad_amd_ccsd_t_energy_vjp_slice(int nocc, int nvir, const double* mo_ene
                                         double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
                                         double* ovoo_bar, double* ovov_bar, double* ovvv_bar, double* e_out,
                                         void* workspace, size_t workspace_bytes, void* stream) {
-  return run(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, precision, t1_bar,
-             t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out, workspace, workspace_bytes,
-             stream, triple_begin, triple_end);
+  return run(make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, t1_bar,
+                          t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out),
+             precision, workspace, workspace_bytes, stream, triple_begin, triple_end);
 }
 
 size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp) {

@@ -469,3 +469,31 @@ extern "C" hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_b
                      mo_energy_bar, ovoo_acc, ovov_acc, t1_acc, fvo_acc, eps_bar, o, v, Op, V);
   return hipGetLastError();
 }
+
+// ---------------- stream-ordered clears and copies as kernels ----------------
+// (kernels rather than hipMemsetAsync / hipMemcpyAsync so that every node of a
+// captured graph is a kernel node)
+__global__ void zero_fill(uint4* __restrict__ p, long long n16) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n16;
+       e += (long long)gridDim.x * blockDim.x)
+    p[e] = uint4{0u, 0u, 0u, 0u};
+}
+
+__global__ void copy_f64(double* __restrict__ dst, const double* __restrict__ src, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) dst[e] = src[e];
+}
+
+// bytes must be a multiple of 16 and p 16-byte aligned
+extern "C" hipError_t t3_zero(void* p, size_t bytes, hipStream_t s) {
+  const long long n16 = (long long)(bytes / 16);
+  if (n16 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_fill, grid_for(n16), dim3(256), 0, s, reinterpret_cast<uint4*>(p), n16);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_copy_f64(double* dst, const double* src, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_f64, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+  return hipGetLastError();
+}

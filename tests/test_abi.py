@@ -61,3 +61,40 @@ def test_workspace_too_small_rejected():
     rc = lib.pyscfad_amd_ccsd_t_energy(3, 5, *([dummy] * 7), 1, 0, dummy, dummy, 16, None)
     assert rc == 4
     assert b"workspace" in lib.pyscfad_amd_last_error()
+
+
+def _fake_instances(shapes):
+    """Instance array with non-null placeholder pointers: sizing and argument
+    checks read only the shapes and never dereference a pointer."""
+    arr = (_lib.Instance * len(shapes))()
+    for x, (o, v) in enumerate(shapes):
+        I = arr[x]
+        I.nocc, I.nvir, I.ovvv_packed, I.et_bar = o, v, 1, 1.0
+        for name, _ in _lib.Instance._fields_[4:]:
+            setattr(I, name, 4096)
+    return arr
+
+
+def test_batched_workspace_bytes():
+    lib = _lib.load()
+    shapes = [(4, 40), (9, 73), (6, 120)]
+    arr = _fake_instances(shapes)
+    for prec in (0, 1):
+        for vjp in (0, 1):
+            need = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(shapes), arr, prec, vjp)
+            single = max(lib.pyscfad_amd_ccsd_t_workspace_bytes(o, v, prec, vjp) for o, v in shapes)
+            # instances share the region part; each keeps its own descriptors
+            assert single <= need < sum(lib.pyscfad_amd_ccsd_t_workspace_bytes(o, v, prec, vjp) for o, v in shapes)
+    assert lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(0, arr, 1, 1) == 0
+    bad = _fake_instances([(4, 40), (0, 10)])
+    assert lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(2, bad, 1, 1) == 0
+
+
+def test_batched_argument_errors():
+    lib = _lib.load()
+    arr = _fake_instances([(4, 40), (5, 19)])
+    # no workspace: rejected before any device work
+    assert lib.pyscfad_amd_ccsd_t_batched(2, arr, 1, 1, None, 0, None) == 1
+    arr[1].ovvv = None
+    assert lib.pyscfad_amd_ccsd_t_batched(2, arr, 1, 1, None, 0, None) == 1
+    assert b"instance 1" in lib.pyscfad_amd_last_error()
