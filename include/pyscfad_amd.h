@@ -93,14 +93,24 @@ typedef struct PyscfadAmdInstance {
   double *e_out; /* may be NULL when with_vjp = 1 */
 } PyscfadAmdInstance;
 
+/* Autotuner (no counterpart in the reference): times every tile
+ * configuration of each contraction family (W build; the ovvv, t2-particle,
+ * ovoo and t2-hole adjoints) on one batch of triples of an (nocc, nvir)
+ * problem in a scratch workspace, and keeps the fastest for every later call
+ * of the same padded size and precision in this process.  cfg_out (may be
+ * NULL) receives the 5 chosen configuration ids.  Setting the environment
+ * variable PYSCFAD_AMD_AUTOTUNE=1 makes every call tune sizes not seen yet. */
+int pyscfad_amd_ccsd_t_autotune(int nocc, int nvir, int precision, int *cfg_out, void *stream);
+
 /* Workspace of pyscfad_amd_ccsd_t_batched for these instances; 0 if invalid. */
 size_t pyscfad_amd_ccsd_t_batched_workspace_bytes(int n, const PyscfadAmdInstance *instances,
                                                   int precision, int with_vjp);
 
-/* Runs every instance, in order, on `stream` as one captured hipGraph (one
- * hipGraphLaunch); the graph is cached and replayed when the same call (same
+/* Runs every instance, in order, on `stream` as captured hipGraphs of at most
+ * 2048 kernel nodes each, launched back to back (one graph for small
+ * batches); the graphs are cached and replayed when the same call (same
  * instance array contents, precision and workspace) comes again, so new
- * values in the same buffers cost a single launch.  workspace is required
+ * values in the same buffers cost only the graph launches.  workspace is required
  * (256-byte aligned, at least the size above) and must not be used by
  * anything else while the call runs. */
 int pyscfad_amd_ccsd_t_batched(int n, const PyscfadAmdInstance *instances, int precision,

@@ -23,6 +23,7 @@ EXPORTS = (
     "pyscfad_amd_ccsd_t_energy_vjp_slice",
     "pyscfad_amd_ccsd_t_batched_workspace_bytes",
     "pyscfad_amd_ccsd_t_batched",
+    "pyscfad_amd_ccsd_t_autotune",
     "pyscfad_amd_last_error",
     "pyscfad_amd_set_timing",
     "pyscfad_amd_last_timing",
@@ -74,6 +75,8 @@ def load():
     lib.pyscfad_amd_ccsd_t_batched_workspace_bytes.argtypes = [i, ip, i, i]
     lib.pyscfad_amd_ccsd_t_batched.restype = i
     lib.pyscfad_amd_ccsd_t_batched.argtypes = [i, ip, i, i, vp, sz, vp]
+    lib.pyscfad_amd_ccsd_t_autotune.restype = i
+    lib.pyscfad_amd_ccsd_t_autotune.argtypes = [i, i, i, ctypes.POINTER(ctypes.c_int), vp]
     lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
     lib.pyscfad_amd_last_error.argtypes = []
     lib.pyscfad_amd_set_timing.restype = None

@@ -186,6 +186,21 @@ def batched(problems, precision=DEFAULT_PRECISION, with_vjp=True, stream=None, w
     return [(k[7], tuple(k[8]) if with_vjp else None) for k in keep]
 
 
+TUNED_FAMILIES = ("w_build", "ovvv_adjoint", "t2_particle_adjoint", "ovoo_adjoint", "t2_hole_adjoint")
+TILE_CONFIGS = {0: (64, 64), 1: (128, 128), 3: (64, 32), 4: (32, 64), 5: (128, 64), 6: (64, 128)}
+
+
+def autotune(nocc, nvir, precision=DEFAULT_PRECISION, stream=None):
+    """Time every tile configuration of each contraction family for this
+    (nocc, nvir) size and keep the fastest for later calls in this process.
+    Returns {family: (tile_m, tile_n)}."""
+    import ctypes
+    out = (ctypes.c_int * 5)()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _lib.check(_lib.load().pyscfad_amd_ccsd_t_autotune(nocc, nvir, _prec(precision), out, s.cuda_stream))
+    return {f: TILE_CONFIGS.get(out[x]) for x, f in enumerate(TUNED_FAMILIES)}
+
+
 def workspace_bytes(nocc, nvir, precision=DEFAULT_PRECISION, with_vjp=True):
     return _lib.load().pyscfad_amd_ccsd_t_workspace_bytes(nocc, nvir, _prec(precision), int(with_vjp))
 

@@ -36,6 +36,7 @@ extern "C" {
 int t3_gemm_tile_m(int cfg);
 int t3_gemm_tile_n(int cfg);
 int t3_gemm_vec(int prec);
+int t3_gemm_cfg_valid(int prec, int cfg);
 int t3_gemm_chunk(int prec);
 hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob, const GemmSeg* segs,
                           long long nblocks, hipStream_t stream);
@@ -79,6 +80,17 @@ enum Family {
 thread_local int g_timing = 0;
 thread_local double g_family_ms[FAM_COUNT];
 thread_local long long g_family_launches[FAM_COUNT];
+
+// Autotuned tile configuration per (precision, contraction family, V, Op),
+// filled by the autotuner, read by every plan built afterwards.
+std::mutex g_tune_mu;
+std::map<std::vector<int>, int> g_tuned;
+
+int tuned_cfg(int prec, int fam, int V, int Op) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tuned.find({prec, fam, V, Op});
+  return it == g_tuned.end() ? -1 : it->second;
+}
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
@@ -268,6 +280,14 @@ struct Ctx {
   std::vector<EpiArgs> epi_args;
   std::vector<EpiReduceArgs> epired_args;
   std::string plan_error;   // set when a contraction does not fit the GEMM's operand modes
+  int force_cfg[FAM_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};   // autotuner trials
+
+  // tile configuration of a contraction: a trial's, else the tuned one, else the default
+  int cfg_for(int fam, int dflt) const {
+    if (force_cfg[fam] >= 0) return force_cfg[fam];
+    const int t = tuned_cfg(d.prec, fam, d.V, d.Op);
+    return t >= 0 ? t : dflt;
+  }
 
   template <class T = char>
   T* at(size_t off) const { return reinterpret_cast<T*>(ws + off); }
@@ -280,8 +300,8 @@ struct Ctx {
     return reinterpret_cast<const double*>(ws + region) + elems;
   }
 
-  // 128x128 tiles for large FP32 / bf16x3 problems; FP64 stays on 64x64 (its
-  // 128x128 variant needs 64 KB of LDS and ~340 registers: one wave per SIMD).
+  // Defaults: 128x128 tiles for large FP32 / bf16x3 problems; FP64 stays on
+  // 64x64 (its 128x128 variant would spill).
   int pick_cfg(const GemmBuild& gb) const {
     if (d.prec == PREC_F64) return 0;
     for (const auto& P : gb.probs)
@@ -452,7 +472,8 @@ struct Ctx {
         for (int q = 0; q < np; q++) gb.add(p, particle_seg(bt[s], pass == 0 ? P1[q] : P2[q], ib, im));
         for (int q = 0; q < np; q++) gb.add(p, hole_seg(bt[s], pass == 0 ? H1[q] : H2[q], ib, im));
       }
-      emit_gemm(gb, -1, pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false, FAM_GEMM_W);
+      emit_gemm(gb, cfg_for(FAM_GEMM_W, pick_cfg(gb)), pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false,
+                FAM_GEMM_W);
     }
   }
 
@@ -536,7 +557,7 @@ struct Ctx {
           sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
           gb.add(q, sg);
         }
-      emit_gemm(gb, -1, GEMM_ACCUM, 1, false, FAM_GEMM_B1);
+      emit_gemm(gb, cfg_for(FAM_GEMM_B1, pick_cfg(gb)), GEMM_ACCUM, 1, false, FAM_GEMM_B1);
     }
   }
 
@@ -566,7 +587,8 @@ struct Ctx {
         gb.add(q, sg);
       }
     // FP64 partials of 64 splits cost more in the reduction than the extra blocks gain
-    emit_gemm(gb, -1, GEMM_ACCUM, 1, true, FAM_GEMM_B2, d.prec == PREC_F64 ? 16 : MAX_KSPLIT);
+    emit_gemm(gb, cfg_for(FAM_GEMM_B2, pick_cfg(gb)), GEMM_ACCUM, 1, true, FAM_GEMM_B2,
+              d.prec == PREC_F64 ? 16 : MAX_KSPLIT);
   }
 
   void emit_b3(const std::vector<Triple>& bt) {
@@ -596,7 +618,7 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, d.Op <= 32 ? 3 : 0, GEMM_ACCUM, 1, true, FAM_GEMM_B3);
+    emit_gemm(gb, cfg_for(FAM_GEMM_B3, d.Op <= 32 ? 3 : 0), GEMM_ACCUM, 1, true, FAM_GEMM_B3);
   }
 
   void emit_b4(const std::vector<Triple>& bt) {
@@ -623,7 +645,7 @@ struct Ctx {
         sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, d.Op <= 32 ? 4 : 0, GEMM_ACCUM, 1, false, FAM_GEMM_B4);
+    emit_gemm(gb, cfg_for(FAM_GEMM_B4, d.Op <= 32 ? 4 : 0), GEMM_ACCUM, 1, false, FAM_GEMM_B4);
   }
 
   void emit_vpart(const std::vector<Triple>& bt) {
@@ -895,6 +917,9 @@ struct Events4 {
   }
 };
 
+int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st);
+bool autotune_enabled();
+
 int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes, void* stream,
         long long t_begin = 0, long long t_end = -1) {
   g_last_error.clear();
@@ -905,6 +930,11 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
   if (t_begin < 0 || t_begin > t_end || t_end > ntrip)
     return fail(PYSCFAD_AMD_ERR_ARGS, "triple range must satisfy 0 <= begin <= end <= nocc(nocc+1)(nocc+2)/6");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (autotune_enabled()) {
+    const Dims d = make_dims(p.nocc, p.nvir, precision);
+    if (tuned_cfg(precision, FAM_GEMM_W, d.V, d.Op) < 0 && (rc = autotune(p.nocc, p.nvir, precision, nullptr, st)))
+      return rc;
+  }
 
   Ctx c;
   c.d = make_dims(p.nocc, p.nvir, precision);
@@ -959,6 +989,104 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
     if ((e = add_setup_time(ev.ev)) != hipSuccess) return hip_fail(e, "timing");
   }
   return cleanup(PYSCFAD_AMD_OK);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: for every contraction family, time each valid tile configuration
+// on one batch of triples of an (nocc, nvir) problem (operands: a zeroed
+// scratch workspace; only GEMM and split-K reduction launches run, so no
+// input is read) and keep the fastest in the tuning table, keyed by
+// (precision, family, V, Op).  Plans built afterwards use the table.
+const int TUNE_FAMILIES[5] = {FAM_GEMM_W, FAM_GEMM_B1, FAM_GEMM_B2, FAM_GEMM_B3, FAM_GEMM_B4};
+const int TUNE_CFGS[7] = {0, 1, 3, 4, 5, 6, -1};
+
+int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
+  int rc = check_args(nocc, nvir, precision);
+  if (rc) return rc;
+  const long long ntrip = (long long)nocc * (nocc + 1) * (nocc + 2) / 6;
+  Regions r;
+  int B = 1;
+  plan_bytes(nocc, nvir, precision, true, &r, &B);
+  const long long t0 = ntrip / 2, t1 = std::min(ntrip, t0 + B);
+  const size_t need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1);
+  char* ws = nullptr;
+  if (hipMalloc(&ws, need) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PYSCFAD_AMD_ERR_OOM, "autotune workspace of " + std::to_string(need) + " bytes");
+  }
+  const int timing = g_timing;   // trial launches are timed here, not per family
+  g_timing = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipError_t e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  if (e == hipSuccess) e = t3_zero(ws, need / 16 * 16, st);
+  // placeholder input pointers: the timed launches never read the inputs
+  static const double dummy[4] = {0, 0, 0, 0};
+  const Dims d = make_dims(nocc, nvir, precision);
+  for (int f = 0; f < 5 && e == hipSuccess; f++) {
+    const int fam = TUNE_FAMILIES[f];
+    int best = -1;
+    double best_ms = 1e300;
+    for (int ci = 0; TUNE_CFGS[ci] >= 0 && e == hipSuccess; ci++) {
+      const int cfg = TUNE_CFGS[ci];
+      if (!t3_gemm_cfg_valid(precision, cfg)) continue;
+      Ctx c;
+      c.d = d;
+      c.B = B;
+      c.r = r;
+      c.t_begin = t0;
+      c.t_end = t1;
+      c.backward = true;
+      c.ws = ws;
+      c.mo_energy = c.t1 = c.t2 = c.fock = c.ovoo = c.ovov = c.ovvv = dummy;
+      c.force_cfg[fam] = cfg;
+      c.build_plan();
+      if (!c.plan_error.empty()) continue;
+      e = hipMemcpy(ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice);
+      if (e != hipSuccess) break;
+      std::vector<size_t> idx;   // the family's GEMM launches and their reductions
+      for (size_t li = 0; li < c.launches.size(); li++) {
+        const Launch& L = c.launches[li];
+        if (L.kind == L_GEMM && L.sub == fam) {
+          idx.push_back(li);
+          if (li + 1 < c.launches.size() && c.launches[li + 1].kind == L_REDUCE) idx.push_back(li + 1);
+        }
+      }
+      if (idx.empty()) continue;
+      double total = 0.0;
+      for (int rep = 0; rep < 4 && e == hipSuccess; rep++) {   // rep 0 warms up
+        e = hipEventRecord(ev0, st);
+        for (size_t li : idx)
+          if (e == hipSuccess) e = c.execute(st, li, li + 1);
+        if (e == hipSuccess) e = hipEventRecord(ev1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(ev1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+        if (rep > 0) total += ms;
+      }
+      if (e == hipSuccess && total < best_ms) {
+        best_ms = total;
+        best = cfg;
+      }
+    }
+    if (best >= 0) {
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      g_tuned[{precision, fam, d.V, d.Op}] = best;
+    }
+    if (cfg_out) cfg_out[f] = best;
+  }
+  (void)hipStreamSynchronize(st);
+  g_timing = timing;
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  (void)hipFree(ws);
+  if (e != hipSuccess) return fail(PYSCFAD_AMD_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+  return PYSCFAD_AMD_OK;
+}
+
+bool autotune_enabled() {
+  const char* env = std::getenv("PYSCFAD_AMD_AUTOTUNE");
+  return env && env[0] == '1';
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,6 +1172,22 @@ hipError_t launch_entry(const GraphEntry& g, hipStream_t st) {
 int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_vjp, void* workspace,
                 size_t workspace_bytes, void* stream) {
   g_last_error.clear();
+  if (autotune_enabled() && n > 0 && inst) {
+    // one tuning run per size class not tuned yet (the largest nocc of the class)
+    std::map<std::pair<int, int>, int> cls;
+    for (int x = 0; x < n; x++) {
+      if (check_args(inst[x].nocc, inst[x].nvir, precision)) continue;
+      const Dims d = make_dims(inst[x].nocc, inst[x].nvir, precision);
+      if (tuned_cfg(precision, FAM_GEMM_W, d.V, d.Op) >= 0) continue;
+      auto key = std::make_pair(d.V, d.Op);
+      if (!cls.count(key) || inst[cls[key]].nocc < inst[x].nocc) cls[key] = x;
+    }
+    for (auto& kv : cls) {
+      const int rc = autotune(inst[kv.second].nocc, inst[kv.second].nvir, precision, nullptr,
+                              reinterpret_cast<hipStream_t>(stream));
+      if (rc) return rc;
+    }
+  }
   BatchPlan bp;
   int rc = plan_batch(n, inst, precision, with_vjp, bp);
   if (rc) return rc;
@@ -1220,6 +1364,11 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double* mo_energy, c
   return run(make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, t1_bar,
                           t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out),
              precision, workspace, workspace_bytes, stream);
+}
+
+int pyscfad_amd_ccsd_t_autotune(int nocc, int nvir, int precision, int* cfg_out, void* stream) {
+  g_last_error.clear();
+  return autotune(nocc, nvir, precision, cfg_out, reinterpret_cast<hipStream_t>(stream));
 }
 
 size_t pyscfad_amd_ccsd_t_batched_workspace_bytes(int n, const PyscfadAmdInstance* instances, int precision,

@@ -475,26 +475,46 @@ reduce_partials(const ReduceProb* __restrict__ probs, int nprob, long long total
 
 using namespace t3;
 
-// tile config ids: 0 -> 64x64 (2x2 waves), 1 -> 128x128 (2x2), 3 -> 64x32 (2x2), 4 -> 32x64 (2x2)
-extern "C" int t3_gemm_tile_m(int cfg) { return cfg == 1 ? 128 : (cfg == 4 ? 32 : 64); }
-extern "C" int t3_gemm_tile_n(int cfg) { return cfg == 1 ? 128 : (cfg == 3 ? 32 : 64); }
+// Tile configurations (4 waves each):
+//   0: 64x64   1: 128x128 (FP32 / bf16x3 only: FP64 would spill)   3: 64x32
+//   4: 32x64   5: 128x64   6: 64x128
+extern "C" int t3_gemm_tile_m(int cfg) {
+  switch (cfg) { case 1: case 5: return 128; case 4: return 32; default: return 64; }
+}
+extern "C" int t3_gemm_tile_n(int cfg) {
+  switch (cfg) { case 1: case 6: return 128; case 3: return 32; default: return 64; }
+}
+extern "C" int t3_gemm_cfg_valid(int prec, int cfg) {
+  return (cfg == 0 || cfg == 3 || cfg == 4 || cfg == 5 || cfg == 6 || (cfg == 1 && prec != PREC_F64)) ? 1 : 0;
+}
 extern "C" int t3_gemm_chunk(int prec) { return prec == PREC_F64 ? 16 : 32; }
 extern "C" int t3_gemm_vec(int prec) { return prec == PREC_F64 ? 2 : 4; }
+
+template <int PR>
+static void launch_cfg(int cfg, dim3 grid, hipStream_t stream, const GemmProb* probs, int nprob,
+                       const GemmSeg* segs) {
+  dim3 block(256);
+  switch (cfg) {
+    case 0: hipLaunchKernelGGL((gemm_grouped<PR, 64, 64, 2>), grid, block, 0, stream, probs, nprob, segs); break;
+    case 1:
+      if constexpr (PR != PREC_F64)
+        hipLaunchKernelGGL((gemm_grouped<PR, 128, 128, 2>), grid, block, 0, stream, probs, nprob, segs);
+      break;
+    case 3: hipLaunchKernelGGL((gemm_grouped<PR, 64, 32, 2>), grid, block, 0, stream, probs, nprob, segs); break;
+    case 4: hipLaunchKernelGGL((gemm_grouped<PR, 32, 64, 2>), grid, block, 0, stream, probs, nprob, segs); break;
+    case 5: hipLaunchKernelGGL((gemm_grouped<PR, 128, 64, 2>), grid, block, 0, stream, probs, nprob, segs); break;
+    default: hipLaunchKernelGGL((gemm_grouped<PR, 64, 128, 2>), grid, block, 0, stream, probs, nprob, segs); break;
+  }
+}
 
 extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob,
                                      const GemmSeg* segs, long long nblocks, hipStream_t stream) {
   if (nblocks <= 0) return hipSuccess;
-  if (cfg != 0 && cfg != 1 && cfg != 3 && cfg != 4) return hipErrorInvalidValue;
-  dim3 grid((unsigned)nblocks), block(256);
-#define T3_DISPATCH(PR)                                                                                    \
-  if (cfg == 0) hipLaunchKernelGGL((gemm_grouped<PR, 64, 64, 2>), grid, block, 0, stream, probs, nprob, segs); \
-  else if (cfg == 1) hipLaunchKernelGGL((gemm_grouped<PR, 128, 128, 2>), grid, block, 0, stream, probs, nprob, segs); \
-  else if (cfg == 3) hipLaunchKernelGGL((gemm_grouped<PR, 64, 32, 2>), grid, block, 0, stream, probs, nprob, segs); \
-  else hipLaunchKernelGGL((gemm_grouped<PR, 32, 64, 2>), grid, block, 0, stream, probs, nprob, segs);
-  if (prec == PREC_F64) { T3_DISPATCH(PREC_F64) }
-  else if (prec == PREC_F32) { T3_DISPATCH(PREC_F32) }
-  else { T3_DISPATCH(PREC_BF16X3) }
-#undef T3_DISPATCH
+  if (!t3_gemm_cfg_valid(prec, cfg)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)nblocks);
+  if (prec == PREC_F64) launch_cfg<PREC_F64>(cfg, grid, stream, probs, nprob, segs);
+  else if (prec == PREC_F32) launch_cfg<PREC_F32>(cfg, grid, stream, probs, nprob, segs);
+  else launch_cfg<PREC_BF16X3>(cfg, grid, stream, probs, nprob, segs);
   return hipGetLastError();
 }
 

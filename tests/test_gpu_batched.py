@@ -76,3 +76,21 @@ def test_batched_many_graphs(gpu_lib, monkeypatch):
     got = ccsd_t.batched(probs, precision="f32")
     ref = [_single(ccsd_t, d, "f32", True) for d in probs]
     _same(got, ref, True)
+
+
+def test_autotune_keeps_results(gpu_lib):
+    """The autotuner only changes tile shapes: results stay within the FP32
+    reduction-order tolerance of the default configuration, and a tuned
+    single call and a tuned batched call agree bit for bit."""
+    from pyscfad_amd.cc import ccsd_t
+    probs = _problems([(6, 72, 31)])
+    d = probs[0]
+    e0, b0 = _single(ccsd_t, d, "f32", True)
+    chosen = ccsd_t.autotune(6, 72, "f32")
+    assert all(v is not None for v in chosen.values()), chosen
+    e1, b1 = _single(ccsd_t, d, "f32", True)
+    assert abs(float(e1) - float(e0)) <= 1e-6 * abs(float(e0))
+    for x, y in zip(b0, b1):
+        assert float((x - y).abs().max()) <= 1e-5 * float(y.abs().max())
+    got = ccsd_t.batched(probs, precision="f32")
+    _same(got, [(e1, b1)], True)
