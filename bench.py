@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--triples", default=None,
                     help="BEGIN:END slice of the unique occupied triples (a bounded sample of a large "
                          "instance, or a profiling run); default: all")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 64 instances, nocc in [4,24], nvir in [40,200] "
                          "(seed 4), as one captured hipGraph, against 64 sequential FP64 calls")
@@ -167,6 +168,14 @@ def main():
         _lib.set_timing(False)
         return e, bars, t_fwd, t_bwd
 
+    # tile shapes from the autotuner (outside the timed region)
+    tuned = None
+    if not args.no_autotune:
+        log(f"[bench] autotuning tile shapes ({args.precision})")
+        tuned = ccsd_t.autotune(o, v, args.precision)
+        if not args.no_compare and args.precision != "f64":
+            ccsd_t.autotune(o, v, "f64")
+
     gemm_ms = None
     fam = {}
     for w in range(args.warmup):
@@ -254,6 +263,7 @@ def main():
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "gemm_grouped", "gemm_ms_per_step": gemm_ms,
                      "gemm_launches_per_step": fam.get("gemm", (0, 0))[1],
+                     "tiles": tuned,
                      "family_ms": {k: val[0] for k, val in fam.items() if not k.startswith("gemm_") and k != "prep"},
                      "contractions": contraction},
         "cpu_baseline": cpu,
@@ -282,6 +292,9 @@ def main_batched(args):
     torch.cuda.empty_cache()
     fe = sum(flops_energy(o, v) for o, v, _ in shapes)
     ws = torch.empty(ccsd_t.batched_workspace_bytes(probs, args.precision), dtype=torch.uint8, device="cuda")
+    # the batched call tunes every size class it has not seen (in its warm-up)
+    if not args.no_autotune:
+        os.environ["PYSCFAD_AMD_AUTOTUNE"] = "1"
 
     log("[bench] warm-up (graph capture)")
     out = ccsd_t.batched(probs, precision=args.precision, workspace=ws)
@@ -304,6 +317,7 @@ def main_batched(args):
     peak = MI355X_PEAK_TFLOPS[args.precision]
     achieved = 4.0 * fe / (gemm_ms / 1e3) / 1e12
 
+    os.environ.pop("PYSCFAD_AMD_AUTOTUNE", None)   # the FP64 baseline runs plain, untuned
     log("[bench] 64 sequential FP64 single-instance calls")
     ws64 = torch.empty(max(ccsd_t.workspace_bytes(o, v, "f64") for o, v, _ in shapes), dtype=torch.uint8,
                        device="cuda")

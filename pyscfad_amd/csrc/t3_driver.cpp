@@ -1025,9 +1025,12 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   const Dims d = make_dims(nocc, nvir, precision);
   for (int f = 0; f < 5 && e == hipSuccess; f++) {
     const int fam = TUNE_FAMILIES[f];
-    int best = -1;
-    double best_ms = 1e300;
-    for (int ci = 0; TUNE_CFGS[ci] >= 0 && e == hipSuccess; ci++) {
+    // one plan per candidate; the candidates are timed round-robin and each
+    // keeps its best round, so clock ramps and noise hit them alike
+    std::vector<int> cands;
+    std::vector<Ctx> plans;
+    std::vector<std::vector<size_t>> runs;
+    for (int ci = 0; TUNE_CFGS[ci] >= 0; ci++) {
       const int cfg = TUNE_CFGS[ci];
       if (!t3_gemm_cfg_valid(precision, cfg)) continue;
       Ctx c;
@@ -1042,8 +1045,6 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
       c.force_cfg[fam] = cfg;
       c.build_plan();
       if (!c.plan_error.empty()) continue;
-      e = hipMemcpy(ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice);
-      if (e != hipSuccess) break;
       std::vector<size_t> idx;   // the family's GEMM launches and their reductions
       for (size_t li = 0; li < c.launches.size(); li++) {
         const Launch& L = c.launches[li];
@@ -1053,23 +1054,31 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
         }
       }
       if (idx.empty()) continue;
-      double total = 0.0;
-      for (int rep = 0; rep < 4 && e == hipSuccess; rep++) {   // rep 0 warms up
-        e = hipEventRecord(ev0, st);
-        for (size_t li : idx)
+      cands.push_back(cfg);
+      plans.push_back(std::move(c));
+      runs.push_back(std::move(idx));
+    }
+    std::vector<double> best_ms(cands.size(), 1e300);
+    const int ROUNDS = 5;   // round 0 warms up
+    for (int round = 0; round < ROUNDS && e == hipSuccess; round++) {
+      for (size_t x = 0; x < cands.size() && e == hipSuccess; x++) {
+        Ctx& c = plans[x];
+        e = hipMemcpy(ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipEventRecord(ev0, st);
+        for (size_t li : runs[x])
           if (e == hipSuccess) e = c.execute(st, li, li + 1);
         if (e == hipSuccess) e = hipEventRecord(ev1, st);
         if (e == hipSuccess) e = hipEventSynchronize(ev1);
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
-        if (rep > 0) total += ms;
-      }
-      if (e == hipSuccess && total < best_ms) {
-        best_ms = total;
-        best = cfg;
+        if (round > 0 && e == hipSuccess) best_ms[x] = std::min(best_ms[x], (double)ms);
       }
     }
-    if (best >= 0) {
+    int best = -1;
+    double bms = 1e300;
+    for (size_t x = 0; x < cands.size(); x++)
+      if (best_ms[x] < bms) { bms = best_ms[x]; best = cands[x]; }
+    if (best >= 0 && e == hipSuccess) {
       std::lock_guard<std::mutex> lock(g_tune_mu);
       g_tuned[{precision, fam, d.V, d.Op}] = best;
     }

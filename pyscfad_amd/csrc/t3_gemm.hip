@@ -3,8 +3,8 @@
 //   PREC_F32    : v_mfma_f32_16x16x4f32, FP32 operands and accumulation
 //   PREC_BF16X3 : FP32 operands split on the way into LDS as x = hi + lo with
 //                 hi, lo in bf16; three v_mfma_f32_16x16x32_bf16 per K-step
-//                 (hi*hi + hi*lo + lo*hi), FP32 accumulation (Henry, Tang and
-//                 Heinecke, ARITH 2019, "bf16x3").
+//                 (hi*hi, then lo*hi + hi*lo), FP32 accumulation (Henry, Tang
+//                 and Heinecke, ARITH 2019, "bf16x3").
 //
 // Every contraction of the (T) energy and its VJP is expressed as a list of
 // problems C[b][m][n] (op)= alpha * sum_seg scale_seg * sum_k A_seg[b][m][k] B_seg[b][k][n]
@@ -340,17 +340,22 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       load_next(sa[p], sb[p]);
       const T* la = lds_a[p] + (wm0 + l16) * LDK + g * KG;
       const T* lb = lds_b[p] + (wn0 + l16) * LDK + g * KG;
+      // reads in the order the MFMAs consume them (k-vector u outermost), so the
+      // first MFMAs wait only for the first half of the reads
       V av[FM][KG / VEC], bv[FN][KG / VEC];
 #pragma unroll
-      for (int i = 0; i < FM; i++)
+      for (int u = 0; u < KG / VEC; u++) {
 #pragma unroll
-        for (int u = 0; u < KG / VEC; u++) av[i][u] = *reinterpret_cast<const V*>(la + i * 16 * LDK + u * VEC);
+        for (int i = 0; i < FM; i++) av[i][u] = *reinterpret_cast<const V*>(la + i * 16 * LDK + u * VEC);
 #pragma unroll
-      for (int j = 0; j < FN; j++)
+        for (int j = 0; j < FN; j++) bv[j][u] = *reinterpret_cast<const V*>(lb + j * 16 * LDK + u * VEC);
+        __builtin_amdgcn_sched_barrier(0);   // keep the reads in k-vector order
+      }
 #pragma unroll
-        for (int u = 0; u < KG / VEC; u++) bv[j][u] = *reinterpret_cast<const V*>(lb + j * 16 * LDK + u * VEC);
-#pragma unroll
-      for (int ks = 0; ks < KG; ks++)
+      for (int ks = 0; ks < KG; ks++) {
+        // the MFMAs of the second k-vector stay after those of the first, so
+        // the first half waits only for its own reads
+        if (ks == VEC) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
@@ -359,6 +364,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
             if constexpr (PREC == PREC_F64) acc[i][j] = mfma_f64(a, b, acc[i][j]);
             else acc[i][j] = mfma_f32(a, b, acc[i][j]);
           }
+      }
       if (it + 1 < nch) {
         sa[p ^ 1].template store<LDK>(lds_a[p ^ 1]);
         sb[p ^ 1].template store<LDK>(lds_b[p ^ 1]);
@@ -387,26 +393,29 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
       load_next(sa[p], sb[p]);
+      // hi parts first: the hi*hi products wait only for them
       bf8v ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
-      for (int i = 0; i < FM; i++) {
-        const int row = wm0 + i * 16 + l16;
-        ah[i] = *reinterpret_cast<const bf8v*>(&lah[p][row * LDK + ko]);
-        al[i] = *reinterpret_cast<const bf8v*>(&lal[p][row * LDK + ko]);
-      }
+      for (int i = 0; i < FM; i++) ah[i] = *reinterpret_cast<const bf8v*>(&lah[p][(wm0 + i * 16 + l16) * LDK + ko]);
 #pragma unroll
-      for (int j = 0; j < FN; j++) {
-        const int row = wn0 + j * 16 + l16;
-        bh[j] = *reinterpret_cast<const bf8v*>(&lbh[p][row * LDK + ko]);
-        bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][row * LDK + ko]);
-      }
+      for (int j = 0; j < FN; j++) bh[j] = *reinterpret_cast<const bf8v*>(&lbh[p][(wn0 + j * 16 + l16) * LDK + ko]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < FM; i++) al[i] = *reinterpret_cast<const bf8v*>(&lal[p][(wm0 + i * 16 + l16) * LDK + ko]);
+#pragma unroll
+      for (int j = 0; j < FN; j++) bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][(wn0 + j * 16 + l16) * LDK + ko]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < FN; j++) {
           acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
           acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
-          acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         }
       if (it + 1 < nch) {
         sa[p ^ 1].template store_split<LDK>(lah[p ^ 1], lal[p ^ 1]);
