@@ -291,10 +291,12 @@ def main_batched(args):
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     fe = sum(flops_energy(o, v) for o, v, _ in shapes)
-    ws = torch.empty(ccsd_t.batched_workspace_bytes(probs, args.precision), dtype=torch.uint8, device="cuda")
-    # the batched call tunes every size class it has not seen (in its warm-up)
+    tuned = None
     if not args.no_autotune:
-        os.environ["PYSCFAD_AMD_AUTOTUNE"] = "1"
+        # tile shapes and batch sizes of every size class, before the workspace is sized
+        log("[bench] autotuning the size classes")
+        tuned = {f"{o}x{v}": t for (o, v), t in ccsd_t.autotune_problems(probs, args.precision).items()}
+    ws = torch.empty(ccsd_t.batched_workspace_bytes(probs, args.precision), dtype=torch.uint8, device="cuda")
 
     log("[bench] warm-up (graph capture)")
     out = ccsd_t.batched(probs, precision=args.precision, workspace=ws)
@@ -317,8 +319,7 @@ def main_batched(args):
     peak = MI355X_PEAK_TFLOPS[args.precision]
     achieved = 4.0 * fe / (gemm_ms / 1e3) / 1e12
 
-    os.environ.pop("PYSCFAD_AMD_AUTOTUNE", None)   # the FP64 baseline runs plain, untuned
-    log("[bench] 64 sequential FP64 single-instance calls")
+    log("[bench] 64 sequential FP64 single-instance calls (default tiles)")
     ws64 = torch.empty(max(ccsd_t.workspace_bytes(o, v, "f64") for o, v, _ in shapes), dtype=torch.uint8,
                        device="cuda")
     keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")
@@ -363,7 +364,7 @@ def main_batched(args):
         "sequential_f64": {"s_per_batch": dt64, "tflops": 3.0 * fe / dt64 / 1e12, "speedup": dt64 / dt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "kernel": "gemm_grouped",
-                     "gemm_ms_per_step": gemm_ms,
+                     "gemm_ms_per_step": gemm_ms, "tuned_classes": tuned,
                      "family_ms": {k: val[0] for k, val in fam.items() if not k.startswith("gemm_")}},
         "cpu_baseline": None,
         "accuracy_vs_f64": accuracy,

@@ -93,13 +93,16 @@ typedef struct PyscfadAmdInstance {
   double *e_out; /* may be NULL when with_vjp = 1 */
 } PyscfadAmdInstance;
 
-/* Autotuner (no counterpart in the reference): times every tile
- * configuration of each contraction family (W build; the ovvv, t2-particle,
- * ovoo and t2-hole adjoints) on one batch of triples of an (nocc, nvir)
- * problem in a scratch workspace, and keeps the fastest for every later call
- * of the same padded size and precision in this process.  cfg_out (may be
- * NULL) receives the 5 chosen configuration ids.  Setting the environment
- * variable PYSCFAD_AMD_AUTOTUNE=1 makes every call tune sizes not seen yet. */
+/* Autotuner (no counterpart in the reference): picks the number of triples
+ * per batch (the default or twice it), then times every tile configuration of
+ * each contraction family (W build; the ovvv, t2-particle, ovoo and t2-hole
+ * adjoints) on one batch of triples of an (nocc, nvir) problem in a scratch
+ * workspace, and keeps the fastest for every later call of the same padded
+ * size and precision in this process.  cfg_out (may be NULL) receives 6 ints:
+ * the 5 chosen configuration ids and the batch size.  The workspace sizes
+ * above follow the tuned batch size: query them after tuning.  Setting the
+ * environment variable PYSCFAD_AMD_AUTOTUNE=1 makes every call tune sizes
+ * not seen yet. */
 int pyscfad_amd_ccsd_t_autotune(int nocc, int nvir, int precision, int *cfg_out, void *stream);
 
 /* Workspace of pyscfad_amd_ccsd_t_batched for these instances; 0 if invalid. */

@@ -191,14 +191,30 @@ TILE_CONFIGS = {0: (64, 64), 1: (128, 128), 3: (64, 32), 4: (32, 64), 5: (128, 6
 
 
 def autotune(nocc, nvir, precision=DEFAULT_PRECISION, stream=None):
-    """Time every tile configuration of each contraction family for this
-    (nocc, nvir) size and keep the fastest for later calls in this process.
-    Returns {family: (tile_m, tile_n)}."""
+    """Pick the batch size and time every tile configuration of each
+    contraction family for this (nocc, nvir) size; later calls in this
+    process use the fastest (query workspace sizes after tuning).
+    Returns {family: (tile_m, tile_n), "batch": triples per batch}."""
     import ctypes
-    out = (ctypes.c_int * 5)()
+    out = (ctypes.c_int * 6)()
     s = stream if stream is not None else torch.cuda.current_stream()
     _lib.check(_lib.load().pyscfad_amd_ccsd_t_autotune(nocc, nvir, _prec(precision), out, s.cuda_stream))
-    return {f: TILE_CONFIGS.get(out[x]) for x, f in enumerate(TUNED_FAMILIES)}
+    res = {f: TILE_CONFIGS.get(out[x]) for x, f in enumerate(TUNED_FAMILIES)}
+    res["batch"] = out[5]
+    return res
+
+
+def autotune_problems(problems, precision=DEFAULT_PRECISION, stream=None):
+    """Autotune every padded size class of a list of problems (the largest
+    nocc of each class stands for it); returns {(nocc, nvir): choice}."""
+    bk = 16 if _prec(precision) == _lib.PREC_F64 else 32
+    cls = {}
+    for pr in problems:
+        o, v = pr["t1"].shape
+        key = ((v + 31) // 32, (o + bk - 1) // bk)
+        if key not in cls or cls[key][0] < o:
+            cls[key] = (o, v)
+    return {ov: autotune(ov[0], ov[1], precision, stream) for ov in cls.values()}
 
 
 def workspace_bytes(nocc, nvir, precision=DEFAULT_PRECISION, with_vjp=True):

@@ -86,10 +86,18 @@ thread_local long long g_family_launches[FAM_COUNT];
 std::mutex g_tune_mu;
 std::map<std::vector<int>, int> g_tuned;
 
+std::map<std::vector<int>, int> g_tuned_batch;   // triples per batch, same key without the family
+
 int tuned_cfg(int prec, int fam, int V, int Op) {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   auto it = g_tuned.find({prec, fam, V, Op});
   return it == g_tuned.end() ? -1 : it->second;
+}
+
+int tuned_batch(int prec, int V, int Op) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tuned_batch.find({prec, V, Op});
+  return it == g_tuned_batch.end() ? -1 : it->second;
 }
 
 int fail(int code, const std::string& msg) {
@@ -149,15 +157,22 @@ struct Regions {
 // H) resident in the 256 MB last-level cache while the adjoint contractions
 // re-read them.  The batch's cubes are held to about 96 MB: one triple at
 // v = 256, dozens for small instances.
-// PYSCFAD_AMD_BATCH (1..64) overrides the choice, for tuning.
-int choose_batch(const Dims& d) {
+// The autotuner may replace it with a measured choice; PYSCFAD_AMD_BATCH
+// (1..64) overrides both, for experiments.  The workspace size depends on it:
+// query the size after tuning.
+int default_batch(const Dims& d) {
   const double cube = (double)d.V * d.V * d.V * (double)d.elt;
+  const long long b = (long long)(96.0e6 / cube);
+  return (int)std::max<long long>(1, std::min<long long>(b, 64));
+}
+
+int choose_batch(const Dims& d) {
   if (const char* env = std::getenv("PYSCFAD_AMD_BATCH")) {
     const long long x = std::atoll(env);
     if (x >= 1 && x <= 64) return (int)x;
   }
-  const long long b = (long long)(96.0e6 / cube);
-  return (int)std::max<long long>(1, std::min<long long>(b, 64));
+  const int t = tuned_batch(d.prec, d.V, d.Op);
+  return t > 0 ? t : default_batch(d);
 }
 
 Regions plan_regions(const Dims& d, int B, bool backward) {
@@ -790,10 +805,10 @@ int check_args(int nocc, int nvir, int precision) {
 
 // Size the descriptor blob by building the plan against a null workspace.
 size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B,
-                  long long t_begin = 0, long long t_end = -1) {
+                  long long t_begin = 0, long long t_end = -1, int batch = -1) {
   Ctx c;
   c.d = make_dims(nocc, nvir, precision);
-  c.B = choose_batch(c.d);
+  c.B = batch > 0 ? batch : choose_batch(c.d);
   c.backward = backward;
   c.t_begin = t_begin;
   c.t_end = t_end;
@@ -1000,10 +1015,82 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
 const int TUNE_FAMILIES[5] = {FAM_GEMM_W, FAM_GEMM_B1, FAM_GEMM_B2, FAM_GEMM_B3, FAM_GEMM_B4};
 const int TUNE_CFGS[7] = {0, 1, 3, 4, 5, 6, -1};
 
+// Batch size: the default and twice it, timed on the same 2x-default
+// triples through their GEMM and reduction launches (the launches whose
+// speed the batch size moves: block counts and cache reuse of the cubes).
+int tune_batch(int nocc, int nvir, int precision, hipStream_t st) {
+  const Dims d = make_dims(nocc, nvir, precision);
+  const long long ntrip = (long long)nocc * (nocc + 1) * (nocc + 2) / 6;
+  const int b0 = default_batch(d);
+  if (2 * b0 > 64 || ntrip < 2 * b0) return b0;
+  const long long t0 = ntrip / 2 - std::min<long long>(ntrip / 2, b0), t1 = t0 + 2 * b0;
+  const int cand[2] = {b0, 2 * b0};
+  Regions rr[2];
+  size_t need = 0;
+  for (int x = 0; x < 2; x++) {
+    int B = cand[x];
+    need = std::max(need, plan_bytes(nocc, nvir, precision, true, &rr[x], &B, t0, t1, cand[x]));
+  }
+  char* ws = nullptr;
+  if (hipMalloc(&ws, need) != hipSuccess) {
+    (void)hipGetLastError();
+    return b0;
+  }
+  static const double dummy[4] = {0, 0, 0, 0};
+  Ctx plans[2];
+  hipError_t e = t3_zero(ws, need / 16 * 16, st);
+  for (int x = 0; x < 2; x++) {
+    Ctx& c = plans[x];
+    c.d = d;
+    c.B = cand[x];
+    c.r = rr[x];
+    c.t_begin = t0;
+    c.t_end = t1;
+    c.backward = true;
+    c.ws = ws;
+    c.mo_energy = c.t1 = c.t2 = c.fock = c.ovoo = c.ovov = c.ovvv = dummy;
+    c.build_plan();
+  }
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  double best[2] = {1e300, 1e300};
+  for (int round = 0; round < 4 && e == hipSuccess; round++) {
+    for (int x = 0; x < 2 && e == hipSuccess; x++) {
+      Ctx& c = plans[x];
+      if (!c.plan_error.empty()) continue;
+      e = hipMemcpy(ws + c.r.desc, c.blob.data.data(), c.blob.data.size(), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipEventRecord(ev0, st);
+      for (size_t li = 0; li < c.launches.size() && e == hipSuccess; li++)
+        if (c.launches[li].kind == L_GEMM || c.launches[li].kind == L_REDUCE) e = c.execute(st, li, li + 1);
+      if (e == hipSuccess) e = hipEventRecord(ev1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(ev1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+      if (round > 0 && e == hipSuccess) best[x] = std::min(best[x], (double)ms);
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  (void)hipFree(ws);
+  (void)hipGetLastError();
+  return (e == hipSuccess && best[1] < best[0]) ? cand[1] : b0;
+}
+
 int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   int rc = check_args(nocc, nvir, precision);
   if (rc) return rc;
   const long long ntrip = (long long)nocc * (nocc + 1) * (nocc + 2) / 6;
+  {
+    const int timing = g_timing;
+    g_timing = 0;
+    const Dims d0 = make_dims(nocc, nvir, precision);
+    const int bt = tune_batch(nocc, nvir, precision, st);
+    g_timing = timing;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned_batch[{precision, d0.V, d0.Op}] = bt;
+  }
   Regions r;
   int B = 1;
   plan_bytes(nocc, nvir, precision, true, &r, &B);
@@ -1084,6 +1171,7 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
     }
     if (cfg_out) cfg_out[f] = best;
   }
+  if (cfg_out) cfg_out[5] = B;
   (void)hipStreamSynchronize(st);
   g_timing = timing;
   if (ev0) (void)hipEventDestroy(ev0);
