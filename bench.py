@@ -45,6 +45,9 @@ def parse():
                     help="BEGIN:END slice of the unique occupied triples (a bounded sample of a large "
                          "instance, or a profiling run); default: all")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--energy-only", action="store_true",
+                    help="time the forward E(T) alone (F_E credited), for instances whose VJP "
+                         "workspace does not fit")
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 64 instances, nocc in [4,24], nvir in [40,200] "
                          "(seed 4), as one captured hipGraph, against 64 sequential FP64 calls")
@@ -84,6 +87,20 @@ def cpu_baseline(inst, nocc, nvir, target_s):
     return {"value": rate, "unit": "TFLOP/s", "cores": orc.num_threads(), "kind": "port",
             "sample": f"energy+VJP of a>=b>=c jobs [{j0}, {j0 + n}) of {njobs_all} "
                       f"(nocc={nocc}, nvir={nvir}) in {t:.1f} s, FP64, credited 3*F_E per job"}
+
+
+def measured_traffic(precision, nocc, nvir):
+    """HBM bytes per gemm_grouped launch from the committed FETCH_SIZE / WRITE_SIZE
+    profile of this workload (rocprofv3 --pmc runs, calibrated on a copy kernel),
+    or None when none was recorded for it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_grouped_traffic.json")
+    try:
+        rec = json.load(open(path)).get(precision)
+    except (OSError, ValueError):
+        return None, None
+    if not rec or rec.get("nocc") != nocc or rec.get("nvir") != nvir:
+        return None, None
+    return rec["bytes_per_launch"], rec.get("source")
 
 
 def slice_flops(nocc, nvir, t0, t1):
@@ -155,7 +172,7 @@ def main():
         if prec not in wsp:
             wsp.clear()
             torch.cuda.empty_cache()
-            nbytes = ccsd_t.workspace_bytes(o, v, prec, with_vjp=True)
+            nbytes = ccsd_t.workspace_bytes(o, v, prec, with_vjp=not args.energy_only)
             wsp[prec] = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         return wsp[prec]
 
@@ -163,6 +180,9 @@ def main():
         _lib.set_timing(timing)
         e = ccsd_t.energy(*a, precision=prec, triples=sl, workspace=workspace(prec))
         t_fwd = _lib.last_timing() if timing else None
+        if args.energy_only:
+            _lib.set_timing(False)
+            return e, None, t_fwd, ({k: (0.0, 0) for k in t_fwd} if timing else None)
         bars = ccsd_t.energy_vjp(*a, et_bar=1.0, precision=prec, triples=sl, workspace=workspace(prec))
         t_bwd = _lib.last_timing() if timing else None
         _lib.set_timing(False)
@@ -191,18 +211,22 @@ def main():
         e, bars, _, _ = step(args.precision)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - c0) / args.steps
-    tflops = 3.0 * fe / dt / 1e12
+    credit = 1.0 if args.energy_only else 3.0    # forward only: F_E; energy + VJP: 3 F_E
+    tflops = credit * fe / dt / 1e12
     if gemm_ms is None:
         _, _, tf, tb = step(args.precision, timing=True)
         fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf}
         gemm_ms = fam["gemm"][0]
     peak = MI355X_PEAK_TFLOPS[args.precision]
-    achieved = 4.0 * fe / (gemm_ms / 1e3) / 1e12
+    achieved = (1.0 if args.energy_only else 4.0) * fe / (gemm_ms / 1e3) / 1e12
     # algorithmic flops of each contraction: W build (forward + recompute)
     # 2 F_E; particle-term adjoints (ovvv, t2) F_E v/(v+o) each; hole-term
     # adjoints (ovoo, t2) F_E o/(v+o) each
     part, hole = fe * v / (v + o), fe * o / (v + o)
     sub_flops = {"gemm_w": 2.0 * fe, "gemm_b1": part, "gemm_b2": part, "gemm_b3": hole, "gemm_b4": hole}
+    if args.energy_only:
+        sub_flops = {"gemm_w": fe}
+    traffic, traffic_src = measured_traffic(args.precision, o, v)
     # a slice of the triples carries the whole per-call cost (input re-layout,
     # accumulator clears, cotangent write-back, host planning); the full solve
     # is projected as that fixed part plus the per-triple kernels scaled by flops
@@ -212,12 +236,18 @@ def main():
     projected_s = None
     if (t0, t1) != (0, ntrip):
         projected_s = max(dt - per_triple_s, 0.0) + per_triple_s * full / fe
+    projected_tflops = (credit * full / projected_s / 1e12) if projected_s else None
     contraction = {k: {"ms": fam[k][0], "launches": fam[k][1],
                        "tflops": (f / (fam[k][0] / 1e3) / 1e12) if fam[k][0] > 0 else None}
                    for k, f in sub_flops.items() if k in fam}
 
     accuracy = None
-    if not args.no_compare and args.precision != "f64":
+    if args.energy_only and not args.no_compare and args.precision != "f64":
+        wsp.clear()
+        torch.cuda.empty_cache()
+        e64 = ccsd_t.energy(*a, precision="f64", triples=sl, workspace=workspace("f64"))
+        accuracy = {"dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64)}
+    elif not args.no_compare and args.precision != "f64":
         # large instances: park the reduced-precision cotangents in host memory
         # while the FP64 step runs, and compare there
         on_host = o * v ** 3 * 8 > 8e9
@@ -239,7 +269,8 @@ def main():
         cpu = cpu_baseline(inst, o, v, args.cpu_seconds)
 
     out = {
-        "metric": "(T) energy+VJP TFLOP/s (3*F_E credited)",
+        "metric": ("(T) energy TFLOP/s (F_E credited)" if args.energy_only
+                   else "(T) energy+VJP TFLOP/s (3*F_E credited)"),
         "value": tflops,
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -248,8 +279,9 @@ def main():
         "ms_per_step": dt * 1e3,
         "s_per_solve": dt,
         "setup_ms": setup_ms,
+        "peak_hbm_GB": torch.cuda.max_memory_allocated() / 1e9,
         "projected_full_solve_s": projected_s,
-        "projected_full_solve_tflops": (3.0 * full / projected_s / 1e12) if projected_s else None,
+        "projected_full_solve_tflops": projected_tflops,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -260,7 +292,8 @@ def main():
                                + ("" if (t0, t1) == (0, ntrip) else f", unique triples [{t0},{t1}) of {ntrip}"),
                    "precision": args.precision, "E_T": float(e)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": traffic, "traffic_unit": "bytes per launch",
+                     "traffic_source": traffic_src,
                      "kernel": "gemm_grouped", "gemm_ms_per_step": gemm_ms,
                      "gemm_launches_per_step": fam.get("gemm", (0, 0))[1],
                      "tiles": tuned,

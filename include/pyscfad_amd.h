@@ -98,8 +98,10 @@ typedef struct PyscfadAmdInstance {
  * each contraction family (W build; the ovvv, t2-particle, ovoo and t2-hole
  * adjoints) on one batch of triples of an (nocc, nvir) problem in a scratch
  * workspace, and keeps the fastest for every later call of the same padded
- * size and precision in this process.  cfg_out (may be NULL) receives 6 ints:
- * the 5 chosen configuration ids and the batch size.  The workspace sizes
+ * size and precision in this process.  When the backward workspace does not
+ * fit in device memory, only the W build is tuned (the others report -1).
+ * cfg_out (may be NULL) receives 6 ints: the 5 chosen configuration ids and
+ * the batch size.  The workspace sizes
  * above follow the tuned batch size: query them after tuning.  Setting the
  * environment variable PYSCFAD_AMD_AUTOTUNE=1 makes every call tune sizes
  * not seen yet. */

@@ -194,7 +194,9 @@ def autotune(nocc, nvir, precision=DEFAULT_PRECISION, stream=None):
     """Pick the batch size and time every tile configuration of each
     contraction family for this (nocc, nvir) size; later calls in this
     process use the fastest (query workspace sizes after tuning).
-    Returns {family: (tile_m, tile_n), "batch": triples per batch}."""
+    Returns {family: (tile_m, tile_n) or None if left at the default (the
+    backward families when only the forward workspace fits), "batch": triples
+    per batch}."""
     import ctypes
     out = (ctypes.c_int * 6)()
     s = stream if stream is not None else torch.cuda.current_stream()

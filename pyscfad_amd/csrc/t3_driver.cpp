@@ -1095,11 +1095,18 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   int B = 1;
   plan_bytes(nocc, nvir, precision, true, &r, &B);
   const long long t0 = ntrip / 2, t1 = std::min(ntrip, t0 + B);
-  const size_t need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1);
+  // the backward plan when its workspace fits, else the forward plan (W build only)
+  bool bw = true;
+  size_t need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1);
   char* ws = nullptr;
   if (hipMalloc(&ws, need) != hipSuccess) {
     (void)hipGetLastError();
-    return fail(PYSCFAD_AMD_ERR_OOM, "autotune workspace of " + std::to_string(need) + " bytes");
+    bw = false;
+    need = plan_bytes(nocc, nvir, precision, false, &r, &B, t0, t1);
+    if (hipMalloc(&ws, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PYSCFAD_AMD_ERR_OOM, "autotune workspace of " + std::to_string(need) + " bytes");
+    }
   }
   const int timing = g_timing;   // trial launches are timed here, not per family
   g_timing = 0;
@@ -1112,6 +1119,10 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   const Dims d = make_dims(nocc, nvir, precision);
   for (int f = 0; f < 5 && e == hipSuccess; f++) {
     const int fam = TUNE_FAMILIES[f];
+    if (!bw && fam != FAM_GEMM_W) {   // untuned: the defaults stay
+      if (cfg_out) cfg_out[f] = -1;
+      continue;
+    }
     // one plan per candidate; the candidates are timed round-robin and each
     // keeps its best round, so clock ramps and noise hit them alike
     std::vector<int> cands;
@@ -1126,7 +1137,7 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
       c.r = r;
       c.t_begin = t0;
       c.t_end = t1;
-      c.backward = true;
+      c.backward = bw;
       c.ws = ws;
       c.mo_energy = c.t1 = c.t2 = c.fock = c.ovoo = c.ovov = c.ovvv = dummy;
       c.force_cfg[fam] = cfg;

@@ -87,7 +87,7 @@ def test_autotune_keeps_results(gpu_lib):
     d = probs[0]
     e0, b0 = _single(ccsd_t, d, "f32", True)
     chosen = ccsd_t.autotune(6, 72, "f32")
-    assert all(v is not None for v in chosen.values()), chosen
+    assert all(v is not None for v in chosen.values()), chosen   # the backward fits here
     assert chosen["batch"] >= 1
     e1, b1 = _single(ccsd_t, d, "f32", True)
     assert abs(float(e1) - float(e0)) <= 1e-6 * abs(float(e0))
