@@ -148,6 +148,7 @@ def main():
     inst = make_instance(o, v, args.seed, device="cuda")
     torch.cuda.synchronize()
     torch.cuda.empty_cache()   # return the generator's temporaries before the workspace is allocated
+    torch.cuda.reset_peak_memory_stats()   # peak_hbm_GB: resident inputs + workspace + outputs
     keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")
     a = [inst[k] for k in keys]
     ntrip = ccsd_t.num_triples(o)
