@@ -86,7 +86,14 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   typedef TP tpv __attribute__((ext_vector_type(VW)));
   typedef double dv2 __attribute__((ext_vector_type(2)));
   const EpiTriple tr = triples[blockIdx.y];
-  const int cube = blockIdx.x;
+  // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+  // range of cube ids, so that neighbouring tiles (which share 128-byte lines
+  // of w0, G and H along the last index) meet in the same L2.
+  int cube = blockIdx.x;
+  {
+    const int nb = gridDim.x, q = nb / 8;
+    if (cube < 8 * q) cube = (cube % 8) * q + cube / 8;
+  }
   int TA, TB, TC;
   decode_cube(cube, TA, TB, TC);
   const int V = args.V, v = args.nvir;
