@@ -145,7 +145,9 @@ struct Regions {
   size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, fvoP, energy;
   size_t ovvv_acc, t2acc, t2hacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
   size_t slots, slot_bytes;
-  size_t slot_w0, slot_G, slot_H, slot_part, slot_vp;
+  size_t slot_w0, slot_H, slot_part, slot_vp;
+  size_t gslots, gslot_bytes;   // G cubes: B x S of them (S batches per ovvv-adjoint group)
+  int S;
   size_t split, split_bytes;
   size_t desc, desc_bytes;
   size_t total;
@@ -175,9 +177,21 @@ int choose_batch(const Dims& d) {
   return t > 0 ? t : default_batch(d);
 }
 
+// Batches per ovvv-adjoint group (backward only): the ovvv adjoint of S
+// consecutive batches runs as one launch, whose problems merge the triples
+// that share an output slab, so the FP64 slab is read and written once per
+// group instead of once per batch.  Their G cubes stay resident meanwhile;
+// the extra cubes are held to about 512 MB.
+int b1_group(const Dims& d, int B) {
+  const double cube = (double)d.V * d.V * d.V * (double)d.elt;
+  const long long g = (long long)(512.0e6 / (cube * B));
+  return (int)std::max<long long>(1, std::min<long long>(g, 4));
+}
+
 Regions plan_regions(const Dims& d, int B, bool backward) {
   Layout L;
   Regions r{};
+  r.S = backward ? b1_group(d, B) : 1;
   const size_t V = d.V, o = d.o, Op = d.Op;
   r.ovvvT = L.take(o * V * V * V * d.elt);
   r.t2T = L.take(Op * Op * V * V * d.elt);
@@ -199,14 +213,13 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   }
   Layout S;
   r.slot_w0 = S.take(V * V * V * d.elt);
-  if (backward) {
-    r.slot_G = S.take(V * V * V * d.elt);
-    r.slot_H = S.take(V * V * V * d.elt);
-  }
+  if (backward) r.slot_H = S.take(V * V * V * d.elt);
   r.slot_vp = S.take((size_t)VP_TOTAL * V * V * 8);
   r.slot_part = S.take((size_t)d.ncube * EPI_PSTRIDE * 8);
   r.slot_bytes = S.off;
   r.slots = L.take(r.slot_bytes * B);
+  r.gslot_bytes = backward ? (size_t)round_up((long long)(V * V * V * d.elt), 256) : 0;
+  r.gslots = L.take(r.gslot_bytes * B * r.S);
   // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6 per triple
   r.split_bytes = backward ? (size_t)6 * B * MAX_KSPLIT * V * std::max(V, Op) * d.elt + (size_t)6 * B * 256 : 0;
   r.split = L.take(r.split_bytes);
@@ -307,6 +320,8 @@ struct Ctx {
   template <class T = char>
   T* at(size_t off) const { return reinterpret_cast<T*>(ws + off); }
   char* slot(int s, size_t field) const { return ws + r.slots + (size_t)s * r.slot_bytes + field; }
+  int gbase = 0;   // first G cube of the current batch within its ovvv-adjoint group
+  char* gslot(int s) const { return ws + r.gslots + (size_t)(gbase + s) * r.gslot_bytes; }
   long long str(int pos) const {
     return pos == 0 ? (long long)d.V * d.V : (pos == 1 ? (long long)d.V : 1LL);
   }
@@ -499,7 +514,7 @@ struct Ctx {
       e.i = bt[s].t[0]; e.j = bt[s].t[1]; e.k = bt[s].t[2];
       e.fo = bt[s].fo;
       e.w0 = slot((int)s, r.slot_w0);
-      e.G = bw ? slot((int)s, r.slot_G) : nullptr;
+      e.G = bw ? gslot((int)s) : nullptr;
       e.H = bw ? slot((int)s, r.slot_H) : nullptr;
       e.part = reinterpret_cast<double*>(slot((int)s, r.slot_part));
       e.cm = reinterpret_cast<const double*>(slot((int)s, r.slot_vp)) + (size_t)VP_CM * d.V * d.V;
@@ -563,7 +578,7 @@ struct Ctx {
           if (orient == 0) { P.c_sb = V * V; P.c_sm = V; } else { P.c_sb = V; P.c_sm = V * V; }
           const int q = gb.problem(tr.t[p[0]], P);
           GemmSeg sg{};
-          sg.A = slot((int)s, r.slot_G);
+          sg.A = gslot((int)s);
           if (orient == 0) { sg.a_sb = str(p[0]); sg.a_sm = str(p[1]); }
           else { sg.a_sb = str(p[1]); sg.a_sm = str(p[0]); }
           sg.a_sk2 = str(p[2]);
@@ -589,7 +604,7 @@ struct Ctx {
         P.c_sm = V; P.c_sn = 1; P.alpha = 1.0;
         const int q = gb.problem((long long)tr.t[p[2]] * 4096 + tr.t[p[1]], P);
         GemmSeg sg{};
-        sg.A = slot((int)s, r.slot_G);
+        sg.A = gslot((int)s);
         sg.a_sm = str(p[2]);
         sg.B = el(r.ovvvT, (long long)tr.t[p[0]] * V * V * V);
         sg.b_sn = 1;
@@ -619,7 +634,7 @@ struct Ctx {
         P.c_sm = Op; P.c_sn = 1; P.alpha = 1.0;
         const int q = gb.problem((long long)tr.t[p[0]] * 4096 + tr.t[p[1]], P);
         GemmSeg sg{};
-        sg.A = slot((int)s, r.slot_G);
+        sg.A = gslot((int)s);
         sg.a_sm = str(p[0]);
         sg.b_sn = V * V;
         if (p[2] == 2) {   // k = (s_p1, s_p2), B from t2H[t_p2][m][s_p1][s_p2]
@@ -653,7 +668,7 @@ struct Ctx {
         GemmSeg sg{};
         sg.A = el(r.ovooT, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
         sg.a_sb = 0; sg.a_sm = 1; sg.a_sk2 = Op;
-        sg.B = slot((int)s, r.slot_G);
+        sg.B = gslot((int)s);
         sg.b_sk2 = str(p[0]);
         if (t2layout) { sg.b_sb = str(p[2]); sg.b_sn = str(p[1]); }
         else { sg.b_sb = str(p[1]); sg.b_sn = str(p[2]); }
@@ -707,18 +722,27 @@ struct Ctx {
     const long long e = (t_end < 0 || t_end > n) ? n : t_end;
     const long long b = std::min(std::max(0LL, t_begin), e);
     all = std::vector<Triple>(all.begin() + b, all.begin() + e);
-    for (size_t b0 = 0; b0 < all.size(); b0 += B) {
-      std::vector<Triple> bt(all.begin() + b0, all.begin() + std::min(all.size(), b0 + B));
-      emit_wbuild(bt);
-      emit_epilogue(bt, backward);
-      if (backward) {
-        emit_b1(bt);
-        emit_b2(bt);
-        emit_b3(bt);
-        emit_b4(bt);
-        emit_vpart(bt);
+    const size_t group = (size_t)B * (backward ? r.S : 1);
+    for (size_t g0 = 0; g0 < all.size(); g0 += group) {
+      const size_t g1 = std::min(all.size(), g0 + group);
+      for (size_t b0 = g0; b0 < g1; b0 += B) {
+        std::vector<Triple> bt(all.begin() + b0, all.begin() + std::min(g1, b0 + B));
+        gbase = (int)(b0 - g0);
+        emit_wbuild(bt);
+        emit_epilogue(bt, backward);
+        if (backward) {
+          emit_b2(bt);
+          emit_b3(bt);
+          emit_b4(bt);
+          emit_vpart(bt);
+        }
+      }
+      if (backward) {   // the group's G cubes are gslot 0 .. g1-g0-1
+        gbase = 0;
+        emit_b1(std::vector<Triple>(all.begin() + g0, all.begin() + g1));
       }
     }
+    gbase = 0;
   }
 
   // kernels enqueued by one Launch (the V-term launch runs four)
