@@ -3,8 +3,8 @@
 //   PREC_F32    : v_mfma_f32_16x16x4f32, FP32 operands and accumulation
 //   PREC_BF16X3 : FP32 operands split on the way into LDS as x = hi + lo with
 //                 hi, lo in bf16; three v_mfma_f32_16x16x32_bf16 per K-step
-//                 (hi*hi, then lo*hi + hi*lo), FP32 accumulation (Henry, Tang
-//                 and Heinecke, ARITH 2019, "bf16x3").
+//                 (hi*hi, lo*hi, hi*lo), FP32 accumulation (Henry, Tang and
+//                 Heinecke, ARITH 2019, "bf16x3").
 //
 // Every contraction of the (T) energy and its VJP is expressed as a list of
 // problems C[b][m][n] (op)= alpha * sum_seg scale_seg * sum_k A_seg[b][m][k] B_seg[b][k][n]
@@ -214,10 +214,12 @@ __device__ __forceinline__ void write_out(const GemmProb& P, const TileCoord& tc
   }
 }
 
-// BM x BN tile, 4 waves arranged WGM x (4/WGM); register budget for two
-// waves per SIMD (two blocks per CU).
+// BM x BN tile, 4 waves arranged WGM x (4/WGM).  Register budget: four waves
+// per SIMD for tiles up to 64x64 (four blocks per CU fit in LDS), two above.
+template <int BM, int BN> struct MinBlocks { static constexpr int value = (BM * BN <= 64 * 64) ? 4 : 2; };
+
 template <int PREC, int BM, int BN, int WGM>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, (MinBlocks<BM, BN>::value))
 gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __restrict__ segs) {
   using TR = PrecTraits<PREC>;
   using T = typename TR::T;
@@ -393,30 +395,44 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
       load_next(sa[p], sb[p]);
-      // hi parts first: the hi*hi products wait only for them
+      // Tiles up to 64x64: hi parts first, so the hi*hi products wait only
+      // for them.  Larger tiles keep the compiler's interleaving (the pinned
+      // order would keep all 4 fragment sets live and spill).
+      constexpr bool PHASED = BM * BN <= 64 * 64;
       bf8v ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) ah[i] = *reinterpret_cast<const bf8v*>(&lah[p][(wm0 + i * 16 + l16) * LDK + ko]);
 #pragma unroll
       for (int j = 0; j < FN; j++) bh[j] = *reinterpret_cast<const bf8v*>(&lbh[p][(wn0 + j * 16 + l16) * LDK + ko]);
-      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (PHASED) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = 0; i < FM; i++) al[i] = *reinterpret_cast<const bf8v*>(&lal[p][(wm0 + i * 16 + l16) * LDK + ko]);
 #pragma unroll
       for (int j = 0; j < FN; j++) bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][(wn0 + j * 16 + l16) * LDK + ko]);
-      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (PHASED) {
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int i = 0; i < FM; i++)
+        for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
-      __builtin_amdgcn_sched_barrier(0);
+          for (int j = 0; j < FN; j++) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int i = 0; i < FM; i++)
+        for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) {
-          acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
-          acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
-        }
+          for (int j = 0; j < FN; j++) {
+            acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+          for (int j = 0; j < FN; j++) {
+            acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+          }
+      }
       if (it + 1 < nch) {
         sa[p ^ 1].template store_split<LDK>(lah[p ^ 1], lal[p ^ 1]);
         sb[p ^ 1].template store_split<LDK>(lbh[p ^ 1], lbl[p ^ 1]);
