@@ -30,7 +30,7 @@ struct GemmSeg {
   long long a_sb, a_sm, a_sk1, a_sk2;
   long long b_sb, b_sn, b_sk1, b_sk2;
   int K1, K2;
-  float scale;
+  float scale;        // +1 or -1 (checked when the plan is built)
   int chunk0;   // first K-chunk of this segment in the problem's concatenated K
   int a_mode, b_mode;
 };

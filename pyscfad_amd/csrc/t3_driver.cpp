@@ -385,6 +385,7 @@ struct Ctx {
                         ((BM - 1) * s.a_sm + (gb.BK - 1) * s.a_sk2 + vec) * (long long)d.elt < (1LL << 32) &&
                         ((BN - 1) * s.b_sn + (gb.BK - 1) * s.b_sk2 + vec) * (long long)d.elt < (1LL << 32);
         if (!ok && plan_error.empty()) plan_error = "GEMM operand is neither row- nor k-contiguous, or misaligned";
+        if (s.scale != 1.0f && s.scale != -1.0f && plan_error.empty()) plan_error = "GEMM segment scale must be +1 or -1";
       }
     }
     std::vector<GemmSeg> flat;

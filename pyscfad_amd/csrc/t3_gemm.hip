@@ -16,7 +16,8 @@
 // Staging: a K-chunk of each operand tile is read from HBM with 16-byte
 // global loads into registers (rows past the edge are clamped to the last
 // valid row, so the loads carry no branches), and written to LDS after the
-// MFMAs of the previous chunk, scaled and with the clamped rows zeroed.  The
+// MFMAs of the previous chunk, negated for segments of scale -1 (the clamped
+// rows are left as loaded: they reach only outputs that are not written).  The
 // LDS layout is always [row][k] with a 16-byte row stagger; row-contiguous
 // tiles are transposed in registers (each thread holds a VEC-row block).  Two
 // LDS buffers and two register stages, one barrier per chunk: the global
@@ -34,7 +35,9 @@ template <> struct PrecTraits<PREC_F64> { using T = double; using Acc = double; 
 template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 template <> struct PrecTraits<PREC_BF16X3> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 
-typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
 
 // Register stage of one operand tile (ROWS x BK elements, NE per thread).
 //   KCONT  : thread holds NV vectors, each VEC consecutive k of one row
@@ -50,12 +53,14 @@ struct Stage {
   static constexpr int RB = ROWS / VEC;       // row blocks (ROWCONT)
   static_assert(NV >= 1 && NE % VEC == 0 && RB * (BK / NV) == 256, "tile must give each thread whole vectors");
   V r[NV];
-  int mode, rows_valid;
-  float scale;
+  int mode;
+  bool neg;   // segment scale -1 (the only scales are +1 and -1)
 
-  // Byte offsets of this thread's NV vectors from the tile origin (rows past
-  // the edge clamped to the last valid row).  They depend only on the
-  // segment's strides, so they are computed once per segment.
+  // Byte offsets of this thread's NV vectors from the tile origin.  Rows past
+  // the edge are clamped to the last valid row, so every load reads real
+  // data; those rows feed only output rows or columns past M or N, which are
+  // never written, so they are not zeroed.  The offsets depend only on the
+  // segment's strides and are computed once per segment.
   static __device__ __forceinline__ void offsets(int mode, long long s_row, long long s_k, int rows_valid,
                                                  uint32_t* off) {
     const int t = threadIdx.x;
@@ -74,11 +79,9 @@ struct Stage {
     }
   }
 
-  __device__ __forceinline__ void load(int mode_, const char* base, const uint32_t* off, int rows_valid_,
-                                       float scale_) {
+  __device__ __forceinline__ void load(int mode_, const char* base, const uint32_t* off, bool neg_) {
     mode = mode_;
-    rows_valid = rows_valid_;
-    scale = scale_;
+    neg = neg_;
     const T3_GLOBAL char* g = (const T3_GLOBAL char*)base;
 #pragma unroll
     for (int q = 0; q < NV; q++) r[q] = *(const T3_GLOBAL V*)(g + off[q]);
@@ -88,10 +91,9 @@ struct Stage {
   template <int LDK>
   __device__ __forceinline__ void store(T* __restrict__ lds) const {
     const int t = threadIdx.x;
-    const T s = (T)scale;
+    const T s = neg ? (T)-1 : (T)1;
     if (mode == LOAD_ROWCONT) {
       const int rb = t % RB, kb = t / RB;
-      const bool ok = rb * VEC < rows_valid;
 #pragma unroll
       for (int e = 0; e < VEC; e++) {
         T* dst = lds + (rb * VEC + e) * LDK + kb * NV;
@@ -100,10 +102,10 @@ struct Stage {
           tnv x;
 #pragma unroll
           for (int j = 0; j < NV; j++) x[j] = r[j][e];
-          *reinterpret_cast<tnv*>(dst) = ok ? x * s : tnv{};
+          *reinterpret_cast<tnv*>(dst) = x * s;
         } else {
 #pragma unroll
-          for (int j = 0; j < NV; j++) dst[j] = ok ? r[j][e] * s : (T)0;
+          for (int j = 0; j < NV; j++) dst[j] = r[j][e] * s;
         }
       }
     } else {
@@ -111,41 +113,60 @@ struct Stage {
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         const int row = row0 + q * RSTEP;
-        *reinterpret_cast<V*>(lds + row * LDK + kv * VEC) = (row < rows_valid) ? r[q] * s : V{};
+        *reinterpret_cast<V*>(lds + row * LDK + kv * VEC) = r[q] * s;
       }
     }
   }
 
-  // LDS [row][k] as split bf16 pairs x = hi + lo (FP32 operands only)
+  // LDS [row][k] as split bf16 pairs x = hi + lo (FP32 operands only).  The
+  // sign is a uniform branch (scale +1 segments spend nothing on it; for -1
+  // the negation folds into source modifiers).  Per pair of values: one
+  // packed RNE conversion gives hi, a shift and a mask give hi back as FP32,
+  // two subtractions the remainders, one packed conversion lo.
   template <int LDK>
   __device__ __forceinline__ void store_split(__bf16* __restrict__ hi, __bf16* __restrict__ lo) const {
+    if (neg) store_split_as<LDK, true>(hi, lo);
+    else store_split_as<LDK, false>(hi, lo);
+  }
+  template <int LDK, bool NEG>
+  __device__ __forceinline__ void store_split_as(__bf16* __restrict__ hi, __bf16* __restrict__ lo) const {
     const int t = threadIdx.x;
-    const float s = scale;
+    auto split2 = [](float x0, float x1, uint32_t& h, uint32_t& l) {
+      if constexpr (NEG) { x0 = -x0; x1 = -x1; }
+      const bf2v hb = __builtin_convertvector(f2v{x0, x1}, bf2v);
+      h = __builtin_bit_cast(uint32_t, hb);
+      const float r0 = x0 - __uint_as_float(h << 16);
+      const float r1 = x1 - __uint_as_float(h & 0xffff0000u);
+      l = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{r0, r1}, bf2v));
+    };
+    auto split4 = [&](f4v x, u2v& h, u2v& l) {
+      uint32_t h0, l0, h1, l1;
+      split2(x[0], x[1], h0, l0);
+      split2(x[2], x[3], h1, l1);
+      h = u2v{h0, h1};
+      l = u2v{l0, l1};
+    };
     if (mode == LOAD_ROWCONT) {
       const int rb = t % RB, kb = t / RB;
-      const bool ok = rb * VEC < rows_valid;
 #pragma unroll
       for (int e = 0; e < VEC; e++) {
         const int off = (rb * VEC + e) * LDK + kb * NV;
         if constexpr (NV == 4) {
-          f4v x = {r[0][e], r[1][e], r[2][e], r[3][e]};
-          x = ok ? x * s : f4v{0, 0, 0, 0};
-          bf4v h, l;
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            h[j] = (__bf16)x[j];
-            l[j] = (__bf16)(x[j] - (float)h[j]);
-          }
-          *reinterpret_cast<bf4v*>(hi + off) = h;
-          *reinterpret_cast<bf4v*>(lo + off) = l;
+          u2v h, l;
+          split4(f4v{r[0][e], r[1][e], r[2][e], r[3][e]}, h, l);
+          *reinterpret_cast<u2v*>(hi + off) = h;
+          *reinterpret_cast<u2v*>(lo + off) = l;
+        } else if constexpr (NV == 2) {
+          uint32_t h, l;
+          split2(r[0][e], r[1][e], h, l);
+          *reinterpret_cast<uint32_t*>(hi + off) = h;
+          *reinterpret_cast<uint32_t*>(lo + off) = l;
         } else {
-#pragma unroll
-          for (int j = 0; j < NV; j++) {
-            const float x = ok ? (float)r[j][e] * s : 0.f;
-            const __bf16 h = (__bf16)x;
-            hi[off + j] = h;
-            lo[off + j] = (__bf16)(x - (float)h);
-          }
+          static_assert(NV == 1, "split store handles 1, 2 or 4 k values per thread");
+          uint32_t h, l;
+          split2(r[0][e], 0.0f, h, l);
+          hi[off] = __builtin_bit_cast(__bf16, (uint16_t)h);
+          lo[off] = __builtin_bit_cast(__bf16, (uint16_t)l);
         }
       }
     } else {
@@ -153,15 +174,10 @@ struct Stage {
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         const int row = row0 + q * RSTEP;
-        const f4v x = (row < rows_valid) ? r[q] * s : f4v{0, 0, 0, 0};
-        bf4v h, l;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          h[j] = (__bf16)x[j];
-          l[j] = (__bf16)(x[j] - (float)h[j]);
-        }
-        *reinterpret_cast<bf4v*>(hi + row * LDK + kv * VEC) = h;
-        *reinterpret_cast<bf4v*>(lo + row * LDK + kv * VEC) = l;
+        u2v h, l;
+        split4(r[q], h, l);
+        *reinterpret_cast<u2v*>(hi + row * LDK + kv * VEC) = h;
+        *reinterpret_cast<u2v*>(lo + row * LDK + kv * VEC) = l;
       }
     }
   }
@@ -273,7 +289,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   const char *pa = nullptr, *pb = nullptr;
   long long da2 = 0, db2 = 0, da1 = 0, db1 = 0;
   int k2 = 0, K2 = 0, seg_end = 0, amode = 0, bmode = 0;
-  float scale = 1.0f;
+  bool neg = false;
   uint32_t oa[SA::NV], ob[SB::NV];
   auto seg_init = [&](int c) {
     const GemmSeg& S = segs[si];
@@ -290,7 +306,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     db1 = (S.b_sk1 - (long long)S.K2 * S.b_sk2) * E;
     amode = S.a_mode;
     bmode = S.b_mode;
-    scale = S.scale;
+    neg = S.scale < 0.0f;
     seg_end = (si + 1 < P.seg0 + P.nseg) ? segs[si + 1].chunk0 : P.nchunks;
     SA::offsets(amode, S.a_sm, S.a_sk2, rows_a, oa);
     SB::offsets(bmode, S.b_sn, S.b_sk2, rows_b, ob);
@@ -300,8 +316,8 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   // load the walker's chunk, then step to the next one (the last chunk is
   // loaded again at the tail, which keeps the loop free of branches)
   auto load_next = [&](SA& xa, SB& xb) {
-    xa.load(amode, pa, oa, rows_a, scale);
-    xb.load(bmode, pb, ob, rows_b, 1.0f);
+    xa.load(amode, pa, oa, neg);
+    xb.load(bmode, pb, ob, false);
     if (lc + 1 < c_end) {
       lc++;
       if (lc == seg_end) {
