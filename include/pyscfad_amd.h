@@ -148,6 +148,41 @@ int pyscfad_amd_ccsd_t_energy_vjp_slice(int nocc, int nvir, const double *mo_ene
                                         double *ovov_bar, double *ovvv_bar, double *e_out,
                                         void *workspace, size_t workspace_bytes, void *stream);
 
+/*
+ * Density-fitted variant: ovvv is not an input.  Its slabs are built on the
+ * device, one occupied index at a time, from the DF factors of the reference's
+ * DF-CCSD (pyscfad/cc/dfccsd.py:81-91):
+ *     Lov (naux, o, v)            = Lpq[:, :o, o:]
+ *     Lvv (naux, v(v+1)/2)        = pack_tril(Lpq[:, o:, o:])  (row-major lower triangle)
+ *     ovvv[i, a, :] = sum_P Lov[P, i, a] Lvv[P, :]               (dfccsd.py:91)
+ * so that the o*v^3 integrals and their FP64 cotangent are never stored: the
+ * unique triples are processed in blocks of occupied indices and a block's
+ * slabs live in a cache while its triples run.  The energy is that of
+ * pyscfad_amd_ccsd_t_energy on ovvv = Lov^T Lvv (pyscfad/cc/ccsd_t.py:41 with
+ * the DF eris).  The VJP returns, in place of ovvv_bar, the cotangents of Lov
+ * and Lvv that the reference obtains by differentiating dfccsd.py:91 after
+ * _ccsd_t_energy_vjp (ccsd_t.py:82-192); all other outputs are those of
+ * pyscfad_amd_ccsd_t_energy_vjp.  Lov_bar (naux, o, v) and Lvv_bar (naux,
+ * v(v+1)/2) are overwritten.  [triple_begin, triple_end) selects a slice of
+ * the unique triples in this variant's blocked order (triple_end < 0: to the
+ * end); the shares of a partition sum to the full result.  The environment
+ * variable PYSCFAD_AMD_DF_BLOCK sets the block size (default: a slab cache of
+ * about 40 GB).
+ */
+size_t pyscfad_amd_ccsd_t_df_workspace_bytes(int nocc, int nvir, int naux, int precision, int with_vjp);
+int pyscfad_amd_ccsd_t_df_energy(int nocc, int nvir, int naux, const double *mo_energy, const double *t1,
+                                 const double *t2, const double *fock, const double *ovoo, const double *ovov,
+                                 const double *Lov, const double *Lvv, int precision, long long triple_begin,
+                                 long long triple_end, double *e_out, void *workspace, size_t workspace_bytes,
+                                 void *stream);
+int pyscfad_amd_ccsd_t_df_energy_vjp(int nocc, int nvir, int naux, const double *mo_energy, const double *t1,
+                                     const double *t2, const double *fock, const double *ovoo, const double *ovov,
+                                     const double *Lov, const double *Lvv, double et_bar, int precision,
+                                     long long triple_begin, long long triple_end, double *t1_bar, double *t2_bar,
+                                     double *fock_bar, double *mo_energy_bar, double *ovoo_bar, double *ovov_bar,
+                                     double *Lov_bar, double *Lvv_bar, double *e_out, void *workspace,
+                                     size_t workspace_bytes, void *stream);
+
 /* Text of the last error of the calling thread ("" if none). */
 const char *pyscfad_amd_last_error(void);
 
@@ -160,7 +195,9 @@ const char *pyscfad_amd_last_error(void);
  * 2 V-term adjoint kernels, 3 split-K reductions, 4 per-call setup (accumulator
  * clears, input re-layout, cotangent write-back), and family 0
  * split by contraction: 5 W build, 6 ovvv adjoint, 7 t2 adjoint (particle
- * term), 8 ovoo adjoint, 9 t2 adjoint (hole term); returns the count filled. */
+ * term), 8 ovoo adjoint, 9 t2 adjoint (hole term), 10 density-fitted slab
+ * builds, cotangent folds and slab clears (also counted in 0 and 3); returns
+ * the count filled. */
 void pyscfad_amd_set_timing(int enable);
 int pyscfad_amd_last_timing(double *ms, long long *launches, int n);
 

@@ -24,6 +24,9 @@ EXPORTS = (
     "pyscfad_amd_ccsd_t_batched_workspace_bytes",
     "pyscfad_amd_ccsd_t_batched",
     "pyscfad_amd_ccsd_t_autotune",
+    "pyscfad_amd_ccsd_t_df_workspace_bytes",
+    "pyscfad_amd_ccsd_t_df_energy",
+    "pyscfad_amd_ccsd_t_df_energy_vjp",
     "pyscfad_amd_last_error",
     "pyscfad_amd_set_timing",
     "pyscfad_amd_last_timing",
@@ -77,6 +80,13 @@ def load():
     lib.pyscfad_amd_ccsd_t_batched.argtypes = [i, ip, i, i, vp, sz, vp]
     lib.pyscfad_amd_ccsd_t_autotune.restype = i
     lib.pyscfad_amd_ccsd_t_autotune.argtypes = [i, i, i, ctypes.POINTER(ctypes.c_int), vp]
+    lib.pyscfad_amd_ccsd_t_df_workspace_bytes.restype = sz
+    lib.pyscfad_amd_ccsd_t_df_workspace_bytes.argtypes = [i, i, i, i, i]
+    lib.pyscfad_amd_ccsd_t_df_energy.restype = i
+    lib.pyscfad_amd_ccsd_t_df_energy.argtypes = [i, i, i] + [dp] * 8 + [i, ll, ll, dp, vp, sz, vp]
+    lib.pyscfad_amd_ccsd_t_df_energy_vjp.restype = i
+    lib.pyscfad_amd_ccsd_t_df_energy_vjp.argtypes = ([i, i, i] + [dp] * 8 + [d, i, ll, ll] + [dp] * 9
+                                                     + [vp, sz, vp])
     lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
     lib.pyscfad_amd_last_error.argtypes = []
     lib.pyscfad_amd_set_timing.restype = None
@@ -100,7 +110,7 @@ def check(rc):
 
 
 FAMILIES = ("gemm", "epilogue", "vpart", "reduce", "prep",
-            "gemm_w", "gemm_b1", "gemm_b2", "gemm_b3", "gemm_b4")
+            "gemm_w", "gemm_b1", "gemm_b2", "gemm_b3", "gemm_b4", "df")
 
 
 def set_timing(enable):

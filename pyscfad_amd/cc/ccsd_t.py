@@ -13,6 +13,10 @@ reference:
   in the layouts of ccsd_t.py:179-192 (ovvv_bar lower-triangle packed).
 * ``CCSDTEnergy`` -- the same pair as a ``torch.autograd.Function`` over
   ``(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)``.
+* ``energy_df`` / ``energy_vjp_df`` / ``CCSDTEnergyDF`` -- the same with the
+  DF factors of pyscfad's DF-CCSD (pyscfad/cc/dfccsd.py:81-91) in place of
+  ovvv: the o*v^3 slabs are built on the device as they are needed, and the
+  cotangent of ovvv comes back as the cotangents of Lov and Lvv.
 
 Both passes run on the GPU through the C ABI of ``libpyscfad_amd.so``;
 arrays may be NumPy (copied to the device) or torch tensors (used in place
@@ -221,6 +225,93 @@ def autotune_problems(problems, precision=DEFAULT_PRECISION, stream=None):
 
 def workspace_bytes(nocc, nvir, precision=DEFAULT_PRECISION, with_vjp=True):
     return _lib.load().pyscfad_amd_ccsd_t_workspace_bytes(nocc, nvir, _prec(precision), int(with_vjp))
+
+
+# ---- density-fitted variant: ovvv built on the device from the DF factors ----
+
+def _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv):
+    """Device copies of the inputs of the density-fitted calls.  Lov is
+    (naux, nocc, nvir) or, as pyscfad/cc/dfccsd.py:82 holds it, (naux,
+    nocc*nvir); Lvv is (naux, nvir(nvir+1)/2) lower-triangle packed
+    (dfccsd.py:83)."""
+    t1d, t2d = _dev(t1), _dev(t2)
+    nocc, nvir = t1d.shape
+    me, fk, oo, ov, lo, lv = (_dev(x) for x in (mo_energy, fock, ovoo, ovov, Lov, Lvv))
+    naux = lo.shape[0]
+    if lo.dim() == 2 and lo.shape[1] == nocc * nvir:
+        lo = lo.reshape(naux, nocc, nvir)
+    n = nocc + nvir
+    expect = {
+        "t2": (t2d, (nocc, nocc, nvir, nvir)), "mo_energy": (me, (n,)), "fock": (fk, (n, n)),
+        "ovoo": (oo, (nocc, nvir, nocc, nocc)), "ovov": (ov, (nocc, nvir, nocc, nvir)),
+        "Lov": (lo, (naux, nocc, nvir)), "Lvv": (lv, (naux, nvir * (nvir + 1) // 2)),
+    }
+    for name, (arr, shape) in expect.items():
+        if tuple(arr.shape) != shape:
+            raise ValueError(f"{name} has shape {tuple(arr.shape)}, expected {shape}")
+    return nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv
+
+
+def df_workspace_bytes(nocc, nvir, naux, precision=DEFAULT_PRECISION, with_vjp=True):
+    return _lib.load().pyscfad_amd_ccsd_t_df_workspace_bytes(nocc, nvir, naux, _prec(precision), int(with_vjp))
+
+
+def energy_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=DEFAULT_PRECISION, stream=None,
+              triples=None, workspace=None):
+    """E(T) with ovvv = Lov^T Lvv (pyscfad/cc/dfccsd.py:91) built slab by slab
+    on the device, never stored whole.  ``triples`` selects a slice of the
+    unique triples in this variant's blocked order."""
+    nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv = _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)
+    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
+    _lib.check(_lib.load().pyscfad_amd_ccsd_t_df_energy(
+        nocc, nvir, naux, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(lo), _ptr(lv),
+        _prec(precision), int(t0), int(t1_), _ptr(out), *_ws(workspace), s.cuda_stream))
+    return out
+
+
+def energy_vjp_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, et_bar=1.0, precision=DEFAULT_PRECISION,
+                  stream=None, return_energy=False, triples=None, workspace=None):
+    """Cotangents (t1, t2, fock, mo_energy, ovoo, ovov, Lov, Lvv) of the
+    density-fitted E(T): those of energy_vjp for the first six, and for the DF
+    factors the cotangent of ovvv carried through ovvv = Lov^T Lvv
+    (Lov_bar in the shape Lov was given in)."""
+    lov_shape = tuple(Lov.shape)
+    nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv = _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)
+    z = lambda a: torch.empty_like(a)
+    bars = [z(t1d), z(t2d), z(fk), z(me), z(oo), z(ov), z(lo), z(lv)]
+    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    if hasattr(et_bar, "item"):
+        et_bar = et_bar.item()
+    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
+    _lib.check(_lib.load().pyscfad_amd_ccsd_t_df_energy_vjp(
+        nocc, nvir, naux, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(lo), _ptr(lv),
+        float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars], _ptr(out),
+        *_ws(workspace), s.cuda_stream))
+    bars[6] = bars[6].reshape(lov_shape)
+    if return_energy:
+        return tuple(bars), out
+    return tuple(bars)
+
+
+class CCSDTEnergyDF(torch.autograd.Function):
+    """The density-fitted E(T) as a torch autograd function of
+    (t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)."""
+
+    @staticmethod
+    def forward(ctx, t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=DEFAULT_PRECISION):
+        ctx.save_for_backward(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)
+        ctx.precision = precision
+        return energy_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=precision)
+
+    @staticmethod
+    def backward(ctx, gbar):
+        t1, t2, me, fk, oo, ov, lo, lv = ctx.saved_tensors
+        t1b, t2b, fkb, meb, oob, ovb, lob, lvb = energy_vjp_df(t1, t2, me, fk, oo, ov, lo, lv, et_bar=float(gbar),
+                                                              precision=ctx.precision)
+        return t1b, t2b, meb, fkb, oob, ovb, lob, lvb, None
 
 
 def _ccsd_t_energy_vjp(eris, t1, t2, et_bar, max_memory=None, precision=DEFAULT_PRECISION):

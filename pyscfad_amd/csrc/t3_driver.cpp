@@ -55,6 +55,10 @@ hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, cons
 hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
                          const double* t2, hipStream_t s);
 hipError_t t3_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t t3_prep_df(double* LovP, double* LvvF, const double* Lov, const double* Lvv, int naux, int o, int v,
+                      int V, int NP, hipStream_t s);
+hipError_t t3_finalize_df(double* Lov_bar, double* Lvv_bar, const double* LovBar, const double* LvvBar, int naux,
+                          int o, int v, int V, int NP, hipStream_t s);
 hipError_t t3_copy_f64(double* dst, const double* src, int n, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
@@ -68,6 +72,7 @@ namespace {
 constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 constexpr int GEMM_TARGET_BLOCKS = 2048;
 constexpr int MAX_KSPLIT = 64;
+constexpr int DF_KSPLIT = 8;   // split-K of the Lov cotangent folds (K = V^2)
 
 thread_local std::string g_last_error;
 // per-family device time (ms) of the last call, recorded with hipEvents
@@ -75,7 +80,9 @@ thread_local std::string g_last_error;
 // (forward and backward recompute), then the four adjoint contractions
 enum Family {
   FAM_GEMM = 0, FAM_EPILOGUE = 1, FAM_VPART = 2, FAM_REDUCE = 3, FAM_PREP = 4,
-  FAM_GEMM_W = 5, FAM_GEMM_B1 = 6, FAM_GEMM_B2 = 7, FAM_GEMM_B3 = 8, FAM_GEMM_B4 = 9, FAM_COUNT = 10
+  FAM_GEMM_W = 5, FAM_GEMM_B1 = 6, FAM_GEMM_B2 = 7, FAM_GEMM_B3 = 8, FAM_GEMM_B4 = 9,
+  FAM_GEMM_DF = 10,   // density-fitted mode: ovvv slab builds and cotangent folds
+  FAM_COUNT = 11
 };
 thread_local int g_timing = 0;
 thread_local double g_family_ms[FAM_COUNT];
@@ -112,6 +119,9 @@ struct Dims {
   size_t elt;      // bytes of a contraction operand (and GEMM accumulator) element
   int tp_is_f64;
   int ntile, ncube;
+  // density-fitted mode (naux > 0): ovvv slabs are built on the device from
+  // the DF factors into a cache of `dfslots` block slots of `dfblk` slabs
+  int naux = 0, NP = 0, dfblk = 0, dfslots = 0;
 };
 
 Dims make_dims(int o, int v, int prec) {
@@ -130,6 +140,33 @@ Dims make_dims(int o, int v, int prec) {
   return d;
 }
 
+// Density-fitted mode: the occupied indices are cut into blocks of dfblk;
+// the unique triples are enumerated block triple by block triple
+// (I >= J >= K), so at most three blocks of ovvv slabs are needed at a time.
+// A block of slabs is built when first needed and (backward) its FP64 slab
+// cotangents are folded into the DF factors' cotangents when it is evicted.
+// The slab cache (and, backward, its FP64 accumulators) is held to about
+// 40 GB; PYSCFAD_AMD_DF_BLOCK (1..nocc) overrides the block size.
+void set_df(Dims& d, int naux, bool backward) {
+  d.naux = naux;
+  d.NP = (int)round_up(naux, 32);
+  const double slab = (double)d.V * d.V * d.V * (double)(d.elt + (backward ? 8 : 0));
+  long long blk = (long long)(40.0e9 / (3.0 * slab));
+  if (const char* env = std::getenv("PYSCFAD_AMD_DF_BLOCK")) {
+    const long long x = std::atoll(env);
+    if (x >= 1) blk = x;
+  }
+  d.dfblk = (int)std::max<long long>(1, std::min<long long>(blk, d.o));
+  const int nblk = (d.o + d.dfblk - 1) / d.dfblk;
+  d.dfslots = std::min(3, nblk);
+}
+
+Dims dims_for(int o, int v, int prec, int naux, bool backward) {
+  Dims d = make_dims(o, v, prec);
+  if (naux > 0) set_df(d, naux, backward);
+  return d;
+}
+
 // ----------------------------------------------------------------------------
 // Workspace: one device allocation carved into named regions.
 struct Layout {
@@ -144,6 +181,9 @@ struct Layout {
 struct Regions {
   size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, fvoP, energy;
   size_t ovvv_acc, t2acc, t2hacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
+  size_t LovP, LvvF, LovBar, LvvBar;   // density-fitted mode only
+  size_t acc_end;                      // end of the FP64 accumulators cleared at setup
+  size_t dfsplit, dfsplit_bytes;       // split-K scratch of the Lov cotangent folds
   size_t slots, slot_bytes;
   size_t slot_w0, slot_H, slot_part, slot_vp;
   size_t gslots, gslot_bytes;   // G cubes: B x S of them (S batches per ovvv-adjoint group)
@@ -193,7 +233,9 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   Regions r{};
   r.S = backward ? b1_group(d, B) : 1;
   const size_t V = d.V, o = d.o, Op = d.Op;
-  r.ovvvT = L.take(o * V * V * V * d.elt);
+  // ovvv slabs: all o of them, or the density-fitted block cache
+  const size_t nslab = d.naux > 0 ? (size_t)d.dfslots * d.dfblk : o;
+  r.ovvvT = L.take(nslab * V * V * V * d.elt);
   r.t2T = L.take(Op * Op * V * V * d.elt);
   r.t2H = L.take(Op * Op * V * V * d.elt);
   r.ovooT = L.take(o * o * V * Op * d.elt);
@@ -201,8 +243,12 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   r.eps_vir = L.take(V * 8);
   r.fvoP = L.take(o * V * 8);
   r.energy = L.take(8);
+  if (d.naux > 0) {
+    r.LovP = L.take(o * V * d.NP * 8);
+    r.LvvF = L.take(V * V * d.NP * 8);
+  }
   if (backward) {
-    r.ovvv_acc = L.take(o * V * V * V * 8);
+    r.ovvv_acc = L.take(nslab * V * V * V * 8);
     r.t2acc = L.take(Op * Op * V * V * 8);
     r.t2hacc = L.take(Op * Op * V * V * 8);
     r.ovov_acc = L.take(o * o * V * V * 8);
@@ -210,6 +256,11 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
     r.t1_acc = L.take(o * V * 8);
     r.fvo_acc = L.take(o * V * 8);
     r.eps_bar = L.take((o + V) * 8);
+    if (d.naux > 0) {
+      r.LovBar = L.take(o * V * d.NP * 8);
+      r.LvvBar = L.take(V * V * d.NP * 8);
+    }
+    r.acc_end = L.off;
   }
   Layout S;
   r.slot_w0 = S.take(V * V * V * d.elt);
@@ -223,13 +274,15 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6 per triple
   r.split_bytes = backward ? (size_t)6 * B * MAX_KSPLIT * V * std::max(V, Op) * d.elt + (size_t)6 * B * 256 : 0;
   r.split = L.take(r.split_bytes);
+  r.dfsplit_bytes = (backward && d.naux > 0) ? (size_t)d.dfblk * DF_KSPLIT * V * d.NP * 8 + (size_t)d.dfblk * 256 : 0;
+  r.dfsplit = L.take(r.dfsplit_bytes);
   r.desc = L.off;
   return r;
 }
 
 // ----------------------------------------------------------------------------
 // Launch plan
-enum LaunchKind { L_GEMM, L_REDUCE, L_EPI, L_EPIRED, L_VPART, L_VMN };
+enum LaunchKind { L_GEMM, L_REDUCE, L_EPI, L_EPIRED, L_VPART, L_VMN, L_ZERO };
 
 struct Launch {
   LaunchKind kind;
@@ -239,6 +292,8 @@ struct Launch {
   long long work = 0;                        // GEMM blocks / reduce elements
   int aux = -1;                              // index of host-side argument struct
   int sub = -1;                              // GEMM contraction family (FAM_GEMM_*)
+  int prec = -1;                             // GEMM / reduce arithmetic (-1: the plan's)
+  // L_ZERO: clears `work` bytes at workspace offset off_a
 };
 
 struct Blob {
@@ -270,14 +325,32 @@ std::vector<Triple> enumerate_triples(int o) {
   return out;
 }
 
+// The unique triples i >= j >= k with i in block I, j in block J, k in block
+// K of `blk` occupied indices (I >= J >= K), in the order of
+// enumerate_triples.  Concatenated over I, J, K in nested ascending order they
+// list every unique triple once.
+std::vector<Triple> block_triples(int o, int blk, int I, int J, int K) {
+  std::vector<Triple> out;
+  const int i1 = std::min(o, (I + 1) * blk), j1 = std::min(o, (J + 1) * blk), k1 = std::min(o, (K + 1) * blk);
+  for (int i = I * blk; i < i1; i++)
+    for (int j = J * blk; j < j1 && j <= i; j++)
+      for (int k = K * blk; k < k1 && k <= j; k++) {
+        Triple tr;
+        tr.t[0] = i; tr.t[1] = j; tr.t[2] = k;
+        tr.fo = (i == k) ? 1.0 / 6.0 : ((i == j || j == k) ? 0.5 : 1.0);
+        out.push_back(tr);
+      }
+  return out;
+}
+
 // Grouped GEMM under construction: problems keyed by their output slab.
 struct GemmBuild {
-  int BK;
+  int prec, BK;
   std::map<long long, int> index;
   std::vector<GemmProb> probs;
   std::vector<std::vector<GemmSeg>> segs;
 
-  explicit GemmBuild(int bk) : BK(bk) {}
+  explicit GemmBuild(int prec_) : prec(prec_), BK(t3_gemm_chunk(prec_)) {}
   int problem(long long key, const GemmProb& tmpl) {
     auto it = index.find(key);
     if (it != index.end()) return it->second;
@@ -308,7 +381,7 @@ struct Ctx {
   std::vector<EpiArgs> epi_args;
   std::vector<EpiReduceArgs> epired_args;
   std::string plan_error;   // set when a contraction does not fit the GEMM's operand modes
-  int force_cfg[FAM_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};   // autotuner trials
+  int force_cfg[FAM_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};   // autotuner trials
 
   // tile configuration of a contraction: a trial's, else the tuned one, else the default
   int cfg_for(int fam, int dflt) const {
@@ -333,7 +406,7 @@ struct Ctx {
   // Defaults: 128x128 tiles for large FP32 / bf16x3 problems; FP64 stays on
   // 64x64 (its 128x128 variant would spill).
   int pick_cfg(const GemmBuild& gb) const {
-    if (d.prec == PREC_F64) return 0;
+    if (gb.prec == PREC_F64) return 0;
     for (const auto& P : gb.probs)
       if (P.M < 128 || P.N < 128) return 0;
     return 1;
@@ -341,7 +414,9 @@ struct Ctx {
 
   // Flatten a GemmBuild into a launch.
   void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub,
-                 int max_ksplit = MAX_KSPLIT) {
+                 int max_ksplit = MAX_KSPLIT, size_t scratch_region = (size_t)-1) {
+    if (scratch_region == (size_t)-1) scratch_region = r.split;
+    const size_t elt = gb.prec == PREC_F64 ? 8 : 4;   // operand and accumulator bytes
     if (gb.empty()) return;
     if (cfg < 0) cfg = pick_cfg(gb);
     const int BM = t3_gemm_tile_m(cfg), BN = t3_gemm_tile_n(cfg);
@@ -366,8 +441,8 @@ struct Ctx {
       ksplit = std::min(ksplit, std::min(max_ksplit, std::max(1, min_chunks)));
     }
     // operand load modes; every base and stride must keep VEC-element vectors aligned
-    const long long vec = t3_gemm_vec(d.prec);
-    const long long vbytes = vec * (long long)d.elt;
+    const long long vec = t3_gemm_vec(gb.prec);
+    const long long vbytes = vec * (long long)elt;
     auto aligned = [&](const void* p) { return ((uintptr_t)p % (uintptr_t)vbytes) == 0; };
     for (size_t p = 0; p < gb.probs.size(); p++) {
       const GemmProb& P = gb.probs[p];
@@ -382,8 +457,8 @@ struct Ctx {
                         s.a_sk1 % vec == 0 && s.b_sk1 % vec == 0 && s.K2 % gb.BK == 0 &&
                         aligned(s.A) && aligned(s.B) &&
                         // per-thread tile offsets are 32-bit byte offsets
-                        ((BM - 1) * s.a_sm + (gb.BK - 1) * s.a_sk2 + vec) * (long long)d.elt < (1LL << 32) &&
-                        ((BN - 1) * s.b_sn + (gb.BK - 1) * s.b_sk2 + vec) * (long long)d.elt < (1LL << 32);
+                        ((BM - 1) * s.a_sm + (gb.BK - 1) * s.a_sk2 + vec) * (long long)elt < (1LL << 32) &&
+                        ((BN - 1) * s.b_sn + (gb.BK - 1) * s.b_sk2 + vec) * (long long)elt < (1LL << 32);
         if (!ok && plan_error.empty()) plan_error = "GEMM operand is neither row- nor k-contiguous, or misaligned";
         if (s.scale != 1.0f && s.scale != -1.0f && plan_error.empty()) plan_error = "GEMM segment scale must be +1 or -1";
       }
@@ -402,9 +477,9 @@ struct Ctx {
       P.block0 = nblocks;
       nblocks += (long long)P.nb * P.tiles_m * P.tiles_n * P.ksplit;
       if (allow_split) {
-        const size_t bytes = (size_t)P.ksplit * P.nb * P.M * P.N * d.elt;
+        const size_t bytes = (size_t)P.ksplit * P.nb * P.M * P.N * elt;
         P.mode = GEMM_PARTIAL;
-        P.P = ws ? ws + r.split + scratch : nullptr;
+        P.P = ws ? ws + scratch_region + scratch : nullptr;
         ReduceProb R{};
         R.P = P.P;
         R.ksplit = P.ksplit;
@@ -436,10 +511,12 @@ struct Ctx {
     L.n_b = (int)flat.size();
     L.work = nblocks;
     L.sub = sub;
+    L.prec = gb.prec;
     launches.push_back(L);
     if (allow_split) {
       Launch R;
       R.kind = L_REDUCE;
+      R.prec = gb.prec;
       R.off_a = blob.put(red);
       R.n_a = (int)red.size();
       R.work = relem;
@@ -447,12 +524,130 @@ struct Ctx {
     }
   }
 
+  // ---- ovvv slabs ----
+  // Slab of occupied index x in r.ovvvT (and its FP64 cotangent accumulator in
+  // r.ovvv_acc): all o are resident in the dense mode; the density-fitted mode
+  // maps x through the block slots of the cache.
+  std::vector<int> df_pos;        // x -> slab position in the cache, -1 if not resident
+  int df_res[3] = {-1, -1, -1};   // block held by each cache slot
+  long long slab_pos(int x) const { return d.naux > 0 ? (long long)df_pos[x] : (long long)x; }
+
+  void emit_zero(size_t off, size_t bytes) {
+    Launch L;
+    L.kind = L_ZERO;
+    L.off_a = off;
+    L.work = (long long)bytes;
+    L.sub = FAM_GEMM_DF;
+    launches.push_back(L);
+  }
+
+  // Build the slabs of block blk into cache slot `slot`, in FP64 arithmetic
+  // (rounded to the contraction precision on store, as the dense mode rounds
+  // its ovvv input):  ovvvT[x][a][y][f] = sum_P LovP[x][a][P] LvvF[y][f][P],
+  // one problem per index (batch y, rows a, columns f, k = P).
+  void emit_df_build(int blk, int slot) {
+    const long long V = d.V, NP = d.NP;
+    const int x0 = blk * d.dfblk, x1 = std::min(d.o, x0 + d.dfblk);
+    GemmBuild gb(PREC_F64);
+    for (int x = x0; x < x1; x++) {
+      GemmProb P{};
+      P.nb = d.V; P.M = d.V; P.N = d.V;
+      P.C = ws + r.ovvvT + (size_t)((long long)slot * d.dfblk + (x - x0)) * V * V * V * d.elt;
+      P.c_sb = V; P.c_sm = V * V; P.c_sn = 1; P.alpha = 1.0;
+      const int q = gb.problem(x, P);
+      GemmSeg sg{};
+      sg.A = ws + r.LovP + (size_t)x * V * NP * 8;
+      sg.a_sb = 0; sg.a_sm = NP; sg.a_sk2 = 1;
+      sg.B = ws + r.LvvF;
+      sg.b_sb = V * NP; sg.b_sn = NP; sg.b_sk2 = 1;
+      sg.K1 = 1; sg.K2 = d.NP; sg.scale = 1.0f;
+      gb.add(q, sg);
+    }
+    emit_gemm(gb, 0, GEMM_STORE, d.tp_is_f64, false, FAM_GEMM_DF);
+  }
+
+  // Fold the FP64 slab cotangents Sbar of the block in cache slot `slot` into
+  // the DF factors' cotangents (the chain rule through the slab build):
+  //   LovBar[x][a][P] += sum_{y,f} Sbar_x[a][y][f] LvvF[y][f][P]   (per x, split-K over (y, f))
+  //   LvvBar[y][f][P] += sum_x sum_a Sbar_x[a][y][f] LovP[x][a][P] (one problem, one segment per x)
+  void emit_df_fold(int blk, int slot) {
+    const long long V = d.V, NP = d.NP;
+    const int x0 = blk * d.dfblk, x1 = std::min(d.o, x0 + d.dfblk);
+    auto sbar = [&](int x) {
+      return ws + r.ovvv_acc + (size_t)((long long)slot * d.dfblk + (x - x0)) * V * V * V * 8;
+    };
+    {
+      GemmBuild gb(PREC_F64);
+      for (int x = x0; x < x1; x++) {
+        GemmProb P{};
+        P.nb = 1; P.M = d.V; P.N = d.NP;
+        P.C = ws + r.LovBar + (size_t)x * V * NP * 8;
+        P.c_sm = NP; P.c_sn = 1; P.alpha = 1.0;
+        const int q = gb.problem(x, P);
+        GemmSeg sg{};
+        sg.A = sbar(x); sg.a_sm = V * V; sg.a_sk1 = V; sg.a_sk2 = 1;
+        sg.B = ws + r.LvvF; sg.b_sk1 = V * NP; sg.b_sk2 = NP; sg.b_sn = 1;
+        sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
+        gb.add(q, sg);
+      }
+      emit_gemm(gb, 0, GEMM_ACCUM, 1, true, FAM_GEMM_DF, DF_KSPLIT, r.dfsplit);
+    }
+    {
+      GemmBuild gb(PREC_F64);
+      GemmProb P{};
+      P.nb = d.V; P.M = d.V; P.N = d.NP;
+      P.C = ws + r.LvvBar;
+      P.c_sb = V * NP; P.c_sm = NP; P.c_sn = 1; P.alpha = 1.0;
+      const int q = gb.problem(0, P);
+      for (int x = x0; x < x1; x++) {
+        GemmSeg sg{};
+        sg.A = sbar(x); sg.a_sb = V; sg.a_sm = 1; sg.a_sk2 = V * V;
+        sg.B = ws + r.LovP + (size_t)x * V * NP * 8; sg.b_sb = 0; sg.b_sk2 = NP; sg.b_sn = 1;
+        sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
+        gb.add(q, sg);
+      }
+      emit_gemm(gb, 0, GEMM_ACCUM, 1, false, FAM_GEMM_DF);
+    }
+  }
+
+  // Make blocks need[0..2] resident: each missing block takes a slot holding
+  // no needed block (one always exists: at most three distinct blocks are
+  // needed and there are min(3, number of blocks) slots); the evicted block's
+  // cotangents are folded first, and the slot's accumulators cleared after.
+  void df_ensure(const int need[3]) {
+    const size_t acc_slot = (size_t)d.dfblk * d.V * d.V * d.V * 8;
+    for (int t = 0; t < 3; t++) {
+      bool have = false;
+      for (int u = 0; u < d.dfslots; u++) have = have || df_res[u] == need[t];
+      if (have) continue;
+      int slot = -1;
+      for (int u = 0; u < d.dfslots && slot < 0; u++)
+        if (df_res[u] != need[0] && df_res[u] != need[1] && df_res[u] != need[2]) slot = u;
+      if (slot < 0) {
+        if (plan_error.empty()) plan_error = "density-fitted slab cache has no free slot";
+        return;
+      }
+      if (df_res[slot] >= 0) {
+        if (backward) emit_df_fold(df_res[slot], slot);
+        set_block_pos(df_res[slot], -1);
+      }
+      emit_df_build(need[t], slot);
+      if (backward) emit_zero(r.ovvv_acc + (size_t)slot * acc_slot, acc_slot);
+      df_res[slot] = need[t];
+      set_block_pos(need[t], slot);
+    }
+  }
+  void set_block_pos(int blk, int slot) {
+    const int x0 = blk * d.dfblk, x1 = std::min(d.o, x0 + d.dfblk);
+    for (int x = x0; x < x1; x++) df_pos[x] = slot < 0 ? -1 : slot * d.dfblk + (x - x0);
+  }
+
   // ---- W-build segments (output layout: batch position ib, row position im, col position 2) ----
   GemmSeg particle_seg(const Triple& tr, int rho, int ib, int im) const {
     const int* p = PERM[rho];
     const long long V = d.V, Op = d.Op;
     GemmSeg s{};
-    const char* L = el(r.ovvvT, (long long)tr.t[p[0]] * V * V * V);
+    const char* L = el(r.ovvvT, slab_pos(tr.t[p[0]]) * V * V * V);
     const char* R = el(r.t2T, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
     if (p[2] == 2) {
       s.A = L; s.a_sb = (p[0] == ib) ? V * V : V; s.a_sm = (p[0] == im) ? V * V : V; s.a_sk2 = 1;
@@ -489,7 +684,7 @@ struct Ctx {
     static const int P1[4] = {0, 2, 3, 5}, H1[4] = {0, 1, 4, 5};   // batch b (pos 1), rows a
     static const int P2[2] = {1, 4}, H2[2] = {2, 3};               // batch a (pos 0), rows b
     for (int pass = 0; pass < 2; pass++) {
-      GemmBuild gb(d.BK);
+      GemmBuild gb(d.prec);
       for (size_t s = 0; s < bt.size(); s++) {
         GemmProb P{};
         P.nb = d.V; P.M = d.V; P.N = d.V;
@@ -566,7 +761,7 @@ struct Ctx {
   void emit_b1(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
     for (int orient = 0; orient < 2; orient++) {
-      GemmBuild gb(d.BK);
+      GemmBuild gb(d.prec);
       for (size_t s = 0; s < bt.size(); s++)
         for (int rho = 0; rho < 6; rho++) {
           const int* p = PERM[rho];
@@ -574,7 +769,7 @@ struct Ctx {
           const Triple& tr = bt[s];
           GemmProb P{};
           P.nb = d.V; P.M = d.V; P.N = d.V;
-          P.C = ws ? ws + r.ovvv_acc + (size_t)tr.t[p[0]] * V * V * V * 8 : nullptr;
+          P.C = ws ? ws + r.ovvv_acc + (size_t)slab_pos(tr.t[p[0]]) * V * V * V * 8 : nullptr;
           P.c_sn = 1; P.alpha = 1.0;
           if (orient == 0) { P.c_sb = V * V; P.c_sm = V; } else { P.c_sb = V; P.c_sm = V * V; }
           const int q = gb.problem(tr.t[p[0]], P);
@@ -594,7 +789,7 @@ struct Ctx {
 
   void emit_b2(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
-    GemmBuild gb(d.BK);
+    GemmBuild gb(d.prec);
     for (size_t s = 0; s < bt.size(); s++)
       for (int rho = 0; rho < 6; rho++) {
         const int* p = PERM[rho];
@@ -607,7 +802,7 @@ struct Ctx {
         GemmSeg sg{};
         sg.A = gslot((int)s);
         sg.a_sm = str(p[2]);
-        sg.B = el(r.ovvvT, (long long)tr.t[p[0]] * V * V * V);
+        sg.B = el(r.ovvvT, slab_pos(tr.t[p[0]]) * V * V * V);
         sg.b_sn = 1;
         if (p[0] != 2) {
           sg.a_sk1 = str(p[0]); sg.a_sk2 = str(p[1]); sg.b_sk1 = V * V; sg.b_sk2 = V;
@@ -624,7 +819,7 @@ struct Ctx {
 
   void emit_b3(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
-    GemmBuild gb(d.BK);
+    GemmBuild gb(d.prec);
     for (size_t s = 0; s < bt.size(); s++)
       for (int rho = 0; rho < 6; rho++) {
         const int* p = PERM[rho];
@@ -654,7 +849,7 @@ struct Ctx {
 
   void emit_b4(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
-    GemmBuild gb(d.BK);
+    GemmBuild gb(d.prec);
     for (size_t s = 0; s < bt.size(); s++)
       for (int rho = 0; rho < 6; rho++) {
         const int* p = PERM[rho];
@@ -718,11 +913,48 @@ struct Ctx {
   }
 
   void build_plan() {
+    if (d.naux > 0) {
+      build_plan_df();
+      return;
+    }
     std::vector<Triple> all = enumerate_triples(d.o);
     const long long n = (long long)all.size();
     const long long e = (t_end < 0 || t_end > n) ? n : t_end;
     const long long b = std::min(std::max(0LL, t_begin), e);
-    all = std::vector<Triple>(all.begin() + b, all.begin() + e);
+    emit_triples(std::vector<Triple>(all.begin() + b, all.begin() + e));
+  }
+
+  // Density-fitted plan: block triples I >= J >= K in order, each preceded
+  // by the slab builds (and evictions) it needs; the slice [t_begin, t_end)
+  // refers to this order.
+  void build_plan_df() {
+    df_pos.assign(d.o, -1);
+    for (auto& x : df_res) x = -1;
+    const int nb = (d.o + d.dfblk - 1) / d.dfblk;
+    const long long n = (long long)d.o * (d.o + 1) * (d.o + 2) / 6;
+    const long long e = (t_end < 0 || t_end > n) ? n : t_end;
+    const long long b = std::min(std::max(0LL, t_begin), e);
+    long long idx = 0;
+    for (int I = 0; I < nb; I++)
+      for (int J = 0; J <= I; J++)
+        for (int K = 0; K <= J; K++) {
+          const std::vector<Triple> seg = block_triples(d.o, d.dfblk, I, J, K);
+          const long long s0 = idx, s1 = idx + (long long)seg.size();
+          idx = s1;
+          const long long lo = std::max(s0, b), hi = std::min(s1, e);
+          if (lo >= hi) continue;
+          const int need[3] = {I, J, K};
+          df_ensure(need);
+          emit_triples(std::vector<Triple>(seg.begin() + (lo - s0), seg.begin() + (hi - s0)));
+        }
+    if (backward)
+      for (int u = 0; u < d.dfslots; u++)
+        if (df_res[u] >= 0) emit_df_fold(df_res[u], u);
+  }
+
+  // Batches (and, backward, ovvv-adjoint groups of S batches) over a run of
+  // triples whose slabs are resident.
+  void emit_triples(const std::vector<Triple>& all) {
     const size_t group = (size_t)B * (backward ? r.S : 1);
     for (size_t g0 = 0; g0 < all.size(); g0 += group) {
       const size_t g1 = std::min(all.size(), g0 + group);
@@ -767,16 +999,20 @@ struct Ctx {
         fam.push_back(L.kind == L_GEMM ? FAM_GEMM
                       : (L.kind == L_EPI || L.kind == L_EPIRED) ? FAM_EPILOGUE
                       : (L.kind == L_VPART || L.kind == L_VMN) ? FAM_VPART : FAM_REDUCE);
-        sub.push_back(L.kind == L_GEMM ? L.sub : -1);
+        sub.push_back((L.kind == L_GEMM || L.kind == L_ZERO) ? L.sub : -1);
         if ((e = hipEventRecord(a, st)) != hipSuccess) return e;
       }
       switch (L.kind) {
         case L_GEMM:
-          e = t3_launch_gemm(d.prec, L.cfg, reinterpret_cast<const GemmProb*>(db + L.off_a), L.n_a,
-                             reinterpret_cast<const GemmSeg*>(db + L.off_b), L.work, st);
+          e = t3_launch_gemm(L.prec >= 0 ? L.prec : d.prec, L.cfg, reinterpret_cast<const GemmProb*>(db + L.off_a),
+                             L.n_a, reinterpret_cast<const GemmSeg*>(db + L.off_b), L.work, st);
           break;
         case L_REDUCE:
-          e = t3_launch_reduce(d.prec, reinterpret_cast<const ReduceProb*>(db + L.off_a), L.n_a, L.work, st);
+          e = t3_launch_reduce(L.prec >= 0 ? L.prec : d.prec, reinterpret_cast<const ReduceProb*>(db + L.off_a),
+                               L.n_a, L.work, st);
+          break;
+        case L_ZERO:
+          e = t3_zero(ws + L.off_a, (size_t)L.work, st);
           break;
         case L_EPI:
           e = t3_launch_epilogue(d.tp_is_f64, &epi_args[L.aux], reinterpret_cast<const EpiTriple*>(db + L.off_a),
@@ -830,9 +1066,9 @@ int check_args(int nocc, int nvir, int precision) {
 
 // Size the descriptor blob by building the plan against a null workspace.
 size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B,
-                  long long t_begin = 0, long long t_end = -1, int batch = -1) {
+                  long long t_begin = 0, long long t_end = -1, int batch = -1, int naux = 0) {
   Ctx c;
-  c.d = make_dims(nocc, nvir, precision);
+  c.d = dims_for(nocc, nvir, precision, naux, backward);
   c.B = batch > 0 ? batch : choose_batch(c.d);
   c.backward = backward;
   c.t_begin = t_begin;
@@ -855,16 +1091,22 @@ struct Problem {
                *ovov = nullptr, *ovvv = nullptr;
   double *t1_bar = nullptr, *t2_bar = nullptr, *fock_bar = nullptr, *mo_energy_bar = nullptr,
          *ovoo_bar = nullptr, *ovov_bar = nullptr, *ovvv_bar = nullptr, *e_out = nullptr;
+  // density-fitted mode (naux > 0): Lov, Lvv replace ovvv, Lov_bar, Lvv_bar replace ovvv_bar
+  int naux = 0;
+  const double *Lov = nullptr, *Lvv = nullptr;
+  double *Lov_bar = nullptr, *Lvv_bar = nullptr;
 };
 
 int check_problem(const Problem& p, int precision) {
   int rc = check_args(p.nocc, p.nvir, precision);
   if (rc) return rc;
-  if (!p.mo_energy || !p.t1 || !p.t2 || !p.fock || !p.ovoo || !p.ovov || !p.ovvv)
+  const bool df = p.naux > 0;
+  if (p.naux < 0 || p.naux > 1000000) return fail(PYSCFAD_AMD_ERR_ARGS, "naux must be in 1..1000000");
+  if (!p.mo_energy || !p.t1 || !p.t2 || !p.fock || !p.ovoo || !p.ovov || (df ? (!p.Lov || !p.Lvv) : !p.ovvv))
     return fail(PYSCFAD_AMD_ERR_ARGS, "null input pointer");
   if (p.packed != 0 && p.packed != 1) return fail(PYSCFAD_AMD_ERR_ARGS, "ovvv_packed must be 0 or 1");
   if (p.backward && (!p.t1_bar || !p.t2_bar || !p.fock_bar || !p.mo_energy_bar || !p.ovoo_bar || !p.ovov_bar ||
-                     !p.ovvv_bar))
+                     (df ? (!p.Lov_bar || !p.Lvv_bar) : !p.ovvv_bar)))
     return fail(PYSCFAD_AMD_ERR_ARGS, "null output pointer");
   if (!p.backward && !p.e_out) return fail(PYSCFAD_AMD_ERR_ARGS, "null e_out");
   return PYSCFAD_AMD_OK;
@@ -885,7 +1127,7 @@ void bind(Ctx& c, const Problem& p) {
 // captured into a graph.  sev: 4 events bracketing setup / plan / finish, or
 // null.
 // kernels enqueued by enqueue_setup / enqueue_finish (upper bounds)
-constexpr int SETUP_KERNELS = 6, FINISH_KERNELS = 4;
+constexpr int SETUP_KERNELS = 10, FINISH_KERNELS = 6;
 
 hipError_t enqueue_setup(Ctx& c, const Problem& p, hipStream_t st) {
   hipError_t e;
@@ -893,12 +1135,15 @@ hipError_t enqueue_setup(Ctx& c, const Problem& p, hipStream_t st) {
   if (p.backward) {
     // G and H cubes are fully rewritten per triple; the split scratch is fully
     // rewritten before each reduction
-    const size_t acc0 = c.r.ovvv_acc, acc1 = c.r.eps_bar + (c.d.o + c.d.V) * 8;
+    const size_t acc0 = c.r.ovvv_acc, acc1 = c.r.acc_end;
     if ((e = t3_zero(c.ws + acc0, (size_t)round_up((long long)(acc1 - acc0), 16), st)) != hipSuccess) return e;
   }
   e = t3_prep_inputs(c.d.tp_is_f64, c.ws + c.r.ovvvT, c.ws + c.r.t2T, c.ws + c.r.t2H, c.ws + c.r.ovooT,
                      c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), c.at<double>(c.r.fvoP), p.ovvv,
                      p.packed, p.t2, p.ovoo, p.mo_energy, p.fock, c.d.o, c.d.v, c.d.Op, c.d.V, st);
+  if (e == hipSuccess && p.naux > 0)
+    e = t3_prep_df(c.at<double>(c.r.LovP), c.at<double>(c.r.LvvF), p.Lov, p.Lvv, p.naux, c.d.o, c.d.v, c.d.V,
+                   c.d.NP, st);
   return e;
 }
 
@@ -910,6 +1155,9 @@ hipError_t enqueue_finish(Ctx& c, const Problem& p, hipStream_t st) {
                     c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc), c.at<double>(c.r.t1_acc),
                     c.at<double>(c.r.fvo_acc), c.at<double>(c.r.eps_bar), c.d.o, c.d.v, c.d.Op, c.d.V, st);
     if (e != hipSuccess) return e;
+    if (p.naux > 0 && (e = t3_finalize_df(p.Lov_bar, p.Lvv_bar, c.at<double>(c.r.LovBar), c.at<double>(c.r.LvvBar),
+                                          p.naux, c.d.o, c.d.v, c.d.V, c.d.NP, st)) != hipSuccess)
+      return e;
   }
   if (p.e_out && (e = t3_copy_f64(p.e_out, c.at<double>(c.r.energy), 1, st)) != hipSuccess) return e;
   return hipSuccess;
@@ -977,11 +1225,11 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
   }
 
   Ctx c;
-  c.d = make_dims(p.nocc, p.nvir, precision);
+  c.d = dims_for(p.nocc, p.nvir, precision, p.naux, p.backward);
   c.t_begin = t_begin;
   c.t_end = t_end;
   bind(c, p);
-  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end);
+  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end, -1, p.naux);
   bool own = false;
   if (workspace) {
     if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
@@ -1544,6 +1792,41 @@ int pyscfad_amd_ccsd_t_energy_vjp_slice(int nocc, int nvir, const double* mo_ene
 size_t pyscfad_amd_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp) {
   if (check_args(nocc, nvir, precision)) return 0;
   return plan_bytes(nocc, nvir, precision, with_vjp != 0, nullptr, nullptr);
+}
+
+size_t pyscfad_amd_ccsd_t_df_workspace_bytes(int nocc, int nvir, int naux, int precision, int with_vjp) {
+  if (check_args(nocc, nvir, precision) || naux < 1 || naux > 1000000) return 0;
+  return plan_bytes(nocc, nvir, precision, with_vjp != 0, nullptr, nullptr, 0, -1, -1, naux);
+}
+
+int pyscfad_amd_ccsd_t_df_energy(int nocc, int nvir, int naux, const double* mo_energy, const double* t1,
+                                 const double* t2, const double* fock, const double* ovoo, const double* ovov,
+                                 const double* Lov, const double* Lvv, int precision, long long triple_begin,
+                                 long long triple_end, double* e_out, void* workspace, size_t workspace_bytes,
+                                 void* stream) {
+  Problem p = make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, nullptr, 1, false, 1.0, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, e_out);
+  p.naux = naux < 1 ? -1 : naux;
+  p.Lov = Lov;
+  p.Lvv = Lvv;
+  return run(p, precision, workspace, workspace_bytes, stream, triple_begin, triple_end);
+}
+
+int pyscfad_amd_ccsd_t_df_energy_vjp(int nocc, int nvir, int naux, const double* mo_energy, const double* t1,
+                                     const double* t2, const double* fock, const double* ovoo, const double* ovov,
+                                     const double* Lov, const double* Lvv, double et_bar, int precision,
+                                     long long triple_begin, long long triple_end, double* t1_bar, double* t2_bar,
+                                     double* fock_bar, double* mo_energy_bar, double* ovoo_bar, double* ovov_bar,
+                                     double* Lov_bar, double* Lvv_bar, double* e_out, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+  Problem p = make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, nullptr, 1, true, et_bar, t1_bar, t2_bar,
+                           fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, nullptr, e_out);
+  p.naux = naux < 1 ? -1 : naux;
+  p.Lov = Lov;
+  p.Lvv = Lvv;
+  p.Lov_bar = Lov_bar;
+  p.Lvv_bar = Lvv_bar;
+  return run(p, precision, workspace, workspace_bytes, stream, triple_begin, triple_end);
 }
 
 const char* pyscfad_amd_last_error(void) { return g_last_error.c_str(); }

@@ -407,11 +407,11 @@ extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void
   const long long n_ovvv = (long long)o * V * V * V, n_t2 = (long long)Op * Op * V * V,
                   n_ovoo = (long long)o * o * V * Op;
   if (tp_is_f64) {
-    hipLaunchKernelGGL(prep_ovvv<double>, grid_for(n_ovvv), dim3(256), 0, s, (double*)ovvvT, ovvv, o, v, V, packed);
+    if (ovvv) hipLaunchKernelGGL(prep_ovvv<double>, grid_for(n_ovvv), dim3(256), 0, s, (double*)ovvvT, ovvv, o, v, V, packed);
     hipLaunchKernelGGL(prep_t2<double>, grid_for(n_t2), dim3(256), 0, s, (double*)t2T, (double*)t2H, t2, o, v, Op, V);
     hipLaunchKernelGGL(prep_ovoo<double>, grid_for(n_ovoo), dim3(256), 0, s, (double*)ovooT, ovoo, o, v, V, Op);
   } else {
-    hipLaunchKernelGGL(prep_ovvv<float>, grid_for(n_ovvv), dim3(256), 0, s, (float*)ovvvT, ovvv, o, v, V, packed);
+    if (ovvv) hipLaunchKernelGGL(prep_ovvv<float>, grid_for(n_ovvv), dim3(256), 0, s, (float*)ovvvT, ovvv, o, v, V, packed);
     hipLaunchKernelGGL(prep_t2<float>, grid_for(n_t2), dim3(256), 0, s, (float*)t2T, (float*)t2H, t2, o, v, Op, V);
     hipLaunchKernelGGL(prep_ovoo<float>, grid_for(n_ovoo), dim3(256), 0, s, (float*)ovooT, ovoo, o, v, V, Op);
   }
@@ -460,13 +460,163 @@ extern "C" hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_b
                                   int v, int Op, int V, hipStream_t s) {
   const long long npair = (long long)v * (v + 1) / 2;
   const long long n_ovvv = packed ? (long long)o * v * npair : (long long)o * v * v * v;
-  hipLaunchKernelGGL(fin_ovvv, grid_for(n_ovvv), dim3(256), 0, s, ovvv_bar, ovvv_acc, o, v, V, packed);
+  if (ovvv_bar) hipLaunchKernelGGL(fin_ovvv, grid_for(n_ovvv), dim3(256), 0, s, ovvv_bar, ovvv_acc, o, v, V, packed);
   hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2acc, t2hacc, o, v,
                      Op, V);
   const long long n = o + v;
   const long long nsmall = (long long)o * v * o * o + (long long)o * v * o * v + (long long)o * v + n * n + n;
   hipLaunchKernelGGL(fin_small, grid_for(nsmall), dim3(256), 0, s, ovoo_bar, ovov_bar, t1_bar, fock_bar,
                      mo_energy_bar, ovoo_acc, ovov_acc, t1_acc, fvo_acc, eps_bar, o, v, Op, V);
+  return hipGetLastError();
+}
+
+// ---------------- density-fitted ovvv (slabs built on the device) ----------------
+// The DF factors of the reference's DF-CCSD (pyscfad/cc/dfccsd.py:81-91):
+//   Lov (naux, o, v),  Lvv (naux, v(v+1)/2) lower-triangle packed over (y, f),
+//   ovvv[p, x, :] = sum_P Lov[P, p, x] Lvv[P, :]   (dfccsd.py:91)
+// are re-laid out with the auxiliary index P innermost and padded to NP, so
+// that a slab build and both cotangent folds are GEMMs whose operands are
+// k-contiguous or row-contiguous strided views:
+//   LovP[p][x][P]  (o x V x NP),   LvvF[y][f][P]  (V x V x NP, both triangles).
+// Rows x, y, f >= v and P >= naux stay zero.  64 x 64 tiles through LDS keep
+// both the reads and the writes coalesced.
+
+// row-major lower-triangle pair index -> (y, f), f <= y
+__device__ __forceinline__ void tri_decode(long long pair, int& y, int& f) {
+  int r = (int)((sqrt(8.0 * (double)pair + 1.0) - 1.0) * 0.5);
+  while ((long long)r * (r + 1) / 2 > pair) r--;
+  while ((long long)(r + 1) * (r + 2) / 2 <= pair) r++;
+  y = r;
+  f = (int)(pair - (long long)r * (r + 1) / 2);
+}
+
+// grid (ceil(o*v / 64), ceil(NP / 64))
+__global__ void __launch_bounds__(256) df_lov_in(double* __restrict__ LovP, const double* __restrict__ Lov,
+                                                 int naux, int o, int v, int V, int NP) {
+  __shared__ double tile[64][65];
+  const long long ncol = (long long)o * v;
+  const long long c0 = (long long)blockIdx.x * 64;
+  const int P0 = blockIdx.y * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + r;
+    const long long c = c0 + tx;
+    tile[r][tx] = (P < naux && c < ncol) ? Lov[(long long)P * ncol + c] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + tx;
+    const long long c = c0 + r;
+    if (c < ncol && P < NP) LovP[((c / v) * V + c % v) * NP + P] = tile[tx][r];
+  }
+}
+
+// grid (ceil(v(v+1)/2 / 64), ceil(NP / 64))
+__global__ void __launch_bounds__(256) df_lvv_in(double* __restrict__ LvvF, const double* __restrict__ Lvv,
+                                                 int naux, int v, int V, int NP) {
+  __shared__ double tile[64][65];
+  const long long npair = (long long)v * (v + 1) / 2;
+  const long long c0 = (long long)blockIdx.x * 64;
+  const int P0 = blockIdx.y * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + r;
+    const long long c = c0 + tx;
+    tile[r][tx] = (P < naux && c < npair) ? Lvv[(long long)P * npair + c] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + tx;
+    const long long c = c0 + r;
+    if (c < npair && P < NP) {
+      int y, f;
+      tri_decode(c, y, f);
+      const double x = tile[tx][r];
+      LvvF[((long long)y * V + f) * NP + P] = x;
+      if (y != f) LvvF[((long long)f * V + y) * NP + P] = x;
+    }
+  }
+}
+
+// Lov_bar[P, p, x] = LovBar[p][x][P];  grid (ceil(o*v / 64), ceil(naux / 64))
+__global__ void __launch_bounds__(256) df_lov_out(double* __restrict__ Lov_bar, const double* __restrict__ LovBar,
+                                                  int naux, int o, int v, int V, int NP) {
+  __shared__ double tile[64][65];
+  const long long ncol = (long long)o * v;
+  const long long c0 = (long long)blockIdx.x * 64;
+  const int P0 = blockIdx.y * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + tx;
+    const long long c = c0 + r;
+    tile[r][tx] = (c < ncol && P < naux) ? LovBar[((c / v) * V + c % v) * NP + P] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + r;
+    const long long c = c0 + tx;
+    if (P < naux && c < ncol) Lov_bar[(long long)P * ncol + c] = tile[tx][r];
+  }
+}
+
+// Lvv_bar[P, pair(y, f)] = LvvBar[y][f][P] + LvvBar[f][y][P] (y != f), LvvBar[y][y][P]
+// (the adjoint of the packed-to-symmetric unpacking, pyscfad/lib/_numpy_helper_opt.py:32-55);
+// grid (ceil(v(v+1)/2 / 64), ceil(naux / 64))
+__global__ void __launch_bounds__(256) df_lvv_out(double* __restrict__ Lvv_bar, const double* __restrict__ LvvBar,
+                                                  int naux, int v, int V, int NP) {
+  __shared__ double tile[64][65];
+  const long long npair = (long long)v * (v + 1) / 2;
+  const long long c0 = (long long)blockIdx.x * 64;
+  const int P0 = blockIdx.y * 64, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + tx;
+    const long long c = c0 + r;
+    double x = 0.0;
+    if (c < npair && P < naux) {
+      int y, f;
+      tri_decode(c, y, f);
+      x = LvvBar[((long long)y * V + f) * NP + P];
+      if (y != f) x += LvvBar[((long long)f * V + y) * NP + P];
+    }
+    tile[r][tx] = x;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int i = 0; i < 16; i++) {
+    const int r = ty + 4 * i, P = P0 + r;
+    const long long c = c0 + tx;
+    if (P < naux && c < npair) Lvv_bar[(long long)P * npair + c] = tile[tx][r];
+  }
+}
+
+extern "C" hipError_t t3_zero(void* p, size_t bytes, hipStream_t s);
+
+extern "C" hipError_t t3_prep_df(double* LovP, double* LvvF, const double* Lov, const double* Lvv, int naux, int o,
+                                 int v, int V, int NP, hipStream_t s) {
+  hipError_t e;
+  if ((e = t3_zero(LovP, (size_t)o * V * NP * 8, s)) != hipSuccess) return e;
+  if ((e = t3_zero(LvvF, (size_t)V * V * NP * 8, s)) != hipSuccess) return e;
+  const long long npair = (long long)v * (v + 1) / 2;
+  const unsigned pt = (unsigned)((NP + 63) / 64);
+  hipLaunchKernelGGL(df_lov_in, dim3((unsigned)(((long long)o * v + 63) / 64), pt), dim3(256), 0, s, LovP, Lov,
+                     naux, o, v, V, NP);
+  hipLaunchKernelGGL(df_lvv_in, dim3((unsigned)((npair + 63) / 64), pt), dim3(256), 0, s, LvvF, Lvv, naux, v, V,
+                     NP);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_finalize_df(double* Lov_bar, double* Lvv_bar, const double* LovBar, const double* LvvBar,
+                                     int naux, int o, int v, int V, int NP, hipStream_t s) {
+  const long long npair = (long long)v * (v + 1) / 2;
+  const unsigned pt = (unsigned)((naux + 63) / 64);
+  hipLaunchKernelGGL(df_lov_out, dim3((unsigned)(((long long)o * v + 63) / 64), pt), dim3(256), 0, s, Lov_bar,
+                     LovBar, naux, o, v, V, NP);
+  hipLaunchKernelGGL(df_lvv_out, dim3((unsigned)((npair + 63) / 64), pt), dim3(256), 0, s, Lvv_bar, LvvBar, naux,
+                     v, V, NP);
   return hipGetLastError();
 }
 

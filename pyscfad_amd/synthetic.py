@@ -49,12 +49,15 @@ def _draws(nocc, nvir, seed, fvo_scale):
     return mo_energy, L, t1, t2_noise, fvo
 
 
-def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None):
+def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None, df=False):
     """Return a dict with t1, t2, mo_energy, fock, ovoo, ovov, ovvv (FP64).
 
     With ``device=None`` everything is NumPy on the host.  With a torch device
     string (e.g. 'cuda') the arrays are torch FP64 tensors on that device and
-    the O(o v^3) slice is built there.
+    the O(o v^3) slice is built there.  With ``df=True`` the dict holds the DF
+    factors in the layouts of pyscfad/cc/dfccsd.py:82-83 in place of ovvv:
+    Lov (naux, o, v) and Lvv (naux, v(v+1)/2), lower-triangle packed, with
+    ovvv = Lov^T Lvv exactly the packed ovvv above.
     """
     o, v = nocc, nvir
     mo_energy, L, t1, t2_noise, fvo = _draws(o, v, seed, fvo_scale)
@@ -72,14 +75,21 @@ def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None):
         ovov = (Lov.T @ Lov).reshape(o, v, o, v)
         ovoo = (Lov.T @ L[:, :o, :o].reshape(L.shape[0], o * o)).reshape(o, v, o, o)
         Lvv = L[:, o:, o:]
-        if packed:
+        if df:
+            ovvv = None
+        elif packed:
             ovvv = (Lov.T @ Lvv[:, idx_b, idx_c]).reshape(o, v, v * (v + 1) // 2)
         else:
             ovvv = (Lov.T @ Lvv.reshape(L.shape[0], v * v)).reshape(o, v, v, v)
         t2 = ovov.transpose(0, 2, 1, 3) / d_ijab + t2_noise
         t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
-        return dict(t1=t1, t2=np.ascontiguousarray(t2), mo_energy=mo_energy, fock=fock,
-                    ovoo=ovoo, ovov=ovov, ovvv=ovvv)
+        out = dict(t1=t1, t2=np.ascontiguousarray(t2), mo_energy=mo_energy, fock=fock, ovoo=ovoo, ovov=ovov)
+        if df:
+            out["Lov"] = np.ascontiguousarray(Lov.reshape(L.shape[0], o, v))
+            out["Lvv"] = np.ascontiguousarray(Lvv[:, idx_b, idx_c])
+        else:
+            out["ovvv"] = ovvv
+        return out
     import torch
     dev = torch.device(device)
     Lt = torch.from_numpy(L).to(dev)
@@ -89,7 +99,13 @@ def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None):
     ovov = (Lov.T @ Lov).reshape(o, v, o, v)
     ovoo = (Lov.T @ Lt[:, :o, :o].reshape(naux, o * o)).reshape(o, v, o, o)
     Lvv = Lt[:, o:, o:]
-    if packed:
+    if df:
+        ib = torch.from_numpy(idx_b).to(dev)
+        ic = torch.from_numpy(idx_c).to(dev)
+        Lvv_p = Lvv[:, ib, ic].contiguous()
+        Lov_3 = Lov.reshape(naux, o, v).contiguous()
+        ovvv = None
+    elif packed:
         ib = torch.from_numpy(idx_b).to(dev)
         ic = torch.from_numpy(idx_c).to(dev)
         Lvv_p = Lvv[:, ib, ic]
@@ -103,10 +119,16 @@ def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None):
     d = torch.from_numpy(d_ijab).to(dev)
     t2 = ovov.permute(0, 2, 1, 3) / d + torch.from_numpy(t2_noise).to(dev)
     t2 = (0.5 * (t2 + t2.permute(1, 0, 3, 2))).contiguous()
-    return dict(t1=torch.from_numpy(t1).to(dev), t2=t2,
-                mo_energy=torch.from_numpy(mo_energy).to(dev),
-                fock=torch.from_numpy(fock).to(dev), ovoo=ovoo.contiguous(),
-                ovov=ovov.contiguous(), ovvv=ovvv)
+    out = dict(t1=torch.from_numpy(t1).to(dev), t2=t2,
+               mo_energy=torch.from_numpy(mo_energy).to(dev),
+               fock=torch.from_numpy(fock).to(dev), ovoo=ovoo.contiguous(),
+               ovov=ovov.contiguous())
+    if df:
+        out["Lov"] = Lov_3
+        out["Lvv"] = Lvv_p
+    else:
+        out["ovvv"] = ovvv
+    return out
 
 
 def batched_shapes(count=64, seed=4, nocc_range=(4, 24), nvir_range=(40, 200)):

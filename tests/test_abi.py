@@ -98,3 +98,40 @@ def test_batched_argument_errors():
     arr[1].ovvv = None
     assert lib.pyscfad_amd_ccsd_t_batched(2, arr, 1, 1, None, 0, None) == 1
     assert b"instance 1" in lib.pyscfad_amd_last_error()
+
+
+def test_df_workspace_bytes(monkeypatch):
+    lib = _lib.load()
+    assert lib.pyscfad_amd_ccsd_t_df_workspace_bytes(5, 19, 72, 0, 0) > 0
+    assert lib.pyscfad_amd_ccsd_t_df_workspace_bytes(5, 19, 0, 0, 0) == 0
+    assert lib.pyscfad_amd_ccsd_t_df_workspace_bytes(0, 19, 72, 0, 0) == 0
+    # with several occupied blocks only three blocks of slabs (and, backward,
+    # their FP64 accumulators) are held, instead of all nocc
+    monkeypatch.setenv("PYSCFAD_AMD_DF_BLOCK", "2")
+    o, v, naux = 24, 96, 3 * 120
+    for prec in (0, 1):
+        dense = lib.pyscfad_amd_ccsd_t_workspace_bytes(o, v, prec, 1)
+        df = lib.pyscfad_amd_ccsd_t_df_workspace_bytes(o, v, naux, prec, 1)
+        elt = 8 if prec == 0 else 4
+        saved = (o - 6) * v ** 3 * (elt + 8)
+        factors = 2 * (o * v + v * v) * naux * 8   # re-laid-out factors and their accumulators (unpadded)
+        assert df <= dense - saved + 2 * factors, (prec, dense, df)
+        assert df < dense
+
+
+def test_df_argument_errors():
+    lib = _lib.load()
+    dummy = ctypes.c_void_p(256)
+    # naux < 1
+    rc = lib.pyscfad_amd_ccsd_t_df_energy(3, 5, 0, *([dummy] * 8), 0, 0, -1, dummy, None, 0, None)
+    assert rc == 1
+    # null DF factor
+    rc = lib.pyscfad_amd_ccsd_t_df_energy(3, 5, 24, *([dummy] * 6), None, dummy, 0, 0, -1, dummy, None, 0, None)
+    assert rc == 1 and b"null" in lib.pyscfad_amd_last_error()
+    # null Lvv_bar in the VJP
+    rc = lib.pyscfad_amd_ccsd_t_df_energy_vjp(3, 5, 24, *([dummy] * 8), 1.0, 0, 0, -1, *([dummy] * 7), None,
+                                              dummy, None, 0, None)
+    assert rc == 1 and b"null" in lib.pyscfad_amd_last_error()
+    # workspace too small
+    rc = lib.pyscfad_amd_ccsd_t_df_energy(3, 5, 24, *([dummy] * 8), 0, 0, -1, dummy, dummy, 16, None)
+    assert rc == 4

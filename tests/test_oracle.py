@@ -162,3 +162,39 @@ def test_planted_convention_mistake_is_detected():
     # symmetrising t2 restores agreement
     d["t2"] = 0.5 * (d["t2"] + d["t2"].transpose(1, 0, 3, 2))
     assert abs(orc.energy(**d) - sf.energy_vectorised(**d)) < 1e-15
+
+
+def test_df_chain_rule_finite_differences():
+    """The density-fitted cotangents (oracle/df.py) against central differences
+    of the oracle energy with ovvv = Lov^T Lvv, along random directions and
+    element by element on diagonal and off-diagonal packed Lvv pairs (a
+    missing or doubled off-diagonal fold fails the latter)."""
+    from oracle import df as odf
+    o, v = 3, 5
+    d = make_instance(o, v, 71, fvo_scale=0.05, df=True)
+    Lov, Lvv = d.pop("Lov"), d.pop("Lvv")
+
+    def e_of(lov, lvv):
+        return orc.energy(**d, ovvv=odf.ovvv_from_df(lov, lvv))
+
+    g = orc.energy_vjp(**d, ovvv=odf.ovvv_from_df(Lov, Lvv), et_bar=1.0)
+    lov_bar, lvv_bar = odf.df_vjp(Lov, Lvv, g["ovvv"])
+    assert lov_bar.shape == Lov.shape and lvv_bar.shape == Lvv.shape
+    rng = np.random.default_rng(72)
+    h = 1e-4
+    for which in ("Lov", "Lvv"):
+        base = Lov if which == "Lov" else Lvv
+        direction = rng.normal(size=base.shape)
+        if which == "Lov":
+            fd = (e_of(Lov + h * direction, Lvv) - e_of(Lov - h * direction, Lvv)) / (2 * h)
+            an = np.vdot(lov_bar, direction)
+        else:
+            fd = (e_of(Lov, Lvv + h * direction) - e_of(Lov, Lvv - h * direction)) / (2 * h)
+            an = np.vdot(lvv_bar, direction)
+        assert abs(fd - an) <= 1e-7 * max(abs(an), 1e-12), (which, fd, an)
+    # packed pairs: 0 = (0,0), 4 = (2,1), 9 = (3,3), 13 = (4,3)
+    for P, pair in ((0, 0), (3, 4), (7, 9), (11, 13)):
+        direction = np.zeros_like(Lvv)
+        direction[P, pair] = 1.0
+        fd = (e_of(Lov, Lvv + h * direction) - e_of(Lov, Lvv - h * direction)) / (2 * h)
+        assert abs(fd - lvv_bar[P, pair]) <= 1e-7 * max(abs(fd), 1e-9), (P, pair, fd, lvv_bar[P, pair])
