@@ -28,8 +28,9 @@ def _hipcc():
 
 def _compile(src):
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
-    cmd = [_hipcc()] + COMMON + (DEVICE if src.endswith(".hip") else []) + [
-        "-c", os.path.join(CSRC, src), "-o", obj]
+    # hipcc compiles .cpp as HIP too (host code here, but its device pass must
+    # target gfx950 like the rest, not the compiler's default architecture)
+    cmd = [_hipcc()] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")

@@ -43,6 +43,7 @@ enum GemmMode : int {
 
 struct GemmProb {
   int nb, M, N;
+  int Mv, Nv;         // rows / columns of C that exist (<= M, N; tiles and operand loads use M, N)
   int seg0, nseg;
   int nchunks;        // total K-chunks over all segments
   int ksplit;
@@ -60,6 +61,7 @@ struct GemmProb {
 struct ReduceProb {
   const void* P;
   int ksplit, nb, M, N;
+  int Mv, Nv;         // rows / columns of C that exist
   void* C;
   long long c_sb, c_sm, c_sn;
   int accumulate;

@@ -50,7 +50,7 @@ hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void
                           int V, hipStream_t s);
 hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs, int nslab,
                            const VpContrib* contribs, int o, int v, int Op, int V, const double* t1,
-                           const double* fvoP, double* ovov_acc, double* t2h_acc, double* t1_acc, double* fvo_acc,
+                           const double* fvoP, double* ovov_bar, double* t2h_acc, double* t1_acc, double* fvo_acc,
                            hipStream_t s);
 hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
                          const double* t2, hipStream_t s);
@@ -60,11 +60,11 @@ hipError_t t3_prep_df(double* LovP, double* LvvF, const double* Lov, const doubl
 hipError_t t3_finalize_df(double* Lov_bar, double* Lvv_bar, const double* LovBar, const double* LvvBar, int naux,
                           int o, int v, int V, int NP, hipStream_t s);
 hipError_t t3_copy_f64(double* dst, const double* src, int n, hipStream_t s);
+hipError_t t3_zero_f64(double* p, long long n, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
-                       double* ovov_bar, double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2acc,
-                       const double* t2hacc, const double* ovoo_acc, const double* ovov_acc,
-                       const double* t1_acc, const double* fvo_acc, const double* eps_bar, int o, int v, int Op,
-                       int V, hipStream_t s);
+                       double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2hacc,
+                       const double* ovoo_acc, const double* t1_acc, const double* fvo_acc, const double* eps_bar,
+                       int o, int v, int Op, int V, hipStream_t s);
 }
 
 namespace {
@@ -180,7 +180,7 @@ struct Layout {
 
 struct Regions {
   size_t ovvvT, t2T, t2H, ovooT, eps_occ, eps_vir, fvoP, energy;
-  size_t ovvv_acc, t2acc, t2hacc, ovov_acc, ovoo_acc, t1_acc, fvo_acc, eps_bar;
+  size_t ovvv_acc, t2hacc, ovoo_acc, t1_acc, fvo_acc, eps_bar;   // (ovov_bar and most of t2_bar accumulate in place)
   size_t LovP, LvvF, LovBar, LvvBar;   // density-fitted mode only
   size_t acc_end;                      // end of the FP64 accumulators cleared at setup
   size_t dfsplit, dfsplit_bytes;       // split-K scratch of the Lov cotangent folds
@@ -249,9 +249,7 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   }
   if (backward) {
     r.ovvv_acc = L.take(nslab * V * V * V * 8);
-    r.t2acc = L.take(Op * Op * V * V * 8);
     r.t2hacc = L.take(Op * Op * V * V * 8);
-    r.ovov_acc = L.take(o * o * V * V * 8);
     r.ovoo_acc = L.take(o * o * V * Op * 8);
     r.t1_acc = L.take(o * V * 8);
     r.fvo_acc = L.take(o * V * 8);
@@ -376,6 +374,9 @@ struct Ctx {
   const double *mo_energy = nullptr, *t1 = nullptr, *t2 = nullptr, *fock = nullptr, *ovoo = nullptr,
                *ovov = nullptr, *ovvv = nullptr;
   int packed = 1;
+  // cotangent outputs accumulated in place by the plan (device, reference layouts)
+  double *t2_out = nullptr, *ovov_out = nullptr;
+  char* outp(double* base, size_t elems) const { return reinterpret_cast<char*>(base) + elems * 8; }
   Blob blob;
   std::vector<Launch> launches;
   std::vector<EpiArgs> epi_args;
@@ -424,6 +425,8 @@ struct Ctx {
     int min_chunks = 1 << 30;
     for (size_t p = 0; p < gb.probs.size(); p++) {
       GemmProb& P = gb.probs[p];
+      if (P.Mv <= 0) P.Mv = P.M;
+      if (P.Nv <= 0) P.Nv = P.N;
       P.tiles_m = (P.M + BM - 1) / BM;
       P.tiles_n = (P.N + BN - 1) / BN;
       tiles += (long long)P.nb * P.tiles_m * P.tiles_n;
@@ -486,6 +489,8 @@ struct Ctx {
         R.nb = P.nb;
         R.M = P.M;
         R.N = P.N;
+        R.Mv = P.Mv;
+        R.Nv = P.Nv;
         R.C = P.C;
         R.c_sb = P.c_sb;
         R.c_sm = P.c_sm;
@@ -788,16 +793,17 @@ struct Ctx {
   }
 
   void emit_b2(const std::vector<Triple>& bt) {
-    const long long V = d.V, Op = d.Op;
+    const long long V = d.V;
     GemmBuild gb(d.prec);
     for (size_t s = 0; s < bt.size(); s++)
       for (int rho = 0; rho < 6; rho++) {
         const int* p = PERM[rho];
         const Triple& tr = bt[s];
+        // t2_bar[t_p2][t_p1][m][n], accumulated in place (rows and columns past v not written)
         GemmProb P{};
-        P.nb = 1; P.M = d.V; P.N = d.V;
-        P.C = ws ? ws + r.t2acc + ((size_t)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V * 8 : nullptr;
-        P.c_sm = V; P.c_sn = 1; P.alpha = 1.0;
+        P.nb = 1; P.M = d.V; P.N = d.V; P.Mv = d.v; P.Nv = d.v;
+        P.C = outp(t2_out, ((size_t)tr.t[p[2]] * d.o + tr.t[p[1]]) * d.v * d.v);
+        P.c_sm = d.v; P.c_sn = 1; P.alpha = 1.0;
         const int q = gb.problem((long long)tr.t[p[2]] * 4096 + tr.t[p[1]], P);
         GemmSeg sg{};
         sg.A = gslot((int)s);
@@ -855,11 +861,23 @@ struct Ctx {
         const int* p = PERM[rho];
         const Triple& tr = bt[s];
         const bool t2layout = (p[1] == 2);
+        // t2 layout (the cube's c is t_p1's partner): t2_bar[t_p2][m][b][n],
+        // accumulated in place (batches and columns: the v that exist, rows:
+        // the o that exist); otherwise the [i][j][b][a] accumulator t2hacc,
+        // added to t2_bar transposed at the end (writing t2_bar directly
+        // would take uncoalesced stores: the cube's contiguous index is not
+        // t2_bar's)
         GemmProb P{};
-        P.nb = d.V; P.M = d.Op; P.N = d.V;
-        const size_t region = t2layout ? r.t2acc : r.t2hacc;
-        P.C = ws ? ws + region + (size_t)tr.t[p[2]] * Op * V * V * 8 : nullptr;
-        P.c_sm = V * V; P.c_sb = V; P.c_sn = 1; P.alpha = 1.0;
+        if (t2layout) {
+          P.nb = d.v; P.M = d.Op; P.N = d.V; P.Mv = d.o; P.Nv = d.v;
+          P.C = outp(t2_out, (size_t)tr.t[p[2]] * d.o * d.v * d.v);
+          P.c_sb = d.v; P.c_sm = (long long)d.v * d.v; P.c_sn = 1;
+        } else {
+          P.nb = d.V; P.M = d.Op; P.N = d.V;
+          P.C = ws ? ws + r.t2hacc + (size_t)tr.t[p[2]] * Op * V * V * 8 : nullptr;
+          P.c_sb = V; P.c_sm = V * V; P.c_sn = 1;
+        }
+        P.alpha = 1.0;
         const int q = gb.problem((t2layout ? 0 : 1) * 4096LL + tr.t[p[2]], P);
         GemmSeg sg{};
         sg.A = el(r.ovooT, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
@@ -1031,7 +1049,7 @@ struct Ctx {
           e = t3_launch_vpart(d.tp_is_f64, reinterpret_cast<const VpTriple*>(db + L.off_a), L.n_a,
                               reinterpret_cast<const VpSlab*>(db + L.off_b), L.n_b,
                               reinterpret_cast<const VpContrib*>(db + L.off_c), d.o, d.v, d.Op, d.V, t1,
-                              at<double>(r.fvoP), at<double>(r.ovov_acc), at<double>(r.t2hacc),
+                              at<double>(r.fvoP), ovov_out, at<double>(r.t2hacc),
                               at<double>(r.t1_acc), at<double>(r.fvo_acc), st);
           break;
       }
@@ -1117,6 +1135,8 @@ void bind(Ctx& c, const Problem& p) {
   c.ovvv = p.ovvv;
   c.packed = p.packed;
   c.backward = p.backward;
+  c.t2_out = p.t2_bar;
+  c.ovov_out = p.ovov_bar;
   c.et_bar = p.et_bar;
 }
 
@@ -1127,7 +1147,7 @@ void bind(Ctx& c, const Problem& p) {
 // captured into a graph.  sev: 4 events bracketing setup / plan / finish, or
 // null.
 // kernels enqueued by enqueue_setup / enqueue_finish (upper bounds)
-constexpr int SETUP_KERNELS = 10, FINISH_KERNELS = 6;
+constexpr int SETUP_KERNELS = 12, FINISH_KERNELS = 6;
 
 hipError_t enqueue_setup(Ctx& c, const Problem& p, hipStream_t st) {
   hipError_t e;
@@ -1137,6 +1157,10 @@ hipError_t enqueue_setup(Ctx& c, const Problem& p, hipStream_t st) {
     // rewritten before each reduction
     const size_t acc0 = c.r.ovvv_acc, acc1 = c.r.acc_end;
     if ((e = t3_zero(c.ws + acc0, (size_t)round_up((long long)(acc1 - acc0), 16), st)) != hipSuccess) return e;
+    // t2_bar and ovov_bar are accumulated in place
+    const long long n22 = (long long)c.d.o * c.d.o * c.d.v * c.d.v;
+    if ((e = t3_zero_f64(p.t2_bar, n22, st)) != hipSuccess) return e;
+    if ((e = t3_zero_f64(p.ovov_bar, n22, st)) != hipSuccess) return e;
   }
   e = t3_prep_inputs(c.d.tp_is_f64, c.ws + c.r.ovvvT, c.ws + c.r.t2T, c.ws + c.r.t2H, c.ws + c.r.ovooT,
                      c.at<double>(c.r.eps_occ), c.at<double>(c.r.eps_vir), c.at<double>(c.r.fvoP), p.ovvv,
@@ -1150,9 +1174,8 @@ hipError_t enqueue_setup(Ctx& c, const Problem& p, hipStream_t st) {
 hipError_t enqueue_finish(Ctx& c, const Problem& p, hipStream_t st) {
   hipError_t e;
   if (p.backward) {
-    e = t3_finalize(p.t1_bar, p.t2_bar, p.fock_bar, p.mo_energy_bar, p.ovoo_bar, p.ovov_bar, p.ovvv_bar, p.packed,
-                    c.at<double>(c.r.ovvv_acc), c.at<double>(c.r.t2acc), c.at<double>(c.r.t2hacc),
-                    c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.ovov_acc), c.at<double>(c.r.t1_acc),
+    e = t3_finalize(p.t1_bar, p.t2_bar, p.fock_bar, p.mo_energy_bar, p.ovoo_bar, p.ovvv_bar, p.packed,
+                    c.at<double>(c.r.ovvv_acc), c.at<double>(c.r.t2hacc), c.at<double>(c.r.ovoo_acc), c.at<double>(c.r.t1_acc),
                     c.at<double>(c.r.fvo_acc), c.at<double>(c.r.eps_bar), c.d.o, c.d.v, c.d.Op, c.d.V, st);
     if (e != hipSuccess) return e;
     if (p.naux > 0 && (e = t3_finalize_df(p.Lov_bar, p.Lvv_bar, c.at<double>(c.r.LovBar), c.at<double>(c.r.LvvBar),
@@ -1304,14 +1327,16 @@ int tune_batch(int nocc, int nvir, int precision, hipStream_t st) {
     int B = cand[x];
     need = std::max(need, plan_bytes(nocc, nvir, precision, true, &rr[x], &B, t0, t1, cand[x]));
   }
+  // the trial plans accumulate t2_bar in place: a scratch one after the workspace
+  const size_t t2_bytes = (size_t)round_up((long long)nocc * nocc * nvir * nvir * 8, 256);
   char* ws = nullptr;
-  if (hipMalloc(&ws, need) != hipSuccess) {
+  if (hipMalloc(&ws, need + t2_bytes) != hipSuccess) {
     (void)hipGetLastError();
     return b0;
   }
   static const double dummy[4] = {0, 0, 0, 0};
   Ctx plans[2];
-  hipError_t e = t3_zero(ws, need / 16 * 16, st);
+  hipError_t e = t3_zero(ws, (need + t2_bytes) / 16 * 16, st);
   for (int x = 0; x < 2; x++) {
     Ctx& c = plans[x];
     c.d = d;
@@ -1322,6 +1347,7 @@ int tune_batch(int nocc, int nvir, int precision, hipStream_t st) {
     c.backward = true;
     c.ws = ws;
     c.mo_energy = c.t1 = c.t2 = c.fock = c.ovoo = c.ovov = c.ovvv = dummy;
+    c.t2_out = reinterpret_cast<double*>(ws + need);
     c.build_plan();
   }
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1370,7 +1396,9 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   const long long t0 = ntrip / 2, t1 = std::min(ntrip, t0 + B);
   // the backward plan when its workspace fits, else the forward plan (W build only)
   bool bw = true;
-  size_t need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1);
+  // the backward trial plans accumulate t2_bar in place: a scratch one after the workspace
+  const size_t t2_bytes = (size_t)round_up((long long)nocc * nocc * nvir * nvir * 8, 256);
+  size_t need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1) + t2_bytes;
   char* ws = nullptr;
   if (hipMalloc(&ws, need) != hipSuccess) {
     (void)hipGetLastError();
@@ -1413,6 +1441,7 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
       c.backward = bw;
       c.ws = ws;
       c.mo_energy = c.t1 = c.t2 = c.fock = c.ovoo = c.ovov = c.ovvv = dummy;
+      if (bw) c.t2_out = reinterpret_cast<double*>(ws + need - t2_bytes);
       c.force_cfg[fam] = cfg;
       c.build_plan();
       if (!c.plan_error.empty()) continue;
@@ -1711,8 +1740,6 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
 
 }  // namespace
 
-extern "C" {
-
 namespace {
 Problem make_problem(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
                      const double* fock, const double* ovoo, const double* ovov, const double* ovvv, int packed,
@@ -1726,6 +1753,8 @@ Problem make_problem(int nocc, int nvir, const double* mo_energy, const double* 
   return p;
 }
 }  // namespace
+
+extern "C" {
 
 int pyscfad_amd_ccsd_t_energy(int nocc, int nvir, const double* mo_energy, const double* t1, const double* t2,
                               const double* fock, const double* ovoo, const double* ovov, const double* ovvv,

@@ -213,12 +213,13 @@ __device__ __forceinline__ TileCoord find_tile(const GemmProb* __restrict__ prob
 template <class Acc>
 __device__ __forceinline__ void write_out(const GemmProb& P, const TileCoord& tc, int m, int n,
                                           Acc v) {
-  if (m >= P.M || n >= P.N) return;
   if (P.mode == GEMM_PARTIAL) {
+    if (m >= P.M || n >= P.N) return;
     T3_GLOBAL Acc* out = (T3_GLOBAL Acc*)P.P;
     out[((long long)(tc.split * P.nb + tc.b) * P.M + m) * P.N + n] = v;
     return;
   }
+  if (m >= P.Mv || n >= P.Nv) return;
   const long long off = (long long)tc.b * P.c_sb + (long long)m * P.c_sm + (long long)n * P.c_sn;
   const double x = P.alpha * (double)v;
   if (P.c_is_f64) {
@@ -498,6 +499,7 @@ reduce_partials(const ReduceProb* __restrict__ probs, int nprob, long long total
   const int n = (int)(local % R.N);
   const int m = (int)((local / R.N) % R.M);
   const int b = (int)(local / ((long long)R.N * R.M));
+  if (m >= R.Mv || n >= R.Nv) return;
   const T3_GLOBAL Acc* P = (const T3_GLOBAL Acc*)R.P;
   double s = 0.0;
   for (int sp = 0; sp < R.ksplit; sp++) s += (double)P[sp * per + local];

@@ -258,21 +258,24 @@ vcm_build(const VpTriple* __restrict__ trs, int o, int v, int V, const double* _
   tr.out[(VP_CM + task) * VV + e] = val;
 }
 
-// Scatter of the V-term adjoints of one batch into the FP64 accumulators.
-// Slab tasks (blockIdx.y < nslab): element (x,y) of ovov_acc[p][q] and of
-// t2h_acc[q][p] (the [i][j][b][a] accumulator of t2_bar, shared with the hole
-// adjoint) += 1/2 of every (triple, d, transposed) contribution, in list
-// order.  Vector tasks (blockIdx.y == nslab): t1_acc[r][z], fvo_acc[r][z].
+// Scatter of the V-term adjoints of one batch.  Slab tasks (blockIdx.y <
+// nslab): element (x, y) of the slab (p, q) adds 1/2 of every (triple, d,
+// transposed) contribution, in list order, to ovov_bar[p, x, q, y] (in place)
+// and to t2h_acc[q][p][x][y] (the [i][j][b][a] accumulator of t2_bar, shared
+// with the hole adjoint).  Vector tasks (blockIdx.y == nslab): t1_acc[r][z],
+// fvo_acc[r][z].
 
 __global__ void __launch_bounds__(256)
 vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __restrict__ contribs,
-              const VpTriple* __restrict__ trs, int ntriple, int o, int Op, int V, double* __restrict__ ovov_acc,
-              double* __restrict__ t2h_acc, double* __restrict__ t1_acc, double* __restrict__ fvo_acc) {
+              const VpTriple* __restrict__ trs, int ntriple, int o, int v, int Op, int V,
+              double* __restrict__ ovov_bar, double* __restrict__ t2h_acc, double* __restrict__ t1_acc,
+              double* __restrict__ fvo_acc) {
   const long long VV = (long long)V * V;
   if ((int)blockIdx.y < nslab) {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= VV) return;
     const int x = (int)(e / V), y = (int)(e % V);
+    if (x >= v || y >= v) return;
     const VpSlab sl = slabs[blockIdx.y];
     double am = 0.0, af = 0.0;
     for (int c = sl.c0; c < sl.c0 + sl.nc; c++) {
@@ -282,7 +285,7 @@ vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __re
       am += out[(2 * ct.d) * VV + idx];
       af += out[(2 * ct.d + 1) * VV + idx];
     }
-    ovov_acc[((long long)sl.p * o + sl.q) * VV + e] += 0.5 * am;
+    ovov_bar[(((long long)sl.p * v + x) * o + sl.q) * v + y] += 0.5 * am;
     t2h_acc[((long long)sl.q * Op + sl.p) * VV + e] += 0.5 * af;
     return;
   }
@@ -332,30 +335,27 @@ __global__ void fin_ovvv(double* __restrict__ out, const double* __restrict__ ac
   }
 }
 
-// t2_bar[i,j,a,b] = t2acc[i][j][a][b] + t2hacc[i][j][b][a]
-__global__ void fin_t2(double* __restrict__ out, const double* __restrict__ t2acc,
-                       const double* __restrict__ t2hacc, int o, int v, int Op, int V) {
+// t2_bar[i,j,a,b] += t2hacc[i][j][b][a] (the rest of t2_bar was accumulated in place)
+__global__ void fin_t2(double* __restrict__ out, const double* __restrict__ t2hacc, int o, int v, int Op, int V) {
   const long long total = (long long)o * o * v * v;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     const int b = (int)(e % v), a = (int)((e / v) % v);
     const int j = (int)((e / ((long long)v * v)) % o), i = (int)(e / ((long long)v * v * o));
-    const long long pij = ((long long)i * Op + j) * V * V;
-    out[e] = t2acc[pij + (long long)a * V + b] + t2hacc[pij + (long long)b * V + a];
+    out[e] += t2hacc[((long long)i * Op + j) * V * V + (long long)b * V + a];
   }
 }
 
-// ovoo_bar[p,x,m,q] = acc[p][q][x][m];  ovov_bar[p,x,q,y] = acc[p][q][x][y]
-__global__ void fin_small(double* __restrict__ ovoo_bar, double* __restrict__ ovov_bar,
-                          double* __restrict__ t1_bar, double* __restrict__ fock_bar,
-                          double* __restrict__ mo_energy_bar, const double* __restrict__ ovoo_acc,
-                          const double* __restrict__ ovov_acc, const double* __restrict__ t1_acc,
+// ovoo_bar[p,x,m,q] = acc[p][q][x][m]; t1_bar, fock_bar and mo_energy_bar from
+// their accumulators (ovov_bar is accumulated in place)
+__global__ void fin_small(double* __restrict__ ovoo_bar, double* __restrict__ t1_bar,
+                          double* __restrict__ fock_bar, double* __restrict__ mo_energy_bar,
+                          const double* __restrict__ ovoo_acc, const double* __restrict__ t1_acc,
                           const double* __restrict__ fvo_acc, const double* __restrict__ eps_bar, int o,
                           int v, int Op, int V) {
   const int n = o + v;
-  const long long n1 = (long long)o * v * o * o, n2 = (long long)o * v * o * v, n3 = (long long)o * v,
-                  n4 = (long long)n * n, n5 = n;
-  const long long total = n1 + n2 + n3 + n4 + n5;
+  const long long n1 = (long long)o * v * o * o, n3 = (long long)o * v, n4 = (long long)n * n, n5 = n;
+  const long long total = n1 + n3 + n4 + n5;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     long long r = e;
@@ -366,13 +366,6 @@ __global__ void fin_small(double* __restrict__ ovoo_bar, double* __restrict__ ov
       continue;
     }
     r -= n1;
-    if (r < n2) {
-      const int y = (int)(r % v), q = (int)((r / v) % o), x = (int)((r / ((long long)v * o)) % v);
-      const int p = (int)(r / ((long long)v * o * v));
-      ovov_bar[r] = ovov_acc[(((long long)p * o + q) * V + x) * V + y];
-      continue;
-    }
-    r -= n2;
     if (r < n3) {
       const int z = (int)(r % v), i = (int)(r / v);
       t1_bar[r] = t1_acc[(long long)i * V + z];
@@ -422,7 +415,7 @@ extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void
 
 extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int ntriple, const VpSlab* slabs,
                                       int nslab, const VpContrib* contribs, int o, int v, int Op, int V,
-                                      const double* t1, const double* fvoP, double* ovov_acc, double* t2h_acc,
+                                      const double* t1, const double* fvoP, double* ovov_bar, double* t2h_acc,
                                       double* t1_acc, double* fvo_acc, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
@@ -438,8 +431,8 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
   hipLaunchKernelGGL(vpart_vecsum, dim3((unsigned)((V + 63) / 64), 6 * ntriple), dim3(256), 0, s, trs, V);
   const long long nx = VV > (long long)o * V ? VV : (long long)o * V;
   dim3 g2((unsigned)((nx + 255) / 256), nslab + 1);
-  hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, Op, V,
-                     ovov_acc, t2h_acc, t1_acc, fvo_acc);
+  hipLaunchKernelGGL(vpart_scatter, g2, dim3(256), 0, s, slabs, nslab, contribs, trs, ntriple, o, v, Op, V,
+                     ovov_bar, t2h_acc, t1_acc, fvo_acc);
   return hipGetLastError();
 }
 
@@ -453,20 +446,18 @@ extern "C" hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int
 }
 
 extern "C" hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar,
-                                  double* ovoo_bar, double* ovov_bar, double* ovvv_bar, int packed,
-                                  const double* ovvv_acc, const double* t2acc, const double* t2hacc,
-                                  const double* ovoo_acc, const double* ovov_acc,
-                                  const double* t1_acc, const double* fvo_acc, const double* eps_bar, int o,
-                                  int v, int Op, int V, hipStream_t s) {
+                                  double* ovoo_bar, double* ovvv_bar, int packed, const double* ovvv_acc,
+                                  const double* t2hacc, const double* ovoo_acc, const double* t1_acc,
+                                  const double* fvo_acc, const double* eps_bar, int o, int v, int Op, int V,
+                                  hipStream_t s) {
   const long long npair = (long long)v * (v + 1) / 2;
   const long long n_ovvv = packed ? (long long)o * v * npair : (long long)o * v * v * v;
   if (ovvv_bar) hipLaunchKernelGGL(fin_ovvv, grid_for(n_ovvv), dim3(256), 0, s, ovvv_bar, ovvv_acc, o, v, V, packed);
-  hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2acc, t2hacc, o, v,
-                     Op, V);
+  hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2hacc, o, v, Op, V);
   const long long n = o + v;
-  const long long nsmall = (long long)o * v * o * o + (long long)o * v * o * v + (long long)o * v + n * n + n;
-  hipLaunchKernelGGL(fin_small, grid_for(nsmall), dim3(256), 0, s, ovoo_bar, ovov_bar, t1_bar, fock_bar,
-                     mo_energy_bar, ovoo_acc, ovov_acc, t1_acc, fvo_acc, eps_bar, o, v, Op, V);
+  const long long nsmall = (long long)o * v * o * o + (long long)o * v + n * n + n;
+  hipLaunchKernelGGL(fin_small, grid_for(nsmall), dim3(256), 0, s, ovoo_bar, t1_bar, fock_bar, mo_energy_bar,
+                     ovoo_acc, t1_acc, fvo_acc, eps_bar, o, v, Op, V);
   return hipGetLastError();
 }
 
@@ -629,6 +620,12 @@ __global__ void zero_fill(uint4* __restrict__ p, long long n16) {
     p[e] = uint4{0u, 0u, 0u, 0u};
 }
 
+__global__ void zero_f64(double* __restrict__ p, long long n) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    p[e] = 0.0;
+}
+
 __global__ void copy_f64(double* __restrict__ dst, const double* __restrict__ src, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) dst[e] = src[e];
@@ -639,6 +636,13 @@ extern "C" hipError_t t3_zero(void* p, size_t bytes, hipStream_t s) {
   const long long n16 = (long long)(bytes / 16);
   if (n16 <= 0) return hipSuccess;
   hipLaunchKernelGGL(zero_fill, grid_for(n16), dim3(256), 0, s, reinterpret_cast<uint4*>(p), n16);
+  return hipGetLastError();
+}
+
+// any 8-byte aligned array of n doubles (caller-owned outputs)
+extern "C" hipError_t t3_zero_f64(double* p, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_f64, grid_for(n), dim3(256), 0, s, p, n);
   return hipGetLastError();
 }
 
