@@ -51,6 +51,12 @@ def parse():
     ap.add_argument("--batched", action="store_true",
                     help="the batched workload instead: 64 instances, nocc in [4,24], nvir in [40,200] "
                          "(seed 4), as one captured hipGraph, against 64 sequential FP64 calls")
+    ap.add_argument("--df", action="store_true",
+                    help="density-fitted inputs (Lov, Lvv of the same generator) in place of ovvv: the "
+                         "o*v^3 slabs are built on the device block by block, which fits the o=80/v=640 "
+                         "backward in HBM; --triples then counts in the blocked order ('auto': the first "
+                         "block triple of three distinct blocks)")
+    ap.add_argument("--df-block", type=int, default=4, help="occupied indices per block in --df mode")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of this many MiB (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -117,6 +123,85 @@ def slice_flops(nocc, nvir, t0, t1):
     return total
 
 
+def df_schedule(nocc, blk):
+    """The density-fitted plan's block triples in order, each with its unique
+    triples and the number of blocks the slab cache builds before it (the
+    library's policy: three block slots, a missing block takes the first slot
+    holding no block the block triple needs)."""
+    nb = (nocc + blk - 1) // blk
+    slots = [-1] * min(3, nb)
+    out = []
+    for I in range(nb):
+        for J in range(I + 1):
+            for K in range(J + 1):
+                need, builds = (I, J, K), 0
+                for b in need:
+                    if b in slots:
+                        continue
+                    u = next(u for u in range(len(slots)) if slots[u] not in need)
+                    slots[u] = b
+                    builds += 1
+                rng = lambda B: range(B * blk, min(nocc, (B + 1) * blk))
+                trip = [(i, j, k) for i in rng(I) for j in rng(J) if j <= i for k in rng(K) if k <= j]
+                out.append(((I, J, K), trip, builds))
+    return out
+
+
+def df_slice_info(nocc, nvir, blk, t0, t1):
+    """(F_E of the triples [t0, t1) of the blocked order, slab blocks built for
+    that slice from a cold cache, slab blocks built by the full plan)."""
+    per = 2.0 * nvir ** 3 * (nvir + nocc)
+    fe, idx, full_builds, slots = 0.0, 0, 0, None
+    nb = (nocc + blk - 1) // blk
+    cache = [-1] * min(3, nb)
+    slice_builds = 0
+    for (I, J, K), trip, builds in df_schedule(nocc, blk):
+        full_builds += builds
+        lo, hi = max(idx, t0), min(idx + len(trip), t1)
+        if lo < hi:
+            need = (I, J, K)
+            for b in need:
+                if b in cache:
+                    continue
+                u = next(u for u in range(len(cache)) if cache[u] not in need)
+                cache[u] = b
+                slice_builds += 1
+            for (i, j, k) in trip[lo - idx:hi - idx]:
+                fe += per * (1 if i == k else (3 if (i == j or j == k) else 6))
+        idx += len(trip)
+    return fe, slice_builds, full_builds
+
+
+def df_auto_slice(nocc, blk):
+    """[t0, t1) of the first block triple with three distinct blocks."""
+    idx = 0
+    for (I, J, K), trip, _ in df_schedule(nocc, blk):
+        if I > J > K:
+            return idx, idx + len(trip)
+        idx += len(trip)
+    return 0, idx
+
+
+def max_rel_chunked(a, b):
+    """max_rel of a (host or device) against device b, moving a to the device
+    a slice along the first axis at a time."""
+    import torch
+    scale = torch.max(torch.abs(b)).item()
+    if scale == 0:
+        return (0.0, 0.0)
+    dmax, elem = 0.0, 0.0
+    step = max(1, (1 << 27) // max(1, b[0].numel()))
+    for x in range(0, b.shape[0], step):
+        ac = a[x:x + step].to(b.device)
+        bc = b[x:x + step]
+        d = torch.abs(ac - bc)
+        dmax = max(dmax, torch.max(d).item())
+        mask = torch.abs(bc) > 1e-12 * scale
+        if bool(mask.any()):
+            elem = max(elem, torch.max(d[mask] / torch.abs(bc[mask])).item())
+    return (dmax / scale, elem)
+
+
 def max_rel(a, b):
     """(max|a-b| / max|b|, max over entries with |b| > 1e-12 max|b| of |a-b|/|b|)."""
     import torch
@@ -144,20 +229,32 @@ def main():
     from pyscfad_amd.synthetic import make_instance, flops_energy
 
     o, v = args.nocc, args.nvir
-    log(f"[bench] building synthetic instance nocc={o} nvir={v} seed={args.seed}")
-    inst = make_instance(o, v, args.seed, device="cuda")
+    if args.df:
+        # the block size fixes the blocked triple order: every run of this
+        # process (FP32 and the FP64 reference) uses the same one
+        os.environ["PYSCFAD_AMD_DF_BLOCK"] = str(args.df_block)
+    log(f"[bench] building synthetic instance nocc={o} nvir={v} seed={args.seed}" + (" (DF factors)" if args.df else ""))
+    inst = make_instance(o, v, args.seed, device="cuda", df=args.df)
     torch.cuda.synchronize()
     torch.cuda.empty_cache()   # return the generator's temporaries before the workspace is allocated
     torch.cuda.reset_peak_memory_stats()   # peak_hbm_GB: resident inputs + workspace + outputs
-    keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")
+    keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov") + (("Lov", "Lvv") if args.df else ("ovvv",))
     a = [inst[k] for k in keys]
+    naux = inst["Lov"].shape[0] if args.df else 0
     ntrip = ccsd_t.num_triples(o)
-    if args.triples:
+    if args.triples == "auto" and args.df:
+        t0, t1 = df_auto_slice(o, args.df_block)
+    elif args.triples:
         t0, t1 = (int(x) for x in args.triples.split(":"))
     else:
         t0, t1 = 0, ntrip
     sl = (t0, t1)
-    fe = flops_energy(o, v) if (t0, t1) == (0, ntrip) else slice_flops(o, v, t0, t1)
+    df_builds = None
+    if args.df:
+        fe, sb, fb = df_slice_info(o, v, args.df_block, t0, t1)
+        df_builds = (sb, fb)
+    else:
+        fe = flops_energy(o, v) if (t0, t1) == (0, ntrip) else slice_flops(o, v, t0, t1)
     if args.calibrate_copy:
         src = torch.empty(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -173,18 +270,22 @@ def main():
         if prec not in wsp:
             wsp.clear()
             torch.cuda.empty_cache()
-            nbytes = ccsd_t.workspace_bytes(o, v, prec, with_vjp=not args.energy_only)
+            nbytes = (ccsd_t.df_workspace_bytes(o, v, naux, prec, with_vjp=not args.energy_only) if args.df
+                      else ccsd_t.workspace_bytes(o, v, prec, with_vjp=not args.energy_only))
             wsp[prec] = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         return wsp[prec]
 
+    f_energy = ccsd_t.energy_df if args.df else ccsd_t.energy
+    f_vjp = ccsd_t.energy_vjp_df if args.df else ccsd_t.energy_vjp
+
     def step(prec, timing=False):
         _lib.set_timing(timing)
-        e = ccsd_t.energy(*a, precision=prec, triples=sl, workspace=workspace(prec))
+        e = f_energy(*a, precision=prec, triples=sl, workspace=workspace(prec))
         t_fwd = _lib.last_timing() if timing else None
         if args.energy_only:
             _lib.set_timing(False)
             return e, None, t_fwd, ({k: (0.0, 0) for k in t_fwd} if timing else None)
-        bars = ccsd_t.energy_vjp(*a, et_bar=1.0, precision=prec, triples=sl, workspace=workspace(prec))
+        bars = f_vjp(*a, et_bar=1.0, precision=prec, triples=sl, workspace=workspace(prec))
         t_bwd = _lib.last_timing() if timing else None
         _lib.set_timing(False)
         return e, bars, t_fwd, t_bwd
@@ -212,6 +313,7 @@ def main():
         e, bars, _, _ = step(args.precision)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - c0) / args.steps
+    peak_hbm = torch.cuda.max_memory_allocated() / 1e9   # before the FP64 reference step
     credit = 1.0 if args.energy_only else 3.0    # forward only: F_E; energy + VJP: 3 F_E
     tflops = credit * fe / dt / 1e12
     if gemm_ms is None:
@@ -232,11 +334,16 @@ def main():
     # accumulator clears, cotangent write-back, host planning); the full solve
     # is projected as that fixed part plus the per-triple kernels scaled by flops
     setup_ms = fam.get("prep", (0.0, 0))[0]
-    per_triple_s = sum(fam.get(k, (0.0, 0))[0] for k in ("gemm", "epilogue", "vpart", "reduce")) / 1e3
+    # density-fitted slab builds, folds and clears scale with the number of
+    # slab blocks built, not with the triples
+    df_s = fam.get("df", (0.0, 0))[0] / 1e3
+    per_triple_s = sum(fam.get(k, (0.0, 0))[0] for k in ("gemm", "epilogue", "vpart", "reduce")) / 1e3 - df_s
     full = flops_energy(o, v)
     projected_s = None
     if (t0, t1) != (0, ntrip):
-        projected_s = max(dt - per_triple_s, 0.0) + per_triple_s * full / fe
+        projected_s = max(dt - per_triple_s - df_s, 0.0) + per_triple_s * full / fe
+        if df_builds and df_builds[0] > 0:
+            projected_s += df_s * df_builds[1] / df_builds[0]
     projected_tflops = (credit * full / projected_s / 1e12) if projected_s else None
     contraction = {k: {"ms": fam[k][0], "launches": fam[k][1],
                        "tflops": (f / (fam[k][0] / 1e3) / 1e12) if fam[k][0] > 0 else None}
@@ -257,10 +364,10 @@ def main():
             torch.cuda.empty_cache()
         log("[bench] FP64 reference step for the accuracy report")
         e64, bars64, _, _ = step("f64")
-        names = ("t1", "t2", "fock", "mo_energy", "ovoo", "ovov", "ovvv")
+        names = ("t1", "t2", "fock", "mo_energy", "ovoo", "ovov") + (("Lov", "Lvv") if args.df else ("ovvv",))
         rel, elem = {}, {}
         for n, b, r in zip(names, bars, bars64):
-            rel[n], elem[n] = max_rel(b, r.cpu() if on_host else r)
+            rel[n], elem[n] = max_rel_chunked(b, r)
         accuracy = {"dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64),
                     "max_rel": rel, "max_rel_elementwise": elem}
 
@@ -280,7 +387,7 @@ def main():
         "ms_per_step": dt * 1e3,
         "s_per_solve": dt,
         "setup_ms": setup_ms,
-        "peak_hbm_GB": torch.cuda.max_memory_allocated() / 1e9,
+        "peak_hbm_GB": peak_hbm,
         "projected_full_solve_s": projected_s,
         "projected_full_solve_tflops": projected_tflops,
         "higher_is_better": True,
@@ -290,7 +397,9 @@ def main():
                   "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"single (T) instance nocc={o} nvir={v} seed={args.seed}"
-                               + ("" if (t0, t1) == (0, ntrip) else f", unique triples [{t0},{t1}) of {ntrip}"),
+                               + (f", density-fitted (naux={naux}, occupied blocks of {args.df_block})" if args.df else "")
+                               + ("" if (t0, t1) == (0, ntrip) else
+                                  f", unique triples [{t0},{t1}) of {ntrip}" + (" (blocked order)" if args.df else "")),
                    "precision": args.precision, "E_T": float(e)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic, "traffic_unit": "bytes per launch",
@@ -301,6 +410,8 @@ def main():
                      "family_ms": {k: val[0] for k, val in fam.items() if not k.startswith("gemm_") and k != "prep"},
                      "contractions": contraction},
         "cpu_baseline": cpu,
+        "df_slab_blocks": ({"slice": df_builds[0], "full_solve": df_builds[1], "seconds_in_slice": df_s}
+                           if df_builds else None),
         "accuracy_vs_f64": accuracy,
     }
     print(json.dumps(out))
