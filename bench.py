@@ -186,11 +186,13 @@ def max_rel_chunked(a, b):
     """max_rel of a (host or device) against device b, moving a to the device
     a slice along the first axis at a time."""
     import torch
-    scale = torch.max(torch.abs(b)).item()
+    step = max(1, (1 << 27) // max(1, b[0].numel()))
+    scale = 0.0
+    for x in range(0, b.shape[0], step):
+        scale = max(scale, torch.max(torch.abs(b[x:x + step])).item())
     if scale == 0:
         return (0.0, 0.0)
     dmax, elem = 0.0, 0.0
-    step = max(1, (1 << 27) // max(1, b[0].numel()))
     for x in range(0, b.shape[0], step):
         ac = a[x:x + step].to(b.device)
         bc = b[x:x + step]

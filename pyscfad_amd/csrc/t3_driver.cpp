@@ -72,6 +72,11 @@ namespace {
 constexpr int PERM[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 constexpr int GEMM_TARGET_BLOCKS = 2048;
 constexpr int MAX_KSPLIT = 64;
+// FP32 / bf16x3 GEMMs accumulate a split's K in FP32 registers; splits are
+// also cut so that none sums more than ACC_CHUNKS K-chunks (2048 products),
+// up to MAX_KSPLIT_ACC splits, before the FP64 split-order reduction
+constexpr int ACC_CHUNKS = 64;
+constexpr int MAX_KSPLIT_ACC = 256;
 constexpr int DF_KSPLIT = 8;   // split-K of the Lov cotangent folds (K = V^2)
 
 thread_local std::string g_last_error;
@@ -165,6 +170,12 @@ Dims dims_for(int o, int v, int prec, int naux, bool backward) {
   Dims d = make_dims(o, v, prec);
   if (naux > 0) set_df(d, naux, backward);
   return d;
+}
+
+// splits a K = V^2 problem needs to keep each FP32 partial within ACC_CHUNKS
+int acc_ksplit(const Dims& d) {
+  const long long chunks = (long long)d.V * d.V / d.BK;
+  return d.prec == PREC_F64 ? 1 : (int)std::min<long long>(MAX_KSPLIT_ACC, (chunks + ACC_CHUNKS - 1) / ACC_CHUNKS);
 }
 
 // ----------------------------------------------------------------------------
@@ -270,7 +281,8 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
   r.gslot_bytes = backward ? (size_t)round_up((long long)(V * V * V * d.elt), 256) : 0;
   r.gslots = L.take(r.gslot_bytes * B * r.S);
   // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6 per triple
-  r.split_bytes = backward ? (size_t)6 * B * MAX_KSPLIT * V * std::max(V, Op) * d.elt + (size_t)6 * B * 256 : 0;
+  const size_t ks = (size_t)std::max(MAX_KSPLIT, acc_ksplit(d));
+  r.split_bytes = backward ? (size_t)6 * B * ks * V * std::max(V, Op) * d.elt + (size_t)6 * B * 256 : 0;
   r.split = L.take(r.split_bytes);
   r.dfsplit_bytes = (backward && d.naux > 0) ? (size_t)d.dfblk * DF_KSPLIT * V * d.NP * 8 + (size_t)d.dfblk * 256 : 0;
   r.dfsplit = L.take(r.dfsplit_bytes);
@@ -415,8 +427,11 @@ struct Ctx {
 
   // Flatten a GemmBuild into a launch.
   void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub,
-                 int max_ksplit = MAX_KSPLIT, size_t scratch_region = (size_t)-1) {
-    if (scratch_region == (size_t)-1) scratch_region = r.split;
+                 int max_ksplit = MAX_KSPLIT, size_t scratch_region = (size_t)-1, size_t scratch_bytes = 0) {
+    if (scratch_region == (size_t)-1) {
+      scratch_region = r.split;
+      scratch_bytes = r.split_bytes;
+    }
     const size_t elt = gb.prec == PREC_F64 ? 8 : 4;   // operand and accumulator bytes
     if (gb.empty()) return;
     if (cfg < 0) cfg = pick_cfg(gb);
@@ -441,7 +456,21 @@ struct Ctx {
     int ksplit = 1;
     if (allow_split) {
       ksplit = (int)std::max<long long>(1, (GEMM_TARGET_BLOCKS + tiles - 1) / tiles);
-      ksplit = std::min(ksplit, std::min(max_ksplit, std::max(1, min_chunks)));
+      ksplit = std::min(ksplit, max_ksplit);
+      if (gb.prec != PREC_F64) {   // bound the FP32 accumulation length (ACC_CHUNKS)
+        int max_chunks = 0;
+        for (const auto& P : gb.probs) max_chunks = std::max(max_chunks, P.nchunks);
+        ksplit = std::max(ksplit, std::min(MAX_KSPLIT_ACC, (max_chunks + ACC_CHUNKS - 1) / ACC_CHUNKS));
+      }
+      ksplit = std::min(ksplit, std::max(1, min_chunks));
+      // the partials of every problem must fit the scratch region (problems
+      // merge all the triples of a batch that share an output, so their K,
+      // and the split the accumulation bound asks for, vary)
+      size_t per_split = 0;
+      for (const auto& P : gb.probs) per_split += (size_t)P.nb * P.M * P.N * elt;
+      const size_t room = scratch_bytes > 256 * gb.probs.size() ? scratch_bytes - 256 * gb.probs.size() : 0;
+      const long long cap = per_split ? (long long)(room / per_split) : ksplit;
+      ksplit = (int)std::max<long long>(1, std::min<long long>(ksplit, cap));
     }
     // operand load modes; every base and stride must keep VEC-element vectors aligned
     const long long vec = t3_gemm_vec(gb.prec);
@@ -502,6 +531,7 @@ struct Ctx {
         relem += (long long)P.nb * P.M * P.N;
         red.push_back(R);
         scratch = (size_t)round_up((long long)(scratch + bytes), 256);
+        if (scratch > scratch_bytes && plan_error.empty()) plan_error = "split-K partials exceed their scratch region";
       } else {
         P.mode = mode;
         P.P = nullptr;
@@ -595,7 +625,7 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
         gb.add(q, sg);
       }
-      emit_gemm(gb, 0, GEMM_ACCUM, 1, true, FAM_GEMM_DF, DF_KSPLIT, r.dfsplit);
+      emit_gemm(gb, 0, GEMM_ACCUM, 1, true, FAM_GEMM_DF, DF_KSPLIT, r.dfsplit, r.dfsplit_bytes);
     }
     {
       GemmBuild gb(PREC_F64);

@@ -1,0 +1,97 @@
+// Host-side bounds check of the launch plan (test infrastructure, no GPU):
+// builds the plan of an (nocc, nvir, precision[, naux]) backward call with
+// the library's own code (t3_driver.cpp is included) against a fake workspace
+// base, and checks, for every GEMM problem and segment, that the furthest
+// element its loads can reach (rows clamped as the kernel clamps them, whole
+// row vectors for row-contiguous operands) and the furthest element its
+// output or split partials can reach lie inside the workspace region or the
+// caller's in-place outputs; likewise every split-K reduction.
+// Usage: plan_check nocc nvir precision [naux [plant]]
+//   plant: moves the split-K scratch to the end of the workspace, so that a
+//   correct checker must report out-of-bounds partials.
+#include DRIVER
+#include <cstdio>
+static const uintptr_t OUT_T2 = 1ull << 44, OUT_OVOV = 2ull << 44;
+int main(int argc, char** argv) {
+  int o = atoi(argv[1]), v = atoi(argv[2]), prec = atoi(argv[3]);
+  int naux = argc > 4 ? atoi(argv[4]) : 0;
+  Ctx c;
+  c.d = dims_for(o, v, prec, naux, true);
+  c.B = choose_batch(c.d);
+  c.backward = true;
+  c.r = plan_regions(c.d, c.B, true);
+  const bool plant = argc > 5 && std::string(argv[5]) == "plant";
+  if (plant) c.r.split = c.r.desc - 4096;
+  c.t2_out = reinterpret_cast<double*>(OUT_T2);
+  c.ovov_out = reinterpret_cast<double*>(OUT_OVOV);
+  c.ws = reinterpret_cast<char*>(1ull << 40);   // fake workspace base (never dereferenced)
+  c.build_plan();
+  const uintptr_t WS = 1ull << 40;
+  const size_t ws_size = c.r.desc;   // regions end where descriptors start
+  const size_t t2_size = (size_t)o * o * v * v * 8;
+  int bad = 0, nl = 0;
+  auto check = [&](uintptr_t lo, uintptr_t hi, const char* what, size_t li, int p) {
+    bool ok = (lo >= WS && hi <= WS + ws_size) || (lo >= OUT_T2 && hi <= OUT_T2 + t2_size) ||
+              (lo >= OUT_OVOV && hi <= OUT_OVOV + t2_size);
+    if (!ok && bad++ < 20)
+      printf("OOB launch %zu prob %d %s: [%#lx, %#lx) ws [%#lx, %#lx)\n", li, p, what, lo, hi, WS, WS + ws_size);
+  };
+  for (size_t li = 0; li < c.launches.size(); li++) {
+    const Launch& L = c.launches[li];
+    if (L.kind == L_REDUCE) {
+      const int pr = L.prec >= 0 ? L.prec : c.d.prec;
+      const size_t e = pr == PREC_F64 ? 8 : 4;
+      const ReduceProb* R = reinterpret_cast<const ReduceProb*>(c.blob.data.data() + L.off_a);
+      long long total = 0;
+      for (int p = 0; p < L.n_a; p++) {
+        const ReduceProb& Q = R[p];
+        if (Q.elem0 != total) { printf("reduce %zu prob %d: elem0 %lld != %lld\n", li, p, Q.elem0, total); bad++; }
+        total += (long long)Q.nb * Q.M * Q.N;
+        check((uintptr_t)Q.P, (uintptr_t)Q.P + (size_t)Q.ksplit * Q.nb * Q.M * Q.N * e, "reduce P", li, p);
+        const size_t ce = Q.c_is_f64 ? 8 : 4;
+        const long long cmax = (long long)(Q.nb - 1) * Q.c_sb + (long long)(Q.Mv - 1) * Q.c_sm + (long long)(Q.Nv - 1) * Q.c_sn;
+        check((uintptr_t)Q.C, (uintptr_t)Q.C + (cmax + 1) * ce, "reduce C", li, p);
+      }
+      if (total != L.work) { printf("reduce %zu: %lld elements != work %lld\n", li, total, L.work); bad++; }
+      continue;
+    }
+    if (L.kind != L_GEMM) continue;
+    nl++;
+    const int pr = L.prec >= 0 ? L.prec : c.d.prec;
+    const size_t e = pr == PREC_F64 ? 8 : 4;
+    const int vec = t3_gemm_vec(pr);
+    const int BM = t3_gemm_tile_m(L.cfg), BN = t3_gemm_tile_n(L.cfg);
+    const GemmProb* P = reinterpret_cast<const GemmProb*>(c.blob.data.data() + L.off_a);
+    const GemmSeg* S = reinterpret_cast<const GemmSeg*>(c.blob.data.data() + L.off_b);
+    long long blocks = 0;
+    for (int p = 0; p < L.n_a; p++) {
+      const GemmProb& Q = P[p];
+      blocks += (long long)Q.nb * Q.tiles_m * Q.tiles_n * Q.ksplit;
+      for (int si = Q.seg0; si < Q.seg0 + Q.nseg; si++) {
+        const GemmSeg& g = S[si];
+        // rows: loads clamp to M-1 (KCONT) or to the last full vector block (ROWCONT)
+        const long long mrow = g.a_mode == LOAD_ROWCONT ? ((Q.M - 1) / vec) * vec + vec - 1 : Q.M - 1;
+        const long long nrow = g.b_mode == LOAD_ROWCONT ? ((Q.N - 1) / vec) * vec + vec - 1 : Q.N - 1;
+        const long long amax = (long long)(Q.nb - 1) * g.a_sb + mrow * g.a_sm + (long long)(g.K1 - 1) * g.a_sk1 +
+                               (long long)(g.K2 - 1) * g.a_sk2;
+        const long long bmax = (long long)(Q.nb - 1) * g.b_sb + nrow * g.b_sn + (long long)(g.K1 - 1) * g.b_sk1 +
+                               (long long)(g.K2 - 1) * g.b_sk2;
+        check((uintptr_t)g.A, (uintptr_t)g.A + (amax + 1) * e, "A", li, p);
+        check((uintptr_t)g.B, (uintptr_t)g.B + (bmax + 1) * e, "B", li, p);
+        if (g.a_sb < 0 || g.a_sm < 0 || g.b_sb < 0 || g.b_sn < 0) { printf("negative stride\n"); bad++; }
+      }
+      if (Q.mode == GEMM_PARTIAL) {
+        const size_t bytes = (size_t)Q.ksplit * Q.nb * Q.M * Q.N * e;
+        check((uintptr_t)Q.P, (uintptr_t)Q.P + bytes, "P", li, p);
+      } else {
+        const size_t ce = Q.c_is_f64 ? 8 : 4;
+        const long long cmax = (long long)(Q.nb - 1) * Q.c_sb + (long long)(Q.Mv - 1) * Q.c_sm + (long long)(Q.Nv - 1) * Q.c_sn;
+        check((uintptr_t)Q.C, (uintptr_t)Q.C + (cmax + 1) * ce, "C", li, p);
+      }
+    }
+    if (blocks != L.work) { printf("launch %zu: block count %lld != work %lld\n", li, blocks, L.work); bad++; }
+  }
+  printf("o=%d v=%d prec=%d naux=%d B=%d launches=%d gemm, bad=%d, plan_error='%s'\n", o, v, prec, naux, c.B, nl, bad,
+         c.plan_error.c_str());
+  return bad ? 1 : 0;
+}
