@@ -152,7 +152,7 @@ Dims make_dims(int o, int v, int prec) {
 // cotangents are folded into the DF factors' cotangents when it is evicted.
 // The slab cache (and, backward, its FP64 accumulators) is held to about
 // 40 GB; PYSCFAD_AMD_DF_BLOCK (1..nocc) overrides the block size.
-void set_df(Dims& d, int naux, bool backward) {
+void set_df(Dims& d, int naux, bool backward, int blk_override = 0) {
   d.naux = naux;
   d.NP = (int)round_up(naux, 32);
   const double slab = (double)d.V * d.V * d.V * (double)(d.elt + (backward ? 8 : 0));
@@ -161,14 +161,15 @@ void set_df(Dims& d, int naux, bool backward) {
     const long long x = std::atoll(env);
     if (x >= 1) blk = x;
   }
+  if (blk_override > 0) blk = blk_override;
   d.dfblk = (int)std::max<long long>(1, std::min<long long>(blk, d.o));
   const int nblk = (d.o + d.dfblk - 1) / d.dfblk;
   d.dfslots = std::min(3, nblk);
 }
 
-Dims dims_for(int o, int v, int prec, int naux, bool backward) {
+Dims dims_for(int o, int v, int prec, int naux, bool backward, int dfblk = 0) {
   Dims d = make_dims(o, v, prec);
-  if (naux > 0) set_df(d, naux, backward);
+  if (naux > 0) set_df(d, naux, backward, dfblk);
   return d;
 }
 
@@ -425,16 +426,53 @@ struct Ctx {
     return 1;
   }
 
-  // Flatten a GemmBuild into a launch.
+  // Flatten a GemmBuild into launches.
   void emit_gemm(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub,
                  int max_ksplit = MAX_KSPLIT, size_t scratch_region = (size_t)-1, size_t scratch_bytes = 0) {
+    if (gb.empty()) return;
+    if (cfg < 0) cfg = pick_cfg(gb);
+    // Unsplit FP32 / bf16x3 adjoint launches: a problem that merges many
+    // segments (all the triples of a batch or group that write one output)
+    // would sum all of them in FP32 registers.  Whole segments are packed into
+    // successive launches of at most ACC_CHUNKS K-chunks per problem; the
+    // later launches accumulate into the same output in stream order.  (A W
+    // build problem is one triple's cube: at most 4 particle and 4 hole
+    // segments per pass.)
+    if (!allow_split && gb.prec != PREC_F64 && sub != FAM_GEMM_W) {
+      std::vector<std::vector<std::vector<GemmSeg>>> passes;   // [pass][problem] -> segments
+      for (size_t p = 0; p < gb.probs.size(); p++) {
+        int pass = 0, chunks = 0;
+        for (const auto& sg : gb.segs[p]) {
+          const int c = (int)((long long)sg.K1 * sg.K2 / gb.BK);
+          if (chunks > 0 && chunks + c > ACC_CHUNKS) { pass++; chunks = 0; }
+          if ((int)passes.size() <= pass) passes.resize(pass + 1, std::vector<std::vector<GemmSeg>>(gb.probs.size()));
+          passes[pass][p].push_back(sg);
+          chunks += c;
+        }
+      }
+      if (passes.size() > 1) {
+        for (size_t k = 0; k < passes.size(); k++) {
+          GemmBuild part(gb.prec);
+          for (size_t p = 0; p < gb.probs.size(); p++) {
+            if (passes[k][p].empty()) continue;
+            const int q = part.problem((long long)p, gb.probs[p]);
+            for (const auto& sg : passes[k][p]) part.add(q, sg);
+          }
+          emit_gemm_launch(part, cfg, k == 0 ? mode : GEMM_ACCUM, c_is_f64, false, sub);
+        }
+        return;
+      }
+    }
+    emit_gemm_launch(gb, cfg, mode, c_is_f64, allow_split, sub, max_ksplit, scratch_region, scratch_bytes);
+  }
+
+  void emit_gemm_launch(GemmBuild& gb, int cfg, int mode, int c_is_f64, bool allow_split, int sub,
+                        int max_ksplit = MAX_KSPLIT, size_t scratch_region = (size_t)-1, size_t scratch_bytes = 0) {
     if (scratch_region == (size_t)-1) {
       scratch_region = r.split;
       scratch_bytes = r.split_bytes;
     }
     const size_t elt = gb.prec == PREC_F64 ? 8 : 4;   // operand and accumulator bytes
-    if (gb.empty()) return;
-    if (cfg < 0) cfg = pick_cfg(gb);
     const int BM = t3_gemm_tile_m(cfg), BN = t3_gemm_tile_n(cfg);
     long long tiles = 0;
     int min_chunks = 1 << 30;
@@ -880,7 +918,8 @@ struct Ctx {
         sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, cfg_for(FAM_GEMM_B3, d.Op <= 32 ? 3 : 0), GEMM_ACCUM, 1, true, FAM_GEMM_B3);
+    // default: 32-wide tiles on the occupied side unless it is a multiple of 64
+    emit_gemm(gb, cfg_for(FAM_GEMM_B3, d.Op % 64 ? 3 : 0), GEMM_ACCUM, 1, true, FAM_GEMM_B3);
   }
 
   void emit_b4(const std::vector<Triple>& bt) {
@@ -919,7 +958,7 @@ struct Ctx {
         sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
         gb.add(q, sg);
       }
-    emit_gemm(gb, cfg_for(FAM_GEMM_B4, d.Op <= 32 ? 4 : 0), GEMM_ACCUM, 1, false, FAM_GEMM_B4);
+    emit_gemm(gb, cfg_for(FAM_GEMM_B4, d.Op % 64 ? 4 : 0), GEMM_ACCUM, 1, false, FAM_GEMM_B4);
   }
 
   void emit_vpart(const std::vector<Triple>& bt) {
@@ -1114,9 +1153,9 @@ int check_args(int nocc, int nvir, int precision) {
 
 // Size the descriptor blob by building the plan against a null workspace.
 size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B,
-                  long long t_begin = 0, long long t_end = -1, int batch = -1, int naux = 0) {
+                  long long t_begin = 0, long long t_end = -1, int batch = -1, int naux = 0, int dfblk = 0) {
   Ctx c;
-  c.d = dims_for(nocc, nvir, precision, naux, backward);
+  c.d = dims_for(nocc, nvir, precision, naux, backward, dfblk);
   c.B = batch > 0 ? batch : choose_batch(c.d);
   c.backward = backward;
   c.t_begin = t_begin;
@@ -1424,19 +1463,31 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   int B = 1;
   plan_bytes(nocc, nvir, precision, true, &r, &B);
   const long long t0 = ntrip / 2, t1 = std::min(ntrip, t0 + B);
-  // the backward plan when its workspace fits, else the forward plan (W build only)
+  // Trial plans, in order of preference: the dense backward plan; the
+  // density-fitted backward plan with one-index blocks (its contractions
+  // have the same shapes, and its slab cache holds 3 slabs instead of o, so
+  // it fits where the dense one does not; its slab builds are never run);
+  // the forward plan (W build only).
   bool bw = true;
+  Dims d = make_dims(nocc, nvir, precision);
   // the backward trial plans accumulate t2_bar in place: a scratch one after the workspace
   const size_t t2_bytes = (size_t)round_up((long long)nocc * nocc * nvir * nvir * 8, 256);
   size_t need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1) + t2_bytes;
   char* ws = nullptr;
   if (hipMalloc(&ws, need) != hipSuccess) {
     (void)hipGetLastError();
-    bw = false;
-    need = plan_bytes(nocc, nvir, precision, false, &r, &B, t0, t1);
-    if (hipMalloc(&ws, need) != hipSuccess) {
+    constexpr int TUNE_NAUX = 32;
+    need = plan_bytes(nocc, nvir, precision, true, &r, &B, t0, t1, -1, TUNE_NAUX, 1) + t2_bytes;
+    if (hipMalloc(&ws, need) == hipSuccess) {
+      d = dims_for(nocc, nvir, precision, TUNE_NAUX, true, 1);
+    } else {
       (void)hipGetLastError();
-      return fail(PYSCFAD_AMD_ERR_OOM, "autotune workspace of " + std::to_string(need) + " bytes");
+      bw = false;
+      need = plan_bytes(nocc, nvir, precision, false, &r, &B, t0, t1);
+      if (hipMalloc(&ws, need) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PYSCFAD_AMD_ERR_OOM, "autotune workspace of " + std::to_string(need) + " bytes");
+      }
     }
   }
   const int timing = g_timing;   // trial launches are timed here, not per family
@@ -1447,7 +1498,6 @@ int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st) {
   if (e == hipSuccess) e = t3_zero(ws, need / 16 * 16, st);
   // placeholder input pointers: the timed launches never read the inputs
   static const double dummy[4] = {0, 0, 0, 0};
-  const Dims d = make_dims(nocc, nvir, precision);
   for (int f = 0; f < 5 && e == hipSuccess; f++) {
     const int fam = TUNE_FAMILIES[f];
     if (!bw && fam != FAM_GEMM_W) {   // untuned: the defaults stay

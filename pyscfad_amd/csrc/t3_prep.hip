@@ -123,7 +123,10 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
     const int u = (int)(ec / V), w = (int)(ec % V);
     const int td = occ[pass];
     double s = 0.0, f = 0.0, pm = 0.0, pn = 0.0;
+    // unrolled so that eight H loads per wave are in flight (the sums keep
+    // their order: x ascending within the wave)
     if (pass == 0) {        // e = (b, c)
+#pragma unroll 8
       for (int x = sp; x < v; x += 4) {
         const double hv = (double)H[(long long)x * VV + ec];
         s += hv * t1[(long long)td * v + x];
@@ -132,6 +135,7 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
     } else {                // e = (a, c)
       const double* CM2 = out + (VP_CM + 4) * VV + (long long)u * V;
       const double* CN2 = out + (VP_CM + 5) * VV + (long long)u * V;
+#pragma unroll 8
       for (int x = sp; x < v; x += 4) {
         const double hv = (double)H[(long long)u * VV + (long long)x * V + w];
         s += hv * t1[(long long)td * v + x];

@@ -49,6 +49,7 @@ CASES = [
     (16, 130, 1, 0, None), (6, 72, 1, 0, None), (2, 3, 1, 0, None), (9, 1, 0, 0, None), (1, 9, 1, 0, None),
     (32, 256, 1, 0, None), (32, 256, 0, 0, None), (64, 512, 1, 0, None),
     (7, 33, 1, 120, 2), (6, 24, 0, 90, 1), (9, 64, 1, 216, None), (80, 640, 1, 2160, 4),
+    (80, 640, 1, 32, 1),   # the autotuner's density-fitted trial plan where the dense one does not fit
 ]
 
 
