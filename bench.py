@@ -26,7 +26,7 @@ import os
 import sys
 import time
 
-MI355X_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2516.6 / 3.0}
+MI355X_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2516.6 / 3.0, "bf16x6": 2516.6 / 6.0}
 
 
 def parse():
@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--nocc", type=int, default=32)
     ap.add_argument("--nvir", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16x3", "bf16x6"])
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 accuracy reference run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
@@ -400,7 +400,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f64": "f64", "f32": "f32 (FP64 denominators and sums)",
-                  "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)"}[args.precision],
+                  "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)",
+                  "bf16x6": "bf16x6 (three-part bf16 split, FP32 accumulate, FP64 denominators and sums)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"single (T) instance nocc={o} nvir={v} seed={args.seed}"
                                + (f", density-fitted (naux={naux}, occupied blocks of {args.df_block})" if args.df else "")

@@ -36,6 +36,8 @@ extern "C" {
 #define PYSCFAD_AMD_PREC_F64 0        /* FP64 MFMA throughout: the drop-in path */
 #define PYSCFAD_AMD_PREC_F32 1        /* FP32 MFMA contractions */
 #define PYSCFAD_AMD_PREC_BF16X3 2     /* split-bf16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 accumulate */
+#define PYSCFAD_AMD_PREC_BF16X6 3     /* three-part split-bf16, six products, FP32 accumulate: FP32-level
+                                         accuracy from the bf16 matrix cores */
 
 /*
  * E(T) of one (t1, t2, eris) instance, written to e_out (DEVICE pointer to one

@@ -11,7 +11,8 @@ LIB_PATH = os.path.join(HERE, "libpyscfad_amd.so")
 PREC_F64 = 0
 PREC_F32 = 1
 PREC_BF16X3 = 2
-PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
+PREC_BF16X6 = 3
+PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3, "bf16x6": PREC_BF16X6}
 
 # every symbol include/pyscfad_amd.h declares
 EXPORTS = (

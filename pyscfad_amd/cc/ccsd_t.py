@@ -21,8 +21,9 @@ reference:
 Both passes run on the GPU through the C ABI of ``libpyscfad_amd.so``;
 arrays may be NumPy (copied to the device) or torch tensors (used in place
 when already contiguous FP64 on the device).  ``precision`` selects the
-arithmetic of the W/V contractions: 'f64' (default, the drop-in), 'f32' or
-'bf16x3'; denominators, the E(T) sum and cotangent accumulation stay FP64.
+arithmetic of the W/V contractions: 'f64' (default, the drop-in), 'f32',
+'bf16x3', or 'bf16x6' (FP32-level accuracy from three-part bf16 splits);
+denominators, the E(T) sum and cotangent accumulation stay FP64.
 """
 import numpy
 import torch

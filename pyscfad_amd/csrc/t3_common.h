@@ -7,7 +7,7 @@
 namespace t3 {
 
 // Arithmetic of the W/V contractions.
-enum Prec : int { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
+enum Prec : int { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
 
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef double d4v __attribute__((ext_vector_type(4)));

@@ -1146,8 +1146,8 @@ struct Ctx {
 int check_args(int nocc, int nvir, int precision) {
   if (nocc < 1 || nvir < 1) return fail(PYSCFAD_AMD_ERR_ARGS, "nocc and nvir must be >= 1");
   if (nocc > 4095 || nvir > 4095) return fail(PYSCFAD_AMD_ERR_ARGS, "nocc and nvir must be < 4096");
-  if (precision < PYSCFAD_AMD_PREC_F64 || precision > PYSCFAD_AMD_PREC_BF16X3)
-    return fail(PYSCFAD_AMD_ERR_PRECISION, "precision must be 0 (F64), 1 (F32) or 2 (BF16X3)");
+  if (precision < PYSCFAD_AMD_PREC_F64 || precision > PYSCFAD_AMD_PREC_BF16X6)
+    return fail(PYSCFAD_AMD_ERR_PRECISION, "precision must be 0 (F64), 1 (F32), 2 (BF16X3) or 3 (BF16X6)");
   return PYSCFAD_AMD_OK;
 }
 

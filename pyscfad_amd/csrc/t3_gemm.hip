@@ -5,6 +5,11 @@
 //                 hi, lo in bf16; three v_mfma_f32_16x16x32_bf16 per K-step
 //                 (hi*hi, lo*hi, hi*lo), FP32 accumulation (Henry, Tang and
 //                 Heinecke, ARITH 2019, "bf16x3").
+//   PREC_BF16X6 : the same with a three-way split x = hi + mid + lo (24
+//                 significant bits) and six products per K-step (hi*hi,
+//                 hi*mid, mid*hi, hi*lo, lo*hi, mid*mid; the dropped terms are
+//                 below 2^-32 relative), FP32 accumulation: FP32-level accuracy
+//                 from bf16 matrix cores (Henry, Tang and Heinecke, "bf16x6").
 //
 // Every contraction of the (T) energy and its VJP is expressed as a list of
 // problems C[b][m][n] (op)= alpha * sum_seg scale_seg * sum_k A_seg[b][m][k] B_seg[b][k][n]
@@ -34,6 +39,7 @@ template <int PREC> struct PrecTraits;
 template <> struct PrecTraits<PREC_F64> { using T = double; using Acc = double; using V = d2v; static constexpr int BK = 16; static constexpr int VEC = 2; };
 template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 template <> struct PrecTraits<PREC_BF16X3> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
+template <> struct PrecTraits<PREC_BF16X6> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
@@ -181,6 +187,79 @@ struct Stage {
       }
     }
   }
+
+  // LDS [row][k] as three bf16 parts x = hi + mid + lo (FP32 operands only).
+  // Each remainder is exact in FP32; per pair of values: three packed RNE
+  // conversions, four shifts / masks, four subtractions.
+  template <int LDK>
+  __device__ __forceinline__ void store_split3(__bf16* __restrict__ hi, __bf16* __restrict__ mid,
+                                               __bf16* __restrict__ lo) const {
+    if (neg) store_split3_as<LDK, true>(hi, mid, lo);
+    else store_split3_as<LDK, false>(hi, mid, lo);
+  }
+  template <int LDK, bool NEG>
+  __device__ __forceinline__ void store_split3_as(__bf16* __restrict__ hi, __bf16* __restrict__ mid,
+                                                  __bf16* __restrict__ lo) const {
+    const int t = threadIdx.x;
+    auto part = [](float& x0, float& x1) {   // bf16 pair of (x0, x1); x0, x1 <- the remainders
+      const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{x0, x1}, bf2v));
+      x0 -= __uint_as_float(h << 16);
+      x1 -= __uint_as_float(h & 0xffff0000u);
+      return h;
+    };
+    auto split2 = [&](float x0, float x1, uint32_t& h, uint32_t& m, uint32_t& l) {
+      if constexpr (NEG) { x0 = -x0; x1 = -x1; }
+      h = part(x0, x1);
+      m = part(x0, x1);
+      l = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{x0, x1}, bf2v));
+    };
+    auto split4 = [&](f4v x, u2v& h, u2v& m, u2v& l) {
+      uint32_t h0, m0, l0, h1, m1, l1;
+      split2(x[0], x[1], h0, m0, l0);
+      split2(x[2], x[3], h1, m1, l1);
+      h = u2v{h0, h1};
+      m = u2v{m0, m1};
+      l = u2v{l0, l1};
+    };
+    if (mode == LOAD_ROWCONT) {
+      const int rb = t % RB, kb = t / RB;
+#pragma unroll
+      for (int e = 0; e < VEC; e++) {
+        const int off = (rb * VEC + e) * LDK + kb * NV;
+        if constexpr (NV == 4) {
+          u2v h, m, l;
+          split4(f4v{r[0][e], r[1][e], r[2][e], r[3][e]}, h, m, l);
+          *reinterpret_cast<u2v*>(hi + off) = h;
+          *reinterpret_cast<u2v*>(mid + off) = m;
+          *reinterpret_cast<u2v*>(lo + off) = l;
+        } else if constexpr (NV == 2) {
+          uint32_t h, m, l;
+          split2(r[0][e], r[1][e], h, m, l);
+          *reinterpret_cast<uint32_t*>(hi + off) = h;
+          *reinterpret_cast<uint32_t*>(mid + off) = m;
+          *reinterpret_cast<uint32_t*>(lo + off) = l;
+        } else {
+          static_assert(NV == 1, "split store handles 1, 2 or 4 k values per thread");
+          uint32_t h, m, l;
+          split2(r[0][e], 0.0f, h, m, l);
+          hi[off] = __builtin_bit_cast(__bf16, (uint16_t)h);
+          mid[off] = __builtin_bit_cast(__bf16, (uint16_t)m);
+          lo[off] = __builtin_bit_cast(__bf16, (uint16_t)l);
+        }
+      }
+    } else {
+      const int kv = t % KV, row0 = t / KV;
+#pragma unroll
+      for (int q = 0; q < NV; q++) {
+        const int row = row0 + q * RSTEP;
+        u2v h, m, l;
+        split4(r[q], h, m, l);
+        *reinterpret_cast<u2v*>(hi + row * LDK + kv * VEC) = h;
+        *reinterpret_cast<u2v*>(mid + row * LDK + kv * VEC) = m;
+        *reinterpret_cast<u2v*>(lo + row * LDK + kv * VEC) = l;
+      }
+    }
+  }
 };
 
 template <class Acc> struct AccFrag;
@@ -232,11 +311,15 @@ __device__ __forceinline__ void write_out(const GemmProb& P, const TileCoord& tc
 }
 
 // BM x BN tile, 4 waves arranged WGM x (4/WGM).  Register budget: four waves
-// per SIMD for tiles up to 64x64 (four blocks per CU fit in LDS), two above.
-template <int BM, int BN> struct MinBlocks { static constexpr int value = (BM * BN <= 64 * 64) ? 4 : 2; };
+// per SIMD for tiles up to 64x64 (four blocks per CU fit in LDS), two above;
+// bf16x6, whose three-part LDS tiles allow two blocks per CU at 64x64 and one
+// above, gets two and one.
+template <int PREC, int BM, int BN> struct MinBlocks {
+  static constexpr int value = PREC == PREC_BF16X6 ? ((BM * BN <= 64 * 64) ? 2 : 1) : ((BM * BN <= 64 * 64) ? 4 : 2);
+};
 
 template <int PREC, int BM, int BN, int WGM>
-__global__ void __launch_bounds__(256, (MinBlocks<BM, BN>::value))
+__global__ void __launch_bounds__(256, (MinBlocks<PREC, BM, BN>::value))
 gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __restrict__ segs) {
   using TR = PrecTraits<PREC>;
   using T = typename TR::T;
@@ -338,7 +421,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   };
   const int nch = c_end - c_begin;
 
-  if constexpr (PREC != PREC_BF16X3) {
+  if constexpr (PREC == PREC_F64 || PREC == PREC_F32) {
     // K is consumed in a permuted order: in step ks the lanes of group
     // g = lane/16 multiply k = g*KG + ks (the sum over k is order-free), so a
     // lane's KG values of a row are contiguous in LDS (two 16-byte reads).
@@ -391,6 +474,60 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       __syncthreads();
     };
     // straight-line pairs keep the register stages in place across the loop
+    int it = 0;
+    for (; it + 1 < nch; it += 2) {
+      step(it, std::integral_constant<int, 0>{});
+      step(it + 1, std::integral_constant<int, 1>{});
+    }
+    if (it < nch) step(it, std::integral_constant<int, 0>{});
+  } else if constexpr (PREC == PREC_BF16X6) {
+    constexpr int LDK = BK + 8;   // bf16 elements; 80-byte rows keep 16-byte reads aligned
+    __shared__ __attribute__((aligned(16))) __bf16 la[3][2][BM * LDK];   // [hi/mid/lo][buffer]
+    __shared__ __attribute__((aligned(16))) __bf16 lb[3][2][BN * LDK];
+    if (nch > 0) {
+      load_next(sa[0], sb[0]);
+      load_next(sa[1], sb[1]);
+      sa[0].template store_split3<LDK>(la[0][0], la[1][0], la[2][0]);
+      sb[0].template store_split3<LDK>(lb[0][0], lb[1][0], lb[2][0]);
+    }
+    __syncthreads();
+    const int ko = g * 8;
+    auto step = [&](int it, auto par) {
+      constexpr int p = decltype(par)::value;
+      load_next(sa[p], sb[p]);
+      bf8v ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
+#pragma unroll
+      for (int i = 0; i < FM; i++) {
+        const int o = (wm0 + i * 16 + l16) * LDK + ko;
+        ah[i] = *reinterpret_cast<const bf8v*>(&la[0][p][o]);
+        am[i] = *reinterpret_cast<const bf8v*>(&la[1][p][o]);
+        al[i] = *reinterpret_cast<const bf8v*>(&la[2][p][o]);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; j++) {
+        const int o = (wn0 + j * 16 + l16) * LDK + ko;
+        bh[j] = *reinterpret_cast<const bf8v*>(&lb[0][p][o]);
+        bm[j] = *reinterpret_cast<const bf8v*>(&lb[1][p][o]);
+        bl[j] = *reinterpret_cast<const bf8v*>(&lb[2][p][o]);
+      }
+      // smallest products first
+#pragma unroll
+      for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+          acc[i][j] = mfma_bf16(am[i], bm[j], acc[i][j]);
+          acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+          acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+          acc[i][j] = mfma_bf16(am[i], bh[j], acc[i][j]);
+          acc[i][j] = mfma_bf16(ah[i], bm[j], acc[i][j]);
+          acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+        }
+      if (it + 1 < nch) {
+        sa[p ^ 1].template store_split3<LDK>(la[0][p ^ 1], la[1][p ^ 1], la[2][p ^ 1]);
+        sb[p ^ 1].template store_split3<LDK>(lb[0][p ^ 1], lb[1][p ^ 1], lb[2][p ^ 1]);
+      }
+      __syncthreads();
+    };
     int it = 0;
     for (; it + 1 < nch; it += 2) {
       step(it, std::integral_constant<int, 0>{});
@@ -557,7 +694,8 @@ extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, i
   const dim3 grid((unsigned)nblocks);
   if (prec == PREC_F64) launch_cfg<PREC_F64>(cfg, grid, stream, probs, nprob, segs);
   else if (prec == PREC_F32) launch_cfg<PREC_F32>(cfg, grid, stream, probs, nprob, segs);
-  else launch_cfg<PREC_BF16X3>(cfg, grid, stream, probs, nprob, segs);
+  else if (prec == PREC_BF16X3) launch_cfg<PREC_BF16X3>(cfg, grid, stream, probs, nprob, segs);
+  else launch_cfg<PREC_BF16X6>(cfg, grid, stream, probs, nprob, segs);
   return hipGetLastError();
 }
 
