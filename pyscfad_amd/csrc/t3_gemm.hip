@@ -45,6 +45,54 @@ typedef float f2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
 typedef uint32_t u2v __attribute__((ext_vector_type(2)));
 
+// LDS tiles are [row][k] with a row pitch of LDK elements, split into 16-byte
+// units of UE elements; unit u of row r is stored at position u ^ s(r).  The
+// permutations s follow the gfx950 LDS bank structure (64 banks of 4 bytes):
+//  * ds_read_b128 serves 16 lanes per pass; with l16 = lane & 15 and
+//    g = lane >> 4, a pass takes groups g and g ^ 1 of a lane pair (0, 1) or
+//    (2, 3), choosing g where l16 is in 0-3 or 12-15 and g ^ 1 where l16 is in
+//    4-11, and is conflict-free when its 16 addresses fall in distinct 16-byte
+//    bank groups modulo 256 bytes;
+//  * ds_write_b128 serves 8 consecutive lanes per pass over 128 bytes of banks,
+//    ds_write_b64 16 consecutive lanes.
+// Fragment reads have lane (l16, g) read row l16 at unit 2g + u (FP32, FP64)
+// or g (bf16); k-contiguous stores write whole rows; row-contiguous stores
+// write one unit of rows 4 (FP32, bf16) or 2 (FP64) apart.
+//  * FP32: 128-byte rows, s = (b2, b1^b3, b3^b4) (bit j of s listed j-th, bk =
+//    bit k of r).  Reads and 16-byte stores are conflict-free; the 8-byte
+//    row-contiguous stores of 64-row tiles take two passes, their minimum.
+//  * FP64: 128-byte rows, s = (b1, b2, b2^b3): conflict-free reads and stores
+//    of rows 2 apart; s depends on row bits 0-3 only, so the fragment reads at
+//    16-row offsets share one address register (the 64x64 tile has none to
+//    spare).
+//  * bf16: 80-byte rows (4 units + 16 bytes of padding), s = (b2^b3, b4):
+//    conflict-free reads; the 8-byte stores take two passes.
+enum { SWZ_F32 = 0, SWZ_F64 = 1, SWZ_BF16 = 2 };
+template <int LDK, int UE, int KIND>
+__device__ __forceinline__ int lds_off(int row, int k) {
+  int s;
+  if constexpr (KIND == SWZ_F32) s = ((row >> 2) & 1) | (((row >> 1) ^ (row >> 3)) & 1) << 1 | (((row >> 3) ^ (row >> 4)) & 1) << 2;
+  else if constexpr (KIND == SWZ_F64) s = ((row >> 1) & 1) | ((row >> 2) & 1) << 1 | (((row >> 2) ^ (row >> 3)) & 1) << 2;
+  else s = (((row >> 2) ^ (row >> 3)) & 1) | ((row >> 4) & 1) << 1;
+  return row * LDK + (((k / UE) ^ s) * UE) + (k % UE);
+}
+
+// LDS offsets of a wave's fragment reads: fragment i covers rows row0 + 16 i
+// (row0 a multiple of 16).  The permutations use row bits below 5, so
+// fragments i and i + 2 sit a fixed 32-row stride apart (FP64: bits below 4,
+// a 16-row stride), and the reads of all fragments share one or two address
+// registers with immediate offsets.
+template <int LDK, int UE, int KIND>
+struct FragOff {
+  int o0, o1;
+  __device__ __forceinline__ FragOff(int row, int k)
+      : o0(lds_off<LDK, UE, KIND>(row, k)), o1(lds_off<LDK, UE, KIND>(row + 16, k)) {}
+  __device__ __forceinline__ int operator()(int i) const {
+    if constexpr (KIND == SWZ_F64) return o0 + i * 16 * LDK;
+    else return ((i & 1) ? o1 : o0) + (i >> 1) * 32 * LDK;
+  }
+};
+
 // Register stage of one operand tile (ROWS x BK elements, NE per thread).
 //   KCONT  : thread holds NV vectors, each VEC consecutive k of one row
 //            (thread t: k-vector t % (BK/VEC), rows t / (BK/VEC) + q * RSTEP)
@@ -57,6 +105,7 @@ struct Stage {
   static constexpr int KV = BK / VEC;         // k-vectors per row (KCONT)
   static constexpr int RSTEP = 256 / KV;      // rows per q step (KCONT)
   static constexpr int RB = ROWS / VEC;       // row blocks (ROWCONT)
+  static constexpr int SWZ = sizeof(T) == 8 ? SWZ_F64 : SWZ_F32;   // LDS unit permutation
   static_assert(NV >= 1 && NE % VEC == 0 && RB * (BK / NV) == 256, "tile must give each thread whole vectors");
   V r[NV];
   int mode;
@@ -102,16 +151,16 @@ struct Stage {
       const int rb = t % RB, kb = t / RB;
 #pragma unroll
       for (int e = 0; e < VEC; e++) {
-        T* dst = lds + (rb * VEC + e) * LDK + kb * NV;
-        if constexpr (NV == 2 || (NV == 4 && sizeof(T) == 4)) {
+        const int row = rb * VEC + e;
+        if constexpr (NV * sizeof(T) <= 16 && NV > 1) {   // within one 16-byte unit
           typedef T tnv __attribute__((ext_vector_type(NV)));
           tnv x;
 #pragma unroll
           for (int j = 0; j < NV; j++) x[j] = r[j][e];
-          *reinterpret_cast<tnv*>(dst) = x * s;
+          *reinterpret_cast<tnv*>(lds + lds_off<LDK, VEC, SWZ>(row, kb * NV)) = x * s;
         } else {
 #pragma unroll
-          for (int j = 0; j < NV; j++) dst[j] = r[j][e] * s;
+          for (int j = 0; j < NV; j++) lds[lds_off<LDK, VEC, SWZ>(row, kb * NV + j)] = r[j][e] * s;
         }
       }
     } else {
@@ -119,7 +168,7 @@ struct Stage {
 #pragma unroll
       for (int q = 0; q < NV; q++) {
         const int row = row0 + q * RSTEP;
-        *reinterpret_cast<V*>(lds + row * LDK + kv * VEC) = r[q] * s;
+        *reinterpret_cast<V*>(lds + lds_off<LDK, VEC, SWZ>(row, kv * VEC)) = r[q] * s;
       }
     }
   }
@@ -156,7 +205,7 @@ struct Stage {
       const int rb = t % RB, kb = t / RB;
 #pragma unroll
       for (int e = 0; e < VEC; e++) {
-        const int off = (rb * VEC + e) * LDK + kb * NV;
+        const int off = lds_off<LDK, 8, SWZ_BF16>(rb * VEC + e, kb * NV);
         if constexpr (NV == 4) {
           u2v h, l;
           split4(f4v{r[0][e], r[1][e], r[2][e], r[3][e]}, h, l);
@@ -182,8 +231,9 @@ struct Stage {
         const int row = row0 + q * RSTEP;
         u2v h, l;
         split4(r[q], h, l);
-        *reinterpret_cast<u2v*>(hi + row * LDK + kv * VEC) = h;
-        *reinterpret_cast<u2v*>(lo + row * LDK + kv * VEC) = l;
+        const int off = lds_off<LDK, 8, SWZ_BF16>(row, kv * VEC);
+        *reinterpret_cast<u2v*>(hi + off) = h;
+        *reinterpret_cast<u2v*>(lo + off) = l;
       }
     }
   }
@@ -225,7 +275,7 @@ struct Stage {
       const int rb = t % RB, kb = t / RB;
 #pragma unroll
       for (int e = 0; e < VEC; e++) {
-        const int off = (rb * VEC + e) * LDK + kb * NV;
+        const int off = lds_off<LDK, 8, SWZ_BF16>(rb * VEC + e, kb * NV);
         if constexpr (NV == 4) {
           u2v h, m, l;
           split4(f4v{r[0][e], r[1][e], r[2][e], r[3][e]}, h, m, l);
@@ -254,9 +304,10 @@ struct Stage {
         const int row = row0 + q * RSTEP;
         u2v h, m, l;
         split4(r[q], h, m, l);
-        *reinterpret_cast<u2v*>(hi + row * LDK + kv * VEC) = h;
-        *reinterpret_cast<u2v*>(mid + row * LDK + kv * VEC) = m;
-        *reinterpret_cast<u2v*>(lo + row * LDK + kv * VEC) = l;
+        const int off = lds_off<LDK, 8, SWZ_BF16>(row, kv * VEC);
+        *reinterpret_cast<u2v*>(hi + off) = h;
+        *reinterpret_cast<u2v*>(mid + off) = m;
+        *reinterpret_cast<u2v*>(lo + off) = l;
       }
     }
   }
@@ -426,7 +477,8 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     // g = lane/16 multiply k = g*KG + ks (the sum over k is order-free), so a
     // lane's KG values of a row are contiguous in LDS (two 16-byte reads).
     constexpr int KG = BK / 4;
-    constexpr int LDK = BK + 16 / (int)sizeof(T);   // row pitch, staggered by 16 B
+    constexpr int LDK = BK;   // 128-byte rows, unit-permuted
+    constexpr int SWZ = SA::SWZ;
     __shared__ __attribute__((aligned(16))) T lds_a[2][BM * LDK];
     __shared__ __attribute__((aligned(16))) T lds_b[2][BN * LDK];
     if (nch > 0) {
@@ -436,21 +488,23 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       sb[0].template store<LDK>(lds_b[0]);
     }
     __syncthreads();
+    using FO = FragOff<LDK, VEC, SWZ>;
+    FO fa[KG / VEC] = {FO(wm0 + l16, g * KG), FO(wm0 + l16, g * KG + VEC)};
+    FO fb[KG / VEC] = {FO(wn0 + l16, g * KG), FO(wn0 + l16, g * KG + VEC)};
+    static_assert(KG / VEC == 2, "two k-vectors per lane group");
     // chunk c_begin + it sits in LDS buffer and register stage it & 1
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
       load_next(sa[p], sb[p]);
-      const T* la = lds_a[p] + (wm0 + l16) * LDK + g * KG;
-      const T* lb = lds_b[p] + (wn0 + l16) * LDK + g * KG;
       // reads in the order the MFMAs consume them (k-vector u outermost), so the
       // first MFMAs wait only for the first half of the reads
       V av[FM][KG / VEC], bv[FN][KG / VEC];
 #pragma unroll
       for (int u = 0; u < KG / VEC; u++) {
 #pragma unroll
-        for (int i = 0; i < FM; i++) av[i][u] = *reinterpret_cast<const V*>(la + i * 16 * LDK + u * VEC);
+        for (int i = 0; i < FM; i++) av[i][u] = *reinterpret_cast<const V*>(lds_a[p] + fa[u](i));
 #pragma unroll
-        for (int j = 0; j < FN; j++) bv[j][u] = *reinterpret_cast<const V*>(lb + j * 16 * LDK + u * VEC);
+        for (int j = 0; j < FN; j++) bv[j][u] = *reinterpret_cast<const V*>(lds_b[p] + fb[u](j));
         __builtin_amdgcn_sched_barrier(0);   // keep the reads in k-vector order
       }
 #pragma unroll
@@ -491,21 +545,21 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       sb[0].template store_split3<LDK>(lb[0][0], lb[1][0], lb[2][0]);
     }
     __syncthreads();
-    const int ko = g * 8;
+    const FragOff<LDK, 8, SWZ_BF16> fa(wm0 + l16, g * 8), fb(wn0 + l16, g * 8);
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
       load_next(sa[p], sb[p]);
       bf8v ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) {
-        const int o = (wm0 + i * 16 + l16) * LDK + ko;
+        const int o = fa(i);
         ah[i] = *reinterpret_cast<const bf8v*>(&la[0][p][o]);
         am[i] = *reinterpret_cast<const bf8v*>(&la[1][p][o]);
         al[i] = *reinterpret_cast<const bf8v*>(&la[2][p][o]);
       }
 #pragma unroll
       for (int j = 0; j < FN; j++) {
-        const int o = (wn0 + j * 16 + l16) * LDK + ko;
+        const int o = fb(j);
         bh[j] = *reinterpret_cast<const bf8v*>(&lb[0][p][o]);
         bm[j] = *reinterpret_cast<const bf8v*>(&lb[1][p][o]);
         bl[j] = *reinterpret_cast<const bf8v*>(&lb[2][p][o]);
@@ -545,7 +599,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       sb[0].template store_split<LDK>(lbh[0], lbl[0]);
     }
     __syncthreads();
-    const int ko = g * 8;
+    const FragOff<LDK, 8, SWZ_BF16> fa(wm0 + l16, g * 8), fb(wn0 + l16, g * 8);
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
       load_next(sa[p], sb[p]);
@@ -555,14 +609,14 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       constexpr bool PHASED = BM * BN <= 64 * 64;
       bf8v ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
-      for (int i = 0; i < FM; i++) ah[i] = *reinterpret_cast<const bf8v*>(&lah[p][(wm0 + i * 16 + l16) * LDK + ko]);
+      for (int i = 0; i < FM; i++) ah[i] = *reinterpret_cast<const bf8v*>(&lah[p][fa(i)]);
 #pragma unroll
-      for (int j = 0; j < FN; j++) bh[j] = *reinterpret_cast<const bf8v*>(&lbh[p][(wn0 + j * 16 + l16) * LDK + ko]);
+      for (int j = 0; j < FN; j++) bh[j] = *reinterpret_cast<const bf8v*>(&lbh[p][fb(j)]);
       if constexpr (PHASED) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int i = 0; i < FM; i++) al[i] = *reinterpret_cast<const bf8v*>(&lal[p][(wm0 + i * 16 + l16) * LDK + ko]);
+      for (int i = 0; i < FM; i++) al[i] = *reinterpret_cast<const bf8v*>(&lal[p][fa(i)]);
 #pragma unroll
-      for (int j = 0; j < FN; j++) bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][(wn0 + j * 16 + l16) * LDK + ko]);
+      for (int j = 0; j < FN; j++) bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][fb(j)]);
       if constexpr (PHASED) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
