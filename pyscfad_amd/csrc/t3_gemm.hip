@@ -408,12 +408,21 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
 #pragma unroll
     for (int j = 0; j < FN; j++) acc[i][j] = Frag{0, 0, 0, 0};
 
-  // Two register stages: the global loads of chunk c+2 are issued while
-  // chunk c is multiplied, and chunk c+1 is written to LDS after it.
+  // FP64, FP32, bf16x3: one register stage per operand.  When step c starts
+  // (LDS buffer c & 1 holds chunk c), the stage holds chunk c+1; after the
+  // step's fragment reads are issued it is written to the other buffer and
+  // refilled with chunk c+2, whose loads then have the whole multiply phase
+  // to land.  The MFMAs touch no memory, so the compiler may move the second
+  // half of them past the closing barrier, overlapping them with the next
+  // step's reads.
+  // bf16x6 keeps two stages (chunk c+2 loaded while c is multiplied, c+1
+  // split and written after it): its three-way split is long VALU work that
+  // measured faster behind the MFMAs than in front of them.
+  constexpr int NSTAGE = PREC == PREC_BF16X6 ? 2 : 1;
   using SA = Stage<T, V, BM, BK, VEC>;
   using SB = Stage<T, V, BN, BK, VEC>;
-  SA sa[2];
-  SB sb[2];
+  SA sa[NSTAGE];
+  SB sb[NSTAGE];
 
   // Segment walker: the chunks of this block are loaded in increasing order;
   // the tile origins advance by a fixed step per chunk (and a carry when k2
@@ -483,19 +492,18 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     __shared__ __attribute__((aligned(16))) T lds_b[2][BN * LDK];
     if (nch > 0) {
       load_next(sa[0], sb[0]);
-      load_next(sa[1], sb[1]);
       sa[0].template store<LDK>(lds_a[0]);
       sb[0].template store<LDK>(lds_b[0]);
+      load_next(sa[0], sb[0]);
     }
     __syncthreads();
     using FO = FragOff<LDK, VEC, SWZ>;
     FO fa[KG / VEC] = {FO(wm0 + l16, g * KG), FO(wm0 + l16, g * KG + VEC)};
     FO fb[KG / VEC] = {FO(wn0 + l16, g * KG), FO(wn0 + l16, g * KG + VEC)};
     static_assert(KG / VEC == 2, "two k-vectors per lane group");
-    // chunk c_begin + it sits in LDS buffer and register stage it & 1
+    // chunk c_begin + it sits in LDS buffer it & 1
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
-      load_next(sa[p], sb[p]);
       // reads in the order the MFMAs consume them (k-vector u outermost), so the
       // first MFMAs wait only for the first half of the reads
       V av[FM][KG / VEC], bv[FN][KG / VEC];
@@ -507,6 +515,12 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
         for (int j = 0; j < FN; j++) bv[j][u] = *reinterpret_cast<const V*>(lds_b[p] + fb[u](j));
         __builtin_amdgcn_sched_barrier(0);   // keep the reads in k-vector order
       }
+      if (it + 1 < nch) {
+        sa[0].template store<LDK>(lds_a[p ^ 1]);
+        sb[0].template store<LDK>(lds_b[p ^ 1]);
+        load_next(sa[0], sb[0]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int ks = 0; ks < KG; ks++) {
         // the MFMAs of the second k-vector stay after those of the first, so
@@ -521,13 +535,9 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
             else acc[i][j] = mfma_f32(a, b, acc[i][j]);
           }
       }
-      if (it + 1 < nch) {
-        sa[p ^ 1].template store<LDK>(lds_a[p ^ 1]);
-        sb[p ^ 1].template store<LDK>(lds_b[p ^ 1]);
-      }
       __syncthreads();
     };
-    // straight-line pairs keep the register stages in place across the loop
+    // straight-line pairs keep the LDS buffer index a constant
     int it = 0;
     for (; it + 1 < nch; it += 2) {
       step(it, std::integral_constant<int, 0>{});
@@ -540,7 +550,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     __shared__ __attribute__((aligned(16))) __bf16 lb[3][2][BN * LDK];
     if (nch > 0) {
       load_next(sa[0], sb[0]);
-      load_next(sa[1], sb[1]);
+      load_next(sa[NSTAGE - 1], sb[NSTAGE - 1]);
       sa[0].template store_split3<LDK>(la[0][0], la[1][0], la[2][0]);
       sb[0].template store_split3<LDK>(lb[0][0], lb[1][0], lb[2][0]);
     }
@@ -548,7 +558,7 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     const FragOff<LDK, 8, SWZ_BF16> fa(wm0 + l16, g * 8), fb(wn0 + l16, g * 8);
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
-      load_next(sa[p], sb[p]);
+      load_next(sa[p % NSTAGE], sb[p % NSTAGE]);
       bf8v ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) {
@@ -577,8 +587,8 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
           acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         }
       if (it + 1 < nch) {
-        sa[p ^ 1].template store_split3<LDK>(la[0][p ^ 1], la[1][p ^ 1], la[2][p ^ 1]);
-        sb[p ^ 1].template store_split3<LDK>(lb[0][p ^ 1], lb[1][p ^ 1], lb[2][p ^ 1]);
+        sa[(p ^ 1) % NSTAGE].template store_split3<LDK>(la[0][p ^ 1], la[1][p ^ 1], la[2][p ^ 1]);
+        sb[(p ^ 1) % NSTAGE].template store_split3<LDK>(lb[0][p ^ 1], lb[1][p ^ 1], lb[2][p ^ 1]);
       }
       __syncthreads();
     };
@@ -594,15 +604,14 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     __shared__ __attribute__((aligned(16))) __bf16 lbh[2][BN * LDK], lbl[2][BN * LDK];
     if (nch > 0) {
       load_next(sa[0], sb[0]);
-      load_next(sa[1], sb[1]);
       sa[0].template store_split<LDK>(lah[0], lal[0]);
       sb[0].template store_split<LDK>(lbh[0], lbl[0]);
+      load_next(sa[0], sb[0]);
     }
     __syncthreads();
     const FragOff<LDK, 8, SWZ_BF16> fa(wm0 + l16, g * 8), fb(wn0 + l16, g * 8);
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
-      load_next(sa[p], sb[p]);
       // Tiles up to 64x64: hi parts first, so the hi*hi products wait only
       // for them.  Larger tiles keep the compiler's interleaving (the pinned
       // order would keep all 4 fragment sets live and spill).
@@ -617,6 +626,11 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
       for (int i = 0; i < FM; i++) al[i] = *reinterpret_cast<const bf8v*>(&lal[p][fa(i)]);
 #pragma unroll
       for (int j = 0; j < FN; j++) bl[j] = *reinterpret_cast<const bf8v*>(&lbl[p][fb(j)]);
+      if (it + 1 < nch) {
+        sa[0].template store_split<LDK>(lah[p ^ 1], lal[p ^ 1]);
+        sb[0].template store_split<LDK>(lbh[p ^ 1], lbl[p ^ 1]);
+        load_next(sa[0], sb[0]);
+      }
       if constexpr (PHASED) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -641,13 +655,9 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
             acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
           }
       }
-      if (it + 1 < nch) {
-        sa[p ^ 1].template store_split<LDK>(lah[p ^ 1], lal[p ^ 1]);
-        sb[p ^ 1].template store_split<LDK>(lbh[p ^ 1], lbl[p ^ 1]);
-      }
       __syncthreads();
     };
-    // straight-line pairs keep the register stages in place across the loop
+    // straight-line pairs keep the LDS buffer index a constant
     int it = 0;
     for (; it + 1 < nch; it += 2) {
       step(it, std::integral_constant<int, 0>{});
