@@ -397,9 +397,15 @@ struct Ctx {
   std::string plan_error;   // set when a contraction does not fit the GEMM's operand modes
   int force_cfg[FAM_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};   // autotuner trials
 
-  // tile configuration of a contraction: a trial's, else the tuned one, else the default
+  // tile configuration of a contraction: a trial's, else the one
+  // PYSCFAD_AMD_TILE names (every family, for experiments and tests; ignored
+  // where the precision has no such tile), else the tuned one, else the default
   int cfg_for(int fam, int dflt) const {
     if (force_cfg[fam] >= 0) return force_cfg[fam];
+    if (const char* env = std::getenv("PYSCFAD_AMD_TILE")) {
+      const int x = std::atoi(env);
+      if (env[0] >= '0' && env[0] <= '9' && t3_gemm_cfg_valid(d.prec, x)) return x;
+    }
     const int t = tuned_cfg(d.prec, fam, d.V, d.Op);
     return t >= 0 ? t : dflt;
   }

@@ -30,6 +30,7 @@ int main(int argc, char** argv) {
   const size_t ws_size = c.r.desc;   // regions end where descriptors start
   const size_t t2_size = (size_t)o * o * v * v * 8;
   int bad = 0, nl = 0;
+  unsigned cfgs = 0;   // bit c: some GEMM launch uses tile configuration c
   auto check = [&](uintptr_t lo, uintptr_t hi, const char* what, size_t li, int p) {
     bool ok = (lo >= WS && hi <= WS + ws_size) || (lo >= OUT_T2 && hi <= OUT_T2 + t2_size) ||
               (lo >= OUT_OVOV && hi <= OUT_OVOV + t2_size);
@@ -57,6 +58,7 @@ int main(int argc, char** argv) {
     }
     if (L.kind != L_GEMM) continue;
     nl++;
+    cfgs |= 1u << L.cfg;
     const int pr = L.prec >= 0 ? L.prec : c.d.prec;
     const size_t e = pr == PREC_F64 ? 8 : 4;
     const int vec = t3_gemm_vec(pr);
@@ -91,7 +93,10 @@ int main(int argc, char** argv) {
     }
     if (blocks != L.work) { printf("launch %zu: block count %lld != work %lld\n", li, blocks, L.work); bad++; }
   }
-  printf("o=%d v=%d prec=%d naux=%d B=%d launches=%d gemm, bad=%d, plan_error='%s'\n", o, v, prec, naux, c.B, nl, bad,
-         c.plan_error.c_str());
+  printf("o=%d v=%d prec=%d naux=%d B=%d launches=%d gemm, bad=%d, plan_error='%s' cfgs=", o, v, prec, naux, c.B, nl,
+         bad, c.plan_error.c_str());
+  for (int k = 0; k < 32; k++)
+    if (cfgs >> k & 1) printf("%d,", k);
+  printf("\n");
   return bad ? 1 : 0;
 }

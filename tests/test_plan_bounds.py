@@ -35,11 +35,14 @@ def checker(tmp_path_factory):
     shutil.rmtree(d, ignore_errors=True)
 
 
-def _run(exe, *args, block=None):
+def _run(exe, *args, block=None, tile=None):
     env = dict(os.environ)
     env.pop("PYSCFAD_AMD_BATCH", None)
+    env.pop("PYSCFAD_AMD_TILE", None)
     if block is not None:
         env["PYSCFAD_AMD_DF_BLOCK"] = str(block)
+    if tile is not None:
+        env["PYSCFAD_AMD_TILE"] = str(tile)
     return subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, env=env, timeout=120)
 
 
@@ -63,3 +66,15 @@ def test_plan_in_bounds(checker, o, v, prec, naux, block):
 def test_checker_reports_planted_overflow(checker):
     r = _run(checker, 12, 100, 1, 0, "plant")
     assert r.returncode != 0 and "OOB" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("tile", [0, 1, 3, 4, 5, 6])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_plan_in_bounds_forced_tile(checker, prec, tile):
+    """Every contraction on one forced tile shape (PYSCFAD_AMD_TILE), as the
+    per-tile GPU test runs them (FP64 ignores the 128x128 shape it lacks)."""
+    r = _run(checker, 6, 130, prec, 0, tile=tile)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout and "plan_error=''" in r.stdout, r.stdout
+    if not (prec == 0 and tile == 1):
+        assert f"cfgs={tile},\n" in r.stdout, r.stdout   # every GEMM launch on the forced tile
