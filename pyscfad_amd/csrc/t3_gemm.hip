@@ -515,7 +515,11 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
         for (int j = 0; j < FN; j++) bv[j][u] = *reinterpret_cast<const V*>(lds_b[p] + fb[u](j));
         __builtin_amdgcn_sched_barrier(0);   // keep the reads in k-vector order
       }
-      if (it + 1 < nch) {
+      // Tiles up to 64x64 store and refill unconditionally (the last step
+      // writes the buffer no later step reads, and the walker reloads its
+      // last chunk): without the branch their schedule measured faster,
+      // while the larger tiles' was slower.
+      if (BM * BN <= 64 * 64 || it + 1 < nch) {
         sa[0].template store<LDK>(lds_a[p ^ 1]);
         sb[0].template store<LDK>(lds_b[p ^ 1]);
         load_next(sa[0], sb[0]);
