@@ -98,9 +98,9 @@ __global__ void prep_eps(double* __restrict__ eps_occ, double* __restrict__ eps_
 
 // Three passes over H per triple, one per contracted position, each with
 // coalesced reads of H:
-//   pass 0 (contract a): 64 (b,c) per block, 4 waves split a  -> S0, F0
-//   pass 1 (contract b): 64 (a,c) per block, 4 waves split b  -> S1, F1, PM2, PN2
-//   pass 2 (contract c): wave per (a,b), lanes over c          -> S2, F2, PM1, PN1, PM0, PN0
+//   pass 0 (contract a): 256 (b,c) per block, 4 per lane, 4 waves split a  -> S0, F0
+//   pass 1 (contract b): 256 (a,c) per block, 4 per lane, 4 waves split b  -> S1, F1, PM2, PN2
+//   pass 2 (contract c): 16 lanes per (a,b), 4 c per lane step -> S2, F2, PM1, PN1, PM0, PN0
 // The wave partials of passes 0/1 are added in wave order.  PM_d / PN_d are
 // partial sums (over the position that is neither d nor the row), summed to
 // vectors by vpart_vecsum.  blockIdx.y = npass * triple + (pass - pass0).
@@ -115,81 +115,121 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
   const long long VV = (long long)V * V;
   double* out = tr.out;
   if (pass < 2) {
-    __shared__ double red[4][4][64];
+    // four consecutive elements per lane, read with one vector load
+    typedef TP tp4 __attribute__((ext_vector_type(4)));
+    __shared__ double red[4][4][256];
     const int el = threadIdx.x & 63, sp = threadIdx.x >> 6;
-    const long long e = (long long)blockIdx.x * 64 + el;
-    const bool live = e < VV;
-    const long long ec = live ? e : 0;
-    const int u = (int)(ec / V), w = (int)(ec % V);
+    const long long e0 = ((long long)blockIdx.x * 64 + el) * 4;
+    const bool live = e0 < VV;   // VV is a multiple of 4: all four or none
+    const long long ec = live ? e0 : 0;
+    const int u = (int)(ec / V), w = (int)(ec % V);   // V is a multiple of 4: one u for all four
     const int td = occ[pass];
-    double s = 0.0, f = 0.0, pm = 0.0, pn = 0.0;
-    // unrolled so that eight H loads per wave are in flight (the sums keep
-    // their order: x ascending within the wave)
+    double s[4] = {0, 0, 0, 0}, f[4] = {0, 0, 0, 0}, pm[4] = {0, 0, 0, 0}, pn[4] = {0, 0, 0, 0};
+    // unrolled so that eight H loads per lane are in flight (each sum keeps
+    // its order: x ascending within the wave)
     if (pass == 0) {        // e = (b, c)
 #pragma unroll 8
       for (int x = sp; x < v; x += 4) {
-        const double hv = (double)H[(long long)x * VV + ec];
-        s += hv * t1[(long long)td * v + x];
-        f += hv * fvoP[(long long)td * V + x];
+        const tp4 hv = *reinterpret_cast<const tp4*>(H + (long long)x * VV + ec);
+        const double t = t1[(long long)td * v + x], fv = fvoP[(long long)td * V + x];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          s[q] += (double)hv[q] * t;
+          f[q] += (double)hv[q] * fv;
+        }
       }
     } else {                // e = (a, c)
       const double* CM2 = out + (VP_CM + 4) * VV + (long long)u * V;
       const double* CN2 = out + (VP_CM + 5) * VV + (long long)u * V;
 #pragma unroll 8
       for (int x = sp; x < v; x += 4) {
-        const double hv = (double)H[(long long)u * VV + (long long)x * V + w];
-        s += hv * t1[(long long)td * v + x];
-        f += hv * fvoP[(long long)td * V + x];
-        pm += hv * CM2[x];
-        pn += hv * CN2[x];
+        const tp4 hv = *reinterpret_cast<const tp4*>(H + (long long)u * VV + (long long)x * V + w);
+        const double t = t1[(long long)td * v + x], fv = fvoP[(long long)td * V + x];
+        const double cm = CM2[x], cn = CN2[x];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          s[q] += (double)hv[q] * t;
+          f[q] += (double)hv[q] * fv;
+          pm[q] += (double)hv[q] * cm;
+          pn[q] += (double)hv[q] * cn;
+        }
       }
     }
-    red[0][sp][el] = s;
-    red[1][sp][el] = f;
-    red[2][sp][el] = pm;
-    red[3][sp][el] = pn;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      red[0][sp][el * 4 + q] = s[q];
+      red[1][sp][el * 4 + q] = f[q];
+      red[2][sp][el * 4 + q] = pm[q];
+      red[3][sp][el * 4 + q] = pn[q];
+    }
     __syncthreads();
     if (sp != 0 || !live) return;
-    double r[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) r[q] = ((red[q][0][el] + red[q][1][el]) + red[q][2][el]) + red[q][3][el];
-    if (pass == 0) {
-      out[0 * VV + e] = r[0];
-      out[1 * VV + e] = r[1];
-    } else {
-      out[2 * VV + e] = r[0];
-      out[3 * VV + e] = r[1];
-      out[(6 + 4) * VV + e] = r[2];   // PM2[a][c]
-      out[(7 + 4) * VV + e] = r[3];   // PN2[a][c]
+    for (int q = 0; q < 4; q++) {
+      const int k = el * 4 + q;
+      const long long e = e0 + q;
+      double r[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) r[m] = ((red[m][0][k] + red[m][1][k]) + red[m][2][k]) + red[m][3][k];
+      if (pass == 0) {
+        out[0 * VV + e] = r[0];
+        out[1 * VV + e] = r[1];
+      } else {
+        out[2 * VV + e] = r[0];
+        out[3 * VV + e] = r[1];
+        out[(6 + 4) * VV + e] = r[2];   // PM2[a][c]
+        out[(7 + 4) * VV + e] = r[3];   // PN2[a][c]
+      }
     }
     return;
   }
-  // pass 2: one wave per (a, b) row
-  const int lane = threadIdx.x & 63;
-  const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= VV) return;
-  const int a = (int)(row / V), b = (int)(row % V);
+  // pass 2: 16 lanes per (a, b) row, four rows per wave; a lane takes four
+  // consecutive c per step with one vector load of H
+  typedef TP tp4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, l16 = lane & 15;
+  const long long row = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
+  const bool live = row < VV;
+  const long long rw = live ? row : 0;
+  const int a = (int)(rw / V), b = (int)(rw % V);
   const int t2o = occ[2];
-  const TP* h = H + row * V;
+  const TP* h = H + rw * V;
   const double* CM1 = out + (VP_CM + 2) * VV + (long long)a * V;
   const double* CN1 = out + (VP_CM + 3) * VV + (long long)a * V;
   const double* CM0 = out + (VP_CM + 0) * VV + (long long)b * V;
   const double* CN0 = out + (VP_CM + 1) * VV + (long long)b * V;
+  const double* t1r = t1 + (long long)t2o * v;
+  const double* fvr = fvoP + (long long)t2o * V;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = lane; c < v; c += 64) {
-    const double hv = (double)h[c];
-    acc[0] += hv * t1[(long long)t2o * v + c];
-    acc[1] += hv * fvoP[(long long)t2o * V + c];
-    acc[2] += hv * CM1[c];
-    acc[3] += hv * CN1[c];
-    acc[4] += hv * CM0[c];
-    acc[5] += hv * CN0[c];
+  // the fvo and pair-matrix rows are V doubles (a multiple of 32) from
+  // 256-byte aligned bases: 32-byte vector loads
+  typedef double dv4 __attribute__((ext_vector_type(4)));
+#pragma unroll 4
+  for (int c0 = l16 * 4; c0 < v; c0 += 64) {   // c0 + 3 < V: V is a multiple of 32
+    const tp4 hv = *reinterpret_cast<const tp4*>(h + c0);
+    const dv4 fv = *reinterpret_cast<const dv4*>(fvr + c0);
+    const dv4 m1 = *reinterpret_cast<const dv4*>(CM1 + c0), n1 = *reinterpret_cast<const dv4*>(CN1 + c0);
+    const dv4 m0 = *reinterpret_cast<const dv4*>(CM0 + c0), n0 = *reinterpret_cast<const dv4*>(CN0 + c0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int c = c0 + q;
+      if (c < v) {
+        const double x = (double)hv[q];
+        acc[0] += x * t1r[c];
+        acc[1] += x * fv[q];
+        acc[2] += x * m1[q];
+        acc[3] += x * n1[q];
+        acc[4] += x * m0[q];
+        acc[5] += x * n0[q];
+      }
+    }
   }
-  // fixed butterfly: every lane ends with the same, order-independent-of-scheduling sum
+  // fixed butterfly over the row's 16 lanes: every lane of the group ends with
+  // the same sum, independent of scheduling
 #pragma unroll
   for (int q = 0; q < 6; q++)
-    for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
-  if (lane == 0) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
+  if (l16 == 0 && live) {
     out[4 * VV + row] = acc[0];            // S2[a][b]
     out[5 * VV + row] = acc[1];            // F2[a][b]
     out[(6 + 2) * VV + row] = acc[2];      // PM1[a][b]
@@ -214,11 +254,14 @@ vpart_vecsum(const VpTriple* __restrict__ trs, int V) {
   double* vs = tr.out + (long long)VP_VSUM * VV + (long long)task * V;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (d == 0) {
-    // 16 rows per wave, lanes over the row
+    // 16 rows per wave, lanes over the row (unrolled: the loads of several
+    // rows in flight together; every sum keeps its order)
+#pragma unroll 4
     for (int rr = 0; rr < 16; rr++) {
       const int z = blockIdx.x * 64 + wv * 16 + rr;
       if (z >= V) break;
       double sacc = 0.0;
+#pragma unroll 4
       for (int b = lane; b < V; b += 64) sacc += P[(long long)z * V + b];
       for (int off = 32; off > 0; off >>= 1) sacc += __shfl_xor(sacc, off, 64);
       if (lane == 0) vs[z] = sacc;
@@ -228,6 +271,7 @@ vpart_vecsum(const VpTriple* __restrict__ trs, int V) {
   const int z = blockIdx.x * 64 + lane;
   double sacc = 0.0;
   if (z < V)
+#pragma unroll 16
     for (int a = wv; a < V; a += 4) sacc += P[(long long)a * V + z];
   red[wv][lane] = sacc;
   __syncthreads();
@@ -423,8 +467,8 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
                                       double* t1_acc, double* fvo_acc, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
   const long long VV = (long long)V * V;
-  // passes 0/1: one thread per output element; pass 2: one wave per row (4 rows per block)
-  dim3 g1((unsigned)((VV + 63) / 64), 2 * ntriple), g2p((unsigned)((VV + 3) / 4), ntriple);
+  // passes 0/1: four output elements per lane (256 per block); pass 2: 16 lanes per row (16 rows per block)
+  dim3 g1((unsigned)((VV + 255) / 256), 2 * ntriple), g2p((unsigned)((VV + 15) / 16), ntriple);
   if (tp_is_f64) {
     hipLaunchKernelGGL(vpart_contract<double>, g1, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 2, 0);
     hipLaunchKernelGGL(vpart_contract<double>, g2p, dim3(256), 0, s, trs, o, v, V, t1, fvoP, 1, 2);
