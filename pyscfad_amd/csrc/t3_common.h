@@ -49,6 +49,8 @@ struct GemmProb {
   int ksplit;
   int tiles_m, tiles_n;
   long long block0;   // first block of this problem in the launch
+  int interleave;     // same in every problem of a launch: 1 when all have the same tile grid and
+                      // block t is tile t / nprob of problem t % nprob (block0 then unused)
   void* C;
   long long c_sb, c_sm, c_sn;
   int mode;

@@ -581,6 +581,16 @@ struct Ctx {
         P.P = nullptr;
       }
     }
+    // The hole-side adjoints read the G cube once per permutation, and the
+    // problems of a permutation pair read it through the same view: with one
+    // tile grid for all problems, their tiles are interleaved, so both
+    // readers of a G block run on one XCD at about the same time.
+    bool inter = (sub == FAM_GEMM_B3 || sub == FAM_GEMM_B4) && gb.probs.size() > 1;
+    for (const auto& P : gb.probs) {
+      const GemmProb& Q = gb.probs[0];
+      inter = inter && P.nb == Q.nb && P.tiles_m == Q.tiles_m && P.tiles_n == Q.tiles_n && P.ksplit == Q.ksplit;
+    }
+    for (auto& P : gb.probs) P.interleave = inter ? 1 : 0;
     Launch L;
     L.kind = L_GEMM;
     L.cfg = cfg;
@@ -913,10 +923,12 @@ struct Ctx {
         sg.A = gslot((int)s);
         sg.a_sm = str(p[0]);
         sg.b_sn = V * V;
-        if (p[2] == 2) {   // k = (s_p1, s_p2), B from t2H[t_p2][m][s_p1][s_p2]
+        // k runs over the two contracted cube positions in increasing order,
+        // so the two permutations with the same free position read G alike
+        if (p[1] < p[2]) {   // k = (s_p1, s_p2), B from t2H[t_p2][m][s_p1][s_p2]
           sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[2]);
           sg.B = el(r.t2H, (long long)tr.t[p[2]] * Op * V * V);
-        } else {           // k = (s_p2, s_p1), B from t2T[t_p2][m][s_p2][s_p1]
+        } else {             // k = (s_p2, s_p1), B from t2T[t_p2][m][s_p2][s_p1]
           sg.a_sk1 = str(p[2]); sg.a_sk2 = str(p[1]);
           sg.B = el(r.t2T, (long long)tr.t[p[2]] * Op * V * V);
         }

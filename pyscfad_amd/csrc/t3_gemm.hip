@@ -324,13 +324,20 @@ struct TileCoord {
 
 __device__ __forceinline__ TileCoord find_tile(const GemmProb* __restrict__ probs, int nprob,
                                                long long bid) {
-  int lo = 0, hi = nprob - 1;
-  while (lo < hi) {   // last problem with block0 <= bid
-    const int mid = (lo + hi + 1) >> 1;
-    if (probs[mid].block0 <= bid) lo = mid; else hi = mid - 1;
+  int lo = 0;
+  long long local;
+  if (probs[0].interleave) {   // problems of one tile grid, their tiles interleaved
+    lo = (int)(bid % nprob);
+    local = bid / nprob;
+  } else {
+    int hi = nprob - 1;
+    while (lo < hi) {   // last problem with block0 <= bid
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].block0 <= bid) lo = mid; else hi = mid - 1;
+    }
+    local = bid - probs[lo].block0;
   }
   const GemmProb& P = probs[lo];
-  long long local = bid - P.block0;
   TileCoord tc;
   tc.p = lo;
   tc.split = (int)(local % P.ksplit); local /= P.ksplit;
