@@ -40,7 +40,8 @@ int t3_gemm_cfg_valid(int prec, int cfg);
 int t3_gemm_chunk(int prec);
 hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob, const GemmSeg* segs,
                           long long nblocks, hipStream_t stream);
-hipError_t t3_launch_reduce(int prec, const ReduceProb* probs, int nprob, long long total, hipStream_t stream);
+hipError_t t3_launch_reduce(int prec, const ReduceProb* probs, int nprob, long long total, int vec,
+                            hipStream_t stream);
 hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, const EpiTriple* triples, int ntriple,
                               hipStream_t stream);
 hipError_t t3_launch_epi_reduce(const EpiReduceArgs* a, hipStream_t stream);
@@ -606,6 +607,11 @@ struct Ctx {
       Launch R;
       R.kind = L_REDUCE;
       R.prec = gb.prec;
+      // vector partial loads: every N a multiple of the vector width, 32-bit element indices
+      const int ve = gb.prec == PREC_F64 ? 2 : 4;
+      bool vec = true;
+      for (const auto& Q : red) vec = vec && Q.N % ve == 0 && (long long)Q.nb * Q.M * Q.N < (1LL << 31);
+      R.cfg = vec ? 1 : 0;
       R.off_a = blob.put(red);
       R.n_a = (int)red.size();
       R.work = relem;
@@ -1114,7 +1120,7 @@ struct Ctx {
           break;
         case L_REDUCE:
           e = t3_launch_reduce(L.prec >= 0 ? L.prec : d.prec, reinterpret_cast<const ReduceProb*>(db + L.off_a),
-                               L.n_a, L.work, st);
+                               L.n_a, L.work, L.cfg, st);
           break;
         case L_ZERO:
           e = t3_zero(ws + L.off_a, (size_t)L.work, st);

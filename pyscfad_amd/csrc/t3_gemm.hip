@@ -694,10 +694,13 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
 }
 
 // C (+)= alpha * sum_s P[s], summed in FP64 in fixed split order.
-template <class Acc>
+// VE consecutive elements of a row per thread (VE > 1: every problem's N is a
+// multiple of VE and nb * M * N < 2^31, so the partials load as 16-byte
+// vectors and the index arithmetic is 32-bit); each element keeps its own sum.
+template <class Acc, int VE>
 __global__ void __launch_bounds__(256)
 reduce_partials(const ReduceProb* __restrict__ probs, int nprob, long long total) {
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long e = ((long long)blockIdx.x * 256 + threadIdx.x) * VE;
   if (e >= total) return;
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
@@ -705,24 +708,55 @@ reduce_partials(const ReduceProb* __restrict__ probs, int nprob, long long total
     if (probs[mid].elem0 <= e) lo = mid; else hi = mid - 1;
   }
   const ReduceProb& R = probs[lo];
-  const long long local = e - R.elem0;
-  const long long per = (long long)R.nb * R.M * R.N;
-  if (local >= per) return;
-  const int n = (int)(local % R.N);
-  const int m = (int)((local / R.N) % R.M);
-  const int b = (int)(local / ((long long)R.N * R.M));
-  if (m >= R.Mv || n >= R.Nv) return;
   const T3_GLOBAL Acc* P = (const T3_GLOBAL Acc*)R.P;
-  double s = 0.0;
-  for (int sp = 0; sp < R.ksplit; sp++) s += (double)P[sp * per + local];
-  s *= R.alpha;
-  const long long off = (long long)b * R.c_sb + (long long)m * R.c_sm + (long long)n * R.c_sn;
-  if (R.c_is_f64) {
-    T3_GLOBAL double* c = (T3_GLOBAL double*)R.C + off;
-    *c = R.accumulate ? *c + s : s;
+  double s[VE];
+  int b, m, n0;
+  long long local;
+  if constexpr (VE == 1) {
+    local = e - R.elem0;
+    const long long per = (long long)R.nb * R.M * R.N;
+    if (local >= per) return;
+    n0 = (int)(local % R.N);
+    m = (int)((local / R.N) % R.M);
+    b = (int)(local / ((long long)R.N * R.M));
+    if (m >= R.Mv || n0 >= R.Nv) return;
+    s[0] = 0.0;
+    for (int sp = 0; sp < R.ksplit; sp++) s[0] += (double)P[sp * per + local];
   } else {
-    T3_GLOBAL float* c = (T3_GLOBAL float*)R.C + off;
-    *c = R.accumulate ? (float)((double)*c + s) : (float)s;
+    typedef Acc accv __attribute__((ext_vector_type(VE)));
+    const unsigned ul = (unsigned)(e - R.elem0), N = (unsigned)R.N, M = (unsigned)R.M;
+    const unsigned per = (unsigned)R.nb * M * N;
+    if (ul >= per) return;
+    local = ul;
+    n0 = (int)(ul % N);
+    const unsigned mb = ul / N;
+    m = (int)(mb % M);
+    b = (int)(mb / M);
+    if (m >= R.Mv || n0 >= R.Nv) return;
+#pragma unroll
+    for (int j = 0; j < VE; j++) s[j] = 0.0;
+    const T3_GLOBAL accv* Pv = (const T3_GLOBAL accv*)(P + ul);
+    const unsigned pstride = per / VE;
+#pragma unroll 4
+    for (int sp = 0; sp < R.ksplit; sp++) {
+      const accv x = Pv[(unsigned)sp * pstride];
+#pragma unroll
+      for (int j = 0; j < VE; j++) s[j] += (double)x[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < VE; j++) {
+    const int n = n0 + j;
+    if (n >= R.Nv) break;
+    const double x = s[j] * R.alpha;
+    const long long off = (long long)b * R.c_sb + (long long)m * R.c_sm + (long long)n * R.c_sn;
+    if (R.c_is_f64) {
+      T3_GLOBAL double* c = (T3_GLOBAL double*)R.C + off;
+      *c = R.accumulate ? *c + x : x;
+    } else {
+      T3_GLOBAL float* c = (T3_GLOBAL float*)R.C + off;
+      *c = R.accumulate ? (float)((double)*c + x) : (float)x;
+    }
   }
 }
 
@@ -774,11 +808,20 @@ extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, i
   return hipGetLastError();
 }
 
-extern "C" hipError_t t3_launch_reduce(int prec, const ReduceProb* probs, int nprob, long long total,
+// vec: the plan found every problem's N a multiple of the vector width
+// (4 FP32 / 2 FP64 partials) and every problem below 2^31 elements
+extern "C" hipError_t t3_launch_reduce(int prec, const ReduceProb* probs, int nprob, long long total, int vec,
                                        hipStream_t stream) {
   if (total <= 0) return hipSuccess;
-  dim3 grid((unsigned)((total + 255) / 256)), block(256);
-  if (prec == PREC_F64) hipLaunchKernelGGL(reduce_partials<double>, grid, block, 0, stream, probs, nprob, total);
-  else hipLaunchKernelGGL(reduce_partials<float>, grid, block, 0, stream, probs, nprob, total);
+  const int ve = vec ? (prec == PREC_F64 ? 2 : 4) : 1;
+  const long long threads = (total + ve - 1) / ve;
+  dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+  if (prec == PREC_F64) {
+    if (ve > 1) hipLaunchKernelGGL((reduce_partials<double, 2>), grid, block, 0, stream, probs, nprob, total);
+    else hipLaunchKernelGGL((reduce_partials<double, 1>), grid, block, 0, stream, probs, nprob, total);
+  } else {
+    if (ve > 1) hipLaunchKernelGGL((reduce_partials<float, 4>), grid, block, 0, stream, probs, nprob, total);
+    else hipLaunchKernelGGL((reduce_partials<float, 1>), grid, block, 0, stream, probs, nprob, total);
+  }
   return hipGetLastError();
 }

@@ -5,7 +5,8 @@
 // element its loads can reach (rows clamped as the kernel clamps them, whole
 // row vectors for row-contiguous operands) and the furthest element its
 // output or split partials can reach lie inside the workspace region or the
-// caller's in-place outputs; likewise every split-K reduction.
+// caller's in-place outputs; likewise every split-K reduction (and, for the
+// vectorised reductions, the alignment and size they assume).
 // Usage: plan_check nocc nvir precision [naux [plant]]
 //   plant: moves the split-K scratch to the end of the workspace, so that a
 //   correct checker must report out-of-bounds partials.
@@ -49,6 +50,11 @@ int main(int argc, char** argv) {
         if (Q.elem0 != total) { printf("reduce %zu prob %d: elem0 %lld != %lld\n", li, p, Q.elem0, total); bad++; }
         total += (long long)Q.nb * Q.M * Q.N;
         check((uintptr_t)Q.P, (uintptr_t)Q.P + (size_t)Q.ksplit * Q.nb * Q.M * Q.N * e, "reduce P", li, p);
+        // vector reduction: 16-byte partial vectors, 32-bit element indices
+        if (L.cfg && ((uintptr_t)Q.P % 16 != 0 || Q.N % (16 / e) != 0 || (long long)Q.nb * Q.M * Q.N >= (1LL << 31))) {
+          printf("reduce %zu prob %d: vector reduction on an unaligned or oversized problem\n", li, p);
+          bad++;
+        }
         const size_t ce = Q.c_is_f64 ? 8 : 4;
         const long long cmax = (long long)(Q.nb - 1) * Q.c_sb + (long long)(Q.Mv - 1) * Q.c_sm + (long long)(Q.Nv - 1) * Q.c_sn;
         check((uintptr_t)Q.C, (uintptr_t)Q.C + (cmax + 1) * ce, "reduce C", li, p);
