@@ -74,6 +74,27 @@ __device__ __forceinline__ double lane_group_sum(double x) {
   return x;
 }
 
+// LDS position of entry (y0, y1, y2) of an 8x8x8 sub-cube: each row y0
+// XOR-permutes its 8-entry groups and the entries within them.  A wave reads
+// the sub-cubes transposed, its lanes spanning two of the three indices in any
+// of the six orders; with this map every such pattern hits 64 distinct banks
+// (the plain [y0][y1][y2] order puts 8 lanes on a bank whenever the lanes vary
+// y0).  A VW-aligned run of y2 stays one aligned vector, its elements permuted
+// by XOR with y0 % VW (xor_perm).
+__device__ __forceinline__ int sub_pos(int y0, int y1, int y2) {
+  return (y0 * EPI_T + (y1 ^ y0)) * EPI_T + (y2 ^ y0);
+}
+template <class VT, int VW>
+__device__ __forceinline__ VT xor_perm(VT v, int k) {
+  if constexpr (VW == 4) {
+    const VT a = (k & 1) ? v.yxwz : v;
+    return (k & 2) ? a.zwxy : a;
+  } else {
+    static_assert(VW == 2, "four floats or two doubles per vector");
+    return (k & 1) ? v.yx : v;
+  }
+}
+
 template <class TP, bool BW>
 __global__ void __launch_bounds__(256)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
@@ -118,8 +139,8 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       case 4: q0 = TC; q1 = TA; q2 = TB; break;
       default: q0 = TC; q1 = TB; q2 = TA; break;
     }
-    *reinterpret_cast<tpv*>(&sc[p][r]) =
-        *reinterpret_cast<const tpv*>(&w0[(((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2)]);
+    const tpv x = *reinterpret_cast<const tpv*>(&w0[(((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2)]);
+    *reinterpret_cast<tpv*>(&sc[p][sub_pos(x0, x1, x2 ^ (x0 & (VW - 1)))]) = xor_perm<tpv, VW>(x, x0 & (VW - 1));
   }
   // --- CM_d / CN_d on the ordered tile pairs: block [d][pair mu][which][8x8] ---
   {
@@ -169,7 +190,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
 #pragma unroll
     for (int p = 0; p < 6; p++) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
-      W[p] = (double)sc[p][(x0 * T + x1) * T + x2];
+      W[p] = (double)sc[p][sub_pos(x0, x1, x2)];
       // v0 at the orbit point pi(s): position m of the point holds s_{pi_m}
       double vv = 0.0;
 #pragma unroll
@@ -242,7 +263,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
 #pragma unroll
     for (int p = 0; p < 6; p++) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
-      const int r = (x0 * T + x1) * T + x2;
+      const int r = sub_pos(x0, x1, x2);
       sc[p][r] = (TP)gval[h][p];
       hl[p * T3 + r] = (TP)hval[h][p];
     }
@@ -265,8 +286,9 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
         const int r = e * VW;
         const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
         const long long off = ((b0 + x0) * V + (b1 + x1)) * V + (b2 + x2);
-        *reinterpret_cast<tpv*>(&G[off]) = *reinterpret_cast<const tpv*>(&sc[p][r]);
-        *reinterpret_cast<tpv*>(&H[off]) = *reinterpret_cast<const tpv*>(&hl[p * T3 + r]);
+        const int k = x0 & (VW - 1), rs = sub_pos(x0, x1, x2 ^ k);
+        *reinterpret_cast<tpv*>(&G[off]) = xor_perm<tpv, VW>(*reinterpret_cast<const tpv*>(&sc[p][rs]), k);
+        *reinterpret_cast<tpv*>(&H[off]) = xor_perm<tpv, VW>(*reinterpret_cast<const tpv*>(&hl[p * T3 + rs]), k);
       }
       continue;
     }
@@ -290,8 +312,9 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       }
       if (!write) continue;
       const long long off = ((b0 + x[0]) * V + (b1 + x[1])) * V + (b2 + x[2]);
-      G[off] = sc[p][r];
-      H[off] = hl[p * T3 + r];
+      const int rs = sub_pos(x[0], x[1], x[2]);
+      G[off] = sc[p][rs];
+      H[off] = hl[p * T3 + rs];
     }
   }
 }
@@ -305,17 +328,15 @@ __device__ __forceinline__ int cube_id(int A, int B, int C) {
 // Block X < ntile: eps_bar[o + 8X + l] += the vir partials of every cube that
 //                  contains tile X (backward only).
 __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
-  __shared__ double red[256];
-  __shared__ double red8[256][EPI_T + 1];
-  const int tid = threadIdx.x;
-  auto tree = [&](double x) -> double {
-    red[tid] = x;
+  __shared__ double wpart[4][EPI_T];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // block sum in a fixed order: a butterfly within each wave, then the four
+  // wave sums in order
+  auto block_sum = [&](double x) -> double {
+    x = wave_sum(x);
+    if (lane == 0) wpart[wv][0] = x;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w) red[tid] += red[tid + w];
-      __syncthreads();
-    }
-    const double r = red[0];
+    const double r = ((wpart[0][0] + wpart[1][0]) + wpart[2][0]) + wpart[3][0];
     __syncthreads();
     return r;
   };
@@ -325,7 +346,7 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
       const EpiTriple& tr = a.triples[t];
       double q = 0.0;
       for (int c = tid; c < a.ncube; c += 256) q += tr.part[(long long)c * EPI_PSTRIDE];
-      e += 2.0 * tr.fo * tree(q);
+      e += 2.0 * tr.fo * block_sum(q);
     }
     if (tid == 0) *a.energy += e;
     if (!a.backward) return;
@@ -333,7 +354,7 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
       const EpiTriple& tr = a.triples[t];
       double r = 0.0;
       for (int c = tid; c < a.ncube; c += 256) r += tr.part[(long long)c * EPI_PSTRIDE + 1];
-      r = tree(r);
+      r = block_sum(r);
       if (tid == 0) {
         a.eps_bar[tr.i] -= r;
         a.eps_bar[tr.j] -= r;
@@ -352,7 +373,9 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
     double acc[EPI_T];
     for (int l = 0; l < EPI_T; l++) acc[l] = 0.0;
     for (int pr = tid; pr < npair; pr += 256) {
-      int X = 0;
+      // pair pr = (X, Y), Y <= X, in the order X(X+1)/2 + Y
+      int X = (int)((sqrtf(8.0f * (float)pr + 1.0f) - 1.0f) * 0.5f);
+      while (X > 0 && X * (X + 1) / 2 > pr) X--;
       while ((X + 1) * (X + 2) / 2 <= pr) X++;
       const int Y = pr - X * (X + 1) / 2;
       int u0 = TX, u1 = X, u2 = Y;   // sort descending
@@ -366,17 +389,25 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
         if (u2 == TX) acc[l] += pc[2 + 2 * EPI_T + l];
       }
     }
-    for (int l = 0; l < EPI_T; l++) red8[tid][l] = acc[l];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w)
-        for (int l = 0; l < EPI_T; l++) red8[tid][l] += red8[tid + w][l];
-      __syncthreads();
+#pragma unroll
+    for (int l = 0; l < EPI_T; l++) acc[l] = wave_sum(acc[l]);
+    if (lane < EPI_T) {
+      double x = acc[0];
+#pragma unroll
+      for (int l = 1; l < EPI_T; l++) x = lane == l ? acc[l] : x;
+      wpart[wv][lane] = x;
     }
-    for (int l = 0; l < EPI_T; l++) total[l] += red8[0][l];
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < EPI_T; l++) total[l] += ((wpart[0][l] + wpart[1][l]) + wpart[2][l]) + wpart[3][l];
     __syncthreads();
   }
-  if (tid < EPI_T) a.eps_bar[a.nocc + TX * EPI_T + tid] += total[tid];
+  if (tid < EPI_T) {
+    double x = total[0];
+#pragma unroll
+    for (int l = 1; l < EPI_T; l++) x = tid == l ? total[l] : x;
+    a.eps_bar[a.nocc + TX * EPI_T + tid] += x;
+  }
 }
 
 }  // namespace t3
