@@ -538,6 +538,8 @@ struct Ctx {
                         ((BN - 1) * s.b_sn + (gb.BK - 1) * s.b_sk2 + vec) * (long long)elt < (1LL << 32);
         if (!ok && plan_error.empty()) plan_error = "GEMM operand is neither row- nor k-contiguous, or misaligned";
         if (s.scale != 1.0f && s.scale != -1.0f && plan_error.empty()) plan_error = "GEMM segment scale must be +1 or -1";
+        // the kernel's chunk and tile index arithmetic is 32-bit
+        if ((long long)s.K1 * s.K2 >= (1LL << 31) && plan_error.empty()) plan_error = "GEMM segment K exceeds 2^31";
       }
     }
     std::vector<GemmSeg> flat;
@@ -553,6 +555,7 @@ struct Ctx {
       P.ksplit = allow_split ? ksplit : 1;
       P.block0 = nblocks;
       nblocks += (long long)P.nb * P.tiles_m * P.tiles_n * P.ksplit;
+      if (nblocks >= (1LL << 31) && plan_error.empty()) plan_error = "GEMM launch exceeds 2^31 tiles";
       if (allow_split) {
         const size_t bytes = (size_t)P.ksplit * P.nb * P.M * P.N * elt;
         P.mode = GEMM_PARTIAL;

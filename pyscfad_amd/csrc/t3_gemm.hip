@@ -327,8 +327,8 @@ __device__ __forceinline__ TileCoord find_tile(const GemmProb* __restrict__ prob
   int lo = 0;
   long long local;
   if (probs[0].interleave) {   // problems of one tile grid, their tiles interleaved
-    lo = (int)(bid % nprob);
-    local = bid / nprob;
+    lo = (int)((unsigned)bid % (unsigned)nprob);
+    local = (unsigned)bid / (unsigned)nprob;
   } else {
     int hi = nprob - 1;
     while (lo < hi) {   // last problem with block0 <= bid
@@ -338,12 +338,14 @@ __device__ __forceinline__ TileCoord find_tile(const GemmProb* __restrict__ prob
     local = bid - probs[lo].block0;
   }
   const GemmProb& P = probs[lo];
+  // a launch has fewer than 2^31 tiles (checked by the plan): 32-bit arithmetic
+  unsigned u = (unsigned)local;
   TileCoord tc;
   tc.p = lo;
-  tc.split = (int)(local % P.ksplit); local /= P.ksplit;
-  tc.tn = (int)(local % P.tiles_n); local /= P.tiles_n;
-  tc.tm = (int)(local % P.tiles_m); local /= P.tiles_m;
-  tc.b = (int)local;
+  tc.split = (int)(u % (unsigned)P.ksplit); u /= (unsigned)P.ksplit;
+  tc.tn = (int)(u % (unsigned)P.tiles_n); u /= (unsigned)P.tiles_n;
+  tc.tm = (int)(u % (unsigned)P.tiles_m); u /= (unsigned)P.tiles_m;
+  tc.b = (int)u;
   return tc;
 }
 
@@ -444,9 +446,10 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   uint32_t oa[SA::NV], ob[SB::NV];
   auto seg_init = [&](int c) {
     const GemmSeg& S = segs[si];
-    const long long kk = (long long)(c - S.chunk0) * BK;
-    const long long k1 = kk / S.K2;
-    k2 = (int)(kk % S.K2);
+    // K1 * K2 < 2^31 for every segment (checked by the plan): 32-bit division
+    const unsigned kk = (unsigned)(c - S.chunk0) * BK;
+    const long long k1 = kk / (unsigned)S.K2;
+    k2 = (int)(kk % (unsigned)S.K2);
     K2 = S.K2;
     constexpr long long E = sizeof(T);
     pa = reinterpret_cast<const char*>(S.A) + (tc.b * S.a_sb + m0 * S.a_sm + k1 * S.a_sk1 + k2 * S.a_sk2) * E;
@@ -799,7 +802,7 @@ static void launch_cfg(int cfg, dim3 grid, hipStream_t stream, const GemmProb* p
 extern "C" hipError_t t3_launch_gemm(int prec, int cfg, const GemmProb* probs, int nprob,
                                      const GemmSeg* segs, long long nblocks, hipStream_t stream) {
   if (nblocks <= 0) return hipSuccess;
-  if (!t3_gemm_cfg_valid(prec, cfg)) return hipErrorInvalidValue;
+  if (!t3_gemm_cfg_valid(prec, cfg) || nblocks >= (1LL << 31)) return hipErrorInvalidValue;
   const dim3 grid((unsigned)nblocks);
   if (prec == PREC_F64) launch_cfg<PREC_F64>(cfg, grid, stream, probs, nprob, segs);
   else if (prec == PREC_F32) launch_cfg<PREC_F32>(cfg, grid, stream, probs, nprob, segs);
