@@ -345,6 +345,8 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
     for (int t = 0; t < a.ntriple; t++) {
       const EpiTriple& tr = a.triples[t];
       double q = 0.0;
+      // unrolled: eight loads in flight, the sum keeps its order
+#pragma unroll 8
       for (int c = tid; c < a.ncube; c += 256) q += tr.part[(long long)c * EPI_PSTRIDE];
       e += 2.0 * tr.fo * block_sum(q);
     }
@@ -353,6 +355,7 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
     for (int t = 0; t < a.ntriple; t++) {
       const EpiTriple& tr = a.triples[t];
       double r = 0.0;
+#pragma unroll 8
       for (int c = tid; c < a.ncube; c += 256) r += tr.part[(long long)c * EPI_PSTRIDE + 1];
       r = block_sum(r);
       if (tid == 0) {
@@ -372,6 +375,7 @@ __global__ void __launch_bounds__(256) epi_reduce(EpiReduceArgs a) {
     const double* part = a.triples[t].part;
     double acc[EPI_T];
     for (int l = 0; l < EPI_T; l++) acc[l] = 0.0;
+#pragma unroll 2
     for (int pr = tid; pr < npair; pr += 256) {
       // pair pr = (X, Y), Y <= X, in the order X(X+1)/2 + Y
       int X = (int)((sqrtf(8.0f * (float)pr + 1.0f) - 1.0f) * 0.5f);

@@ -284,26 +284,43 @@ vpart_vecsum(const VpTriple* __restrict__ trs, int V) {
 // (p1, p2) = (t_e1, t_e2), e1 < e2 the positions other than d.  They sum the
 // two orderings with the same contracted position in ccsd_t.c:149-150 and
 // ccsd_t_vjp.c:171-204, 228-245.
+// A block builds a 32 x 32 (x, y) tile: the term read transposed (contiguous
+// in x) goes through LDS, so both reads and the write are coalesced.
 __global__ void __launch_bounds__(256)
 vcm_build(const VpTriple* __restrict__ trs, int o, int v, int V, const double* __restrict__ ovov,
           const double* __restrict__ t2) {
+  __shared__ double tt[32][33];
   const VpTriple tr = trs[blockIdx.y / 6];
   const int task = blockIdx.y % 6;
   const long long VV = (long long)V * V;
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= VV) return;
-  const int x = (int)(e / V), y = (int)(e % V);
+  const int tiles = (V + 31) / 32;
+  const int x0 = (int)(blockIdx.x / tiles) * 32, y0 = (int)(blockIdx.x % tiles) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
   const int d = task / 2, which = task % 2;
   const int occ[3] = {tr.i, tr.j, tr.k};
   const int p1 = occ[d == 0 ? 1 : 0], p2 = occ[d == 2 ? 1 : 2];
-  double val = 0.0;
-  if (x < v && y < v) {
-    if (which == 0)
-      val = ovov[(((long long)p1 * v + x) * o + p2) * v + y] + ovov[(((long long)p2 * v + y) * o + p1) * v + x];
-    else
-      val = t2[(((long long)p2 * o + p1) * v + y) * v + x] + t2[(((long long)p1 * o + p2) * v + x) * v + y];
+  // transposed term at (x, y) = x0 + tx, y0 + ty + 8 k, stored as tt[y - y0][x - x0]
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int x = x0 + tx, y = y0 + ty + 8 * k;
+    double b = 0.0;
+    if (x < v && y < v)
+      b = which == 0 ? ovov[(((long long)p2 * v + y) * o + p1) * v + x] : t2[(((long long)p2 * o + p1) * v + y) * v + x];
+    tt[ty + 8 * k][tx] = b;
   }
-  tr.out[(VP_CM + task) * VV + e] = val;
+  __syncthreads();
+  // direct term and write at (x, y) = x0 + ty + 8 k, y0 + tx
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int x = x0 + ty + 8 * k, y = y0 + tx;
+    if (x >= V || y >= V) continue;
+    double val = 0.0;
+    if (x < v && y < v)
+      val = (which == 0 ? ovov[(((long long)p1 * v + x) * o + p2) * v + y]
+                        : t2[(((long long)p1 * o + p2) * v + x) * v + y]) +
+            tt[tx][ty + 8 * k];
+    tr.out[(VP_CM + task) * VV + (long long)x * V + y] = val;
+  }
 }
 
 // Scatter of the V-term adjoints of one batch.  Slab tasks (blockIdx.y <
@@ -487,8 +504,8 @@ extern "C" hipError_t t3_launch_vpart(int tp_is_f64, const VpTriple* trs, int nt
 extern "C" hipError_t t3_launch_vcm(const VpTriple* trs, int ntriple, int o, int v, int V, const double* ovov,
                                     const double* t2, hipStream_t s) {
   if (ntriple <= 0) return hipSuccess;
-  const long long VV = (long long)V * V;
-  dim3 g((unsigned)((VV + 255) / 256), 6 * ntriple);
+  const int tiles = (V + 31) / 32;
+  dim3 g((unsigned)(tiles * tiles), 6 * ntriple);
   hipLaunchKernelGGL(vcm_build, g, dim3(256), 0, s, trs, o, v, V, ovov, t2);
   return hipGetLastError();
 }
