@@ -517,6 +517,20 @@ struct Ctx {
       const long long cap = per_split ? (long long)(room / per_split) : ksplit;
       ksplit = (int)std::max<long long>(1, std::min<long long>(ksplit, cap));
     }
+    // Problems merge all the triples of a batch that write one output, so
+    // their K differ: each problem is split into about as many chunks per
+    // split as the largest one gets, so that the blocks carry equal work (a
+    // uniform split count would leave the long problems' blocks as the tail).
+    // The hole-side adjoints keep one split count when all their problems
+    // share one tile grid: their tiles are then interleaved (below), which
+    // measured worth more than the balance.
+    int max_chunks_all = 1;
+    for (const auto& P : gb.probs) max_chunks_all = std::max(max_chunks_all, P.nchunks);
+    bool one_grid = (sub == FAM_GEMM_B3 || sub == FAM_GEMM_B4) && gb.probs.size() > 1;
+    for (const auto& P : gb.probs)
+      one_grid = one_grid && P.nb == gb.probs[0].nb && P.tiles_m == gb.probs[0].tiles_m &&
+                 P.tiles_n == gb.probs[0].tiles_n;
+    const int cps = one_grid ? max_chunks_all : (max_chunks_all + ksplit - 1) / ksplit;
     // operand load modes; every base and stride must keep VEC-element vectors aligned
     const long long vec = t3_gemm_vec(gb.prec);
     const long long vbytes = vec * (long long)elt;
@@ -552,7 +566,7 @@ struct Ctx {
       P.nseg = (int)gb.segs[p].size();
       for (auto& s : gb.segs[p]) flat.push_back(s);
       P.c_is_f64 = c_is_f64;
-      P.ksplit = allow_split ? ksplit : 1;
+      P.ksplit = !allow_split ? 1 : one_grid ? ksplit : std::max(1, std::min(ksplit, (P.nchunks + cps - 1) / cps));
       P.block0 = nblocks;
       nblocks += (long long)P.nb * P.tiles_m * P.tiles_n * P.ksplit;
       if (nblocks >= (1LL << 31) && plan_error.empty()) plan_error = "GEMM launch exceeds 2^31 tiles";
