@@ -6,7 +6,10 @@
 // row vectors for row-contiguous operands) and the furthest element its
 // output or split partials can reach lie inside the workspace region or the
 // caller's in-place outputs; likewise every split-K reduction (and, for the
-// vectorised reductions, the alignment and size they assume).
+// vectorised reductions, the alignment and size they assume).  Also checks
+// that interleaved launches have one tile grid and that no split is empty,
+// and counts the interleaved launches and the problems with their own split
+// count.
 // Usage: plan_check nocc nvir precision [naux [plant]]
 //   plant: moves the split-K scratch to the end of the workspace, so that a
 //   correct checker must report out-of-bounds partials.
@@ -30,7 +33,7 @@ int main(int argc, char** argv) {
   const uintptr_t WS = 1ull << 40;
   const size_t ws_size = c.r.desc;   // regions end where descriptors start
   const size_t t2_size = (size_t)o * o * v * v * 8;
-  int bad = 0, nl = 0;
+  int bad = 0, nl = 0, ninter = 0, nsplit = 0, nvaried = 0;
   unsigned cfgs = 0;   // bit c: some GEMM launch uses tile configuration c
   auto check = [&](uintptr_t lo, uintptr_t hi, const char* what, size_t li, int p) {
     bool ok = (lo >= WS && hi <= WS + ws_size) || (lo >= OUT_T2 && hi <= OUT_T2 + t2_size) ||
@@ -72,9 +75,26 @@ int main(int argc, char** argv) {
     const GemmProb* P = reinterpret_cast<const GemmProb*>(c.blob.data.data() + L.off_a);
     const GemmSeg* S = reinterpret_cast<const GemmSeg*>(c.blob.data.data() + L.off_b);
     long long blocks = 0;
+    if (P[0].interleave) ninter++;
     for (int p = 0; p < L.n_a; p++) {
       const GemmProb& Q = P[p];
       blocks += (long long)Q.nb * Q.tiles_m * Q.tiles_n * Q.ksplit;
+      // interleaved tiles (block t = tile t / nprob of problem t % nprob) need one tile grid
+      if (Q.interleave != P[0].interleave ||
+          (P[0].interleave && (Q.nb != P[0].nb || Q.tiles_m != P[0].tiles_m || Q.tiles_n != P[0].tiles_n ||
+                               Q.ksplit != P[0].ksplit))) {
+        printf("launch %zu prob %d: interleaved problems with different tile grids\n", li, p);
+        bad++;
+      }
+      // every split of a problem holds at least one K-chunk
+      if (Q.ksplit < 1 || (Q.nchunks > 0 && Q.ksplit > Q.nchunks)) {
+        printf("launch %zu prob %d: %d splits of %d chunks\n", li, p, Q.ksplit, Q.nchunks);
+        bad++;
+      }
+      if (Q.ksplit > 1) {
+        nsplit++;
+        if (Q.ksplit != P[0].ksplit) nvaried++;
+      }
       for (int si = Q.seg0; si < Q.seg0 + Q.nseg; si++) {
         const GemmSeg& g = S[si];
         // rows: loads clamp to M-1 (KCONT) or to the last full vector block (ROWCONT)
@@ -99,8 +119,8 @@ int main(int argc, char** argv) {
     }
     if (blocks != L.work) { printf("launch %zu: block count %lld != work %lld\n", li, blocks, L.work); bad++; }
   }
-  printf("o=%d v=%d prec=%d naux=%d B=%d launches=%d gemm, bad=%d, plan_error='%s' cfgs=", o, v, prec, naux, c.B, nl,
-         bad, c.plan_error.c_str());
+  printf("o=%d v=%d prec=%d naux=%d B=%d launches=%d gemm, interleaved=%d, split problems=%d (%d with their own count), "
+         "bad=%d, plan_error='%s' cfgs=", o, v, prec, naux, c.B, nl, ninter, nsplit, nvaried, bad, c.plan_error.c_str());
   for (int k = 0; k < 32; k++)
     if (cfgs >> k & 1) printf("%d,", k);
   printf("\n");

@@ -8,6 +8,7 @@ of the last build.  Covers the dense and the density-fitted plans, every
 precision, uneven sizes, batch merging (many triples per output slab) and
 small occupied blocks (cache evictions)."""
 import os
+import re
 import shutil
 import subprocess
 
@@ -35,10 +36,12 @@ def checker(tmp_path_factory):
     shutil.rmtree(d, ignore_errors=True)
 
 
-def _run(exe, *args, block=None, tile=None):
+def _run(exe, *args, block=None, tile=None, batch=None):
     env = dict(os.environ)
     env.pop("PYSCFAD_AMD_BATCH", None)
     env.pop("PYSCFAD_AMD_TILE", None)
+    if batch is not None:
+        env["PYSCFAD_AMD_BATCH"] = str(batch)
     if block is not None:
         env["PYSCFAD_AMD_DF_BLOCK"] = str(block)
     if tile is not None:
@@ -61,6 +64,16 @@ def test_plan_in_bounds(checker, o, v, prec, naux, block):
     r = _run(checker, o, v, prec, naux, block=block)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "bad=0" in r.stdout and "plan_error=''" in r.stdout, r.stdout
+
+
+def test_plan_interleave_and_split_counts(checker):
+    """A two-triple batch plan interleaves its hole-side adjoints and gives
+    merged problems (twice the K) their own split count."""
+    r = _run(checker, 12, 100, 1, 0, batch=2)
+    assert r.returncode == 0 and "bad=0" in r.stdout, r.stdout + r.stderr
+    m = re.search(r"interleaved=(\d+), split problems=(\d+) \((\d+) with", r.stdout)
+    assert m, r.stdout
+    assert int(m.group(1)) > 0 and int(m.group(2)) > 0 and int(m.group(3)) > 0, r.stdout
 
 
 def test_checker_reports_planted_overflow(checker):
