@@ -114,10 +114,11 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
   const int occ[3] = {tr.i, tr.j, tr.k};
   const long long VV = (long long)V * V;
   double* out = tr.out;
+  // passes 0, 1: per-wave partial sums; pass 2: staged rows (red[0])
+  __shared__ __attribute__((aligned(32))) double red[4][4][256];
   if (pass < 2) {
     // four consecutive elements per lane, read with one vector load
     typedef TP tp4 __attribute__((ext_vector_type(4)));
-    __shared__ double red[4][4][256];
     const int el = threadIdx.x & 63, sp = threadIdx.x >> 6;
     const long long e0 = ((long long)blockIdx.x * 64 + el) * 4;
     const bool live = e0 < VV;   // VV is a multiple of 4: all four or none
@@ -184,42 +185,59 @@ vpart_contract(const VpTriple* __restrict__ trs, int o, int v, int V, const doub
     return;
   }
   // pass 2: 16 lanes per (a, b) row, four rows per wave; a lane takes four
-  // consecutive c per step with one vector load of H
+  // consecutive c per step with one vector load of H.  The block's 16 rows
+  // share a (V is a multiple of 32), so the rows every one of them needs
+  // (CM1[a], CN1[a], fvo, t1) are staged once per 256 c in LDS.
   typedef TP tp4 __attribute__((ext_vector_type(4)));
+  double (*srow)[256] = red[0];
   const int lane = threadIdx.x & 63, l16 = lane & 15;
   const long long row = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + (lane >> 4);
   const bool live = row < VV;
   const long long rw = live ? row : 0;
-  const int a = (int)(rw / V), b = (int)(rw % V);
+  const int b = (int)(rw % V);
+  const int ablk = (int)(((long long)blockIdx.x * 16) / V);   // a of all 16 rows
   const int t2o = occ[2];
   const TP* h = H + rw * V;
-  const double* CM1 = out + (VP_CM + 2) * VV + (long long)a * V;
-  const double* CN1 = out + (VP_CM + 3) * VV + (long long)a * V;
+  const double* CM1 = out + (VP_CM + 2) * VV + (long long)ablk * V;
+  const double* CN1 = out + (VP_CM + 3) * VV + (long long)ablk * V;
   const double* CM0 = out + (VP_CM + 0) * VV + (long long)b * V;
   const double* CN0 = out + (VP_CM + 1) * VV + (long long)b * V;
   const double* t1r = t1 + (long long)t2o * v;
   const double* fvr = fvoP + (long long)t2o * V;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  // the fvo and pair-matrix rows are V doubles (a multiple of 32) from
-  // 256-byte aligned bases: 32-byte vector loads
+  // the pair-matrix rows are V doubles (a multiple of 32) from 256-byte
+  // aligned bases: 32-byte vector loads
   typedef double dv4 __attribute__((ext_vector_type(4)));
-#pragma unroll 4
-  for (int c0 = l16 * 4; c0 < v; c0 += 64) {   // c0 + 3 < V: V is a multiple of 32
-    const tp4 hv = *reinterpret_cast<const tp4*>(h + c0);
-    const dv4 fv = *reinterpret_cast<const dv4*>(fvr + c0);
-    const dv4 m1 = *reinterpret_cast<const dv4*>(CM1 + c0), n1 = *reinterpret_cast<const dv4*>(CN1 + c0);
-    const dv4 m0 = *reinterpret_cast<const dv4*>(CM0 + c0), n0 = *reinterpret_cast<const dv4*>(CN0 + c0);
+  for (int cc = 0; cc < v; cc += 256) {
+    __syncthreads();   // the previous chunk's LDS reads are done
+    {
+      const int c = cc + (int)threadIdx.x;
+      srow[0][threadIdx.x] = c < V ? CM1[c] : 0.0;
+      srow[1][threadIdx.x] = c < V ? CN1[c] : 0.0;
+      srow[2][threadIdx.x] = c < V ? fvr[c] : 0.0;
+      srow[3][threadIdx.x] = c < v ? t1r[c] : 0.0;
+    }
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int c = c0 + q;
-      if (c < v) {
-        const double x = (double)hv[q];
-        acc[0] += x * t1r[c];
-        acc[1] += x * fv[q];
-        acc[2] += x * m1[q];
-        acc[3] += x * n1[q];
-        acc[4] += x * m0[q];
-        acc[5] += x * n0[q];
+    for (int k = 0; k < 4; k++) {
+      const int cl = l16 * 4 + 64 * k, c0 = cc + cl;   // c0 + 3 < V: V is a multiple of 32
+      if (c0 >= v) break;
+      const tp4 hv = *reinterpret_cast<const tp4*>(h + c0);
+      const dv4 m0 = *reinterpret_cast<const dv4*>(CM0 + c0), n0 = *reinterpret_cast<const dv4*>(CN0 + c0);
+      const dv4 m1 = *reinterpret_cast<const dv4*>(&srow[0][cl]), n1 = *reinterpret_cast<const dv4*>(&srow[1][cl]);
+      const dv4 fv = *reinterpret_cast<const dv4*>(&srow[2][cl]), tv = *reinterpret_cast<const dv4*>(&srow[3][cl]);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int c = c0 + q;
+        if (c < v) {
+          const double x = (double)hv[q];
+          acc[0] += x * tv[q];
+          acc[1] += x * fv[q];
+          acc[2] += x * m1[q];
+          acc[3] += x * n1[q];
+          acc[4] += x * m0[q];
+          acc[5] += x * n0[q];
+        }
       }
     }
   }
