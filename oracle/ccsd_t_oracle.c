@@ -5,8 +5,7 @@
  * kernels are compared against, and the timed CPU baseline of the bench.
  * The product path (pyscfad_amd) never links or calls it.
  *
- * It restates, in plain C99 + OpenMP with its own loops (no BLAS), the
- * algorithm of the reference:
+ * It restates, in C99 + OpenMP, the algorithm of the reference:
  *   - virtual-triple jobs a >= b >= c          (pyscfadlib/.../vjp/cc/ccsd_t.c:6-72)
  *   - the six (p,q,r) orderings of (a,b,c) and the matching occupied index
  *     permutations sigma_s                     (ccsd_t_vjp.c:32-37, ccsd_t.c:74-96)
@@ -17,21 +16,71 @@
  *                                               factor 2 as in ccsd_t_slow.py:124)
  *   - the reverse pass                         (ccsd_t_vjp.c:55-117, 139-269, 271-329)
  *
+ * The four products of every ordering (the forward particle and hole terms,
+ * ccsd_t.c:137-142, and the adjoints of ccsd_t_vjp.c:139-167, 205-226) are
+ * column-major GEMM calls, as the reference makes them.  They go to a
+ * caller-supplied Fortran-ABI dgemm (oracle_set_dgemm; the Python side hands
+ * over the BLAS that scipy ships) or, without one, to the plain triple loop
+ * `gemm_loops` below.  The tests check the two against each other and the
+ * loop form against closed forms and finite differences.
+ *
  * Array conventions are the reference's host arrays (ccsd_t.py:88-106):
  *   t1T[a,i]        = t1[i,a]                       shape (v,o)
  *   t2T[a,b,i,j]    = t2[j,i,a,b]                   shape (v,v,o,o)
  *   vooo[a,i,j,k]   = ovoo[i,a,k,j]                 shape (v,o,o,o)
  *   fvo[a,i]        = fock[o+a,i]                   shape (v,o)
- *   vvop[a,b,i,j]   = ovov[i,a,j,b]   (j <  o)      shape (v,v,o,o+v)
+ *   vvop[a,b,i,j]   = ovov[i,a,j,b]   (j <  o)      shape (w,w,o,o+v)
  *   vvop[a,b,i,o+f] = ovvv[i,a,f,b]   (unpacked)
- * Outputs are accumulated (+=) into caller-zeroed buffers of the same shapes,
- * as the reference driver does (ccsd_t.py:89-106).
+ * where w (`vrow`) is the number of virtual indices that the job slots a, b,
+ * c may take: w = v for a whole instance, or w < v for a bounded sample made
+ * of the first jobs only (jobs [0, v'(v'+1)(v'+2)/6) have a, b, c < v'), whose
+ * host arrays then need only the first w rows of t1T, t2T, vooo, fvo and the
+ * w x w leading block of vvop.  Outputs are accumulated (+=) into
+ * caller-zeroed buffers of the same shapes, as the reference driver does
+ * (ccsd_t.py:89-106).
  */
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+/* Fortran-ABI dgemm (column-major), the signature of BLAS's dgemm_ */
+typedef void (*dgemm_fn)(const char *ta, const char *tb, const int *m, const int *n, const int *k,
+                         const double *alpha, const double *a, const int *lda, const double *b,
+                         const int *ldb, const double *beta, double *c, const int *ldc);
+static dgemm_fn g_dgemm = NULL;
+
+/* Route the oracle's products through `fn` (NULL: back to the plain loops).
+ * `fn` must be safe to call from several threads at once (a BLAS limited to
+ * one thread of its own). */
+void oracle_set_dgemm(void *fn) { g_dgemm = (dgemm_fn)fn; }
+
+/* C = beta C + alpha op(A) op(B), column-major, op = identity ('N') or transpose ('T') */
+static void gemm_loops(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                       const double *B, int ldb, double beta, double *C, int ldc)
+{
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            double s = 0.0;
+            for (int l = 0; l < k; l++) {
+                const double x = (ta == 'N') ? A[i + (size_t)l * lda] : A[l + (size_t)i * lda];
+                const double y = (tb == 'N') ? B[l + (size_t)j * ldb] : B[j + (size_t)l * ldb];
+                s += x * y;
+            }
+            double *c = C + i + (size_t)j * ldc;
+            *c = (beta == 0.0 ? 0.0 : beta * *c) + alpha * s;
+        }
+}
+
+static void gemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc)
+{
+    if (g_dgemm)
+        g_dgemm(&ta, &tb, &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C, &ldc);
+    else
+        gemm_loops(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
 
 /* (p,q,r) for s = 0..5 as positions into (a,b,c): (abc),(acb),(bac),(bca),(cab),(cba) */
 static const int PERM_ABC[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -68,49 +117,73 @@ static void r3_apply(double *out, const double *x, int o)
 }
 
 typedef struct {
-    int o, v, n;
+    int o, v, n, w;
     const double *mo_energy, *t1T, *t2T, *vooo, *fvo, *vvop;
 } T3In;
 
-/* W_s and V_s of one ordering (p,q,r), scattered into w0/v0 through sigma_s.
- * W[i,j,k] = sum_f vvop[p,q,i,o+f] t2T[r,f,j,k] - sum_m vooo[p,i,j,m] t2T[r,q,m,k]
- * V[i,j,k] = 1/2 vvop[p,q,i,j] t1T[r,k] + 1/2 t2T[q,p,i,j] fvo[r,k]          */
-static void build_wv(const T3In *in, int p, int q, int r, int s,
-                     double *w0, double *v0, double *wtmp)
+static inline const double *vvop_block(const T3In *in, int p, int q)
+{
+    return in->vvop + ((size_t)p * in->w + q) * in->o * in->n;   /* vvop[p,q] : (o, n) */
+}
+
+/* The orderings s come in pairs with the same r (the t2T row of the
+ * particle term): (0,2) r = c, (1,4) r = b, (3,5) r = a.  The particle
+ * products of a pair are made as one product with the pair's vvop columns
+ * side by side, so t2T[r] is read once per pair. */
+static const int PAIRS[3][2] = {{0, 2}, {1, 4}, {3, 5}};
+
+/* pack[f + v*(x*o + i)] = vvop[p_x,q_x,i,o+f] for the two orderings x = 0, 1 of a pair */
+static void pack_pair(const T3In *in, const double *pq0, const double *pq1, double *pack)
 {
     const int o = in->o, v = in->v, n = in->n;
-    const size_t oo = (size_t)o * o, ooo = oo * o, voo = (size_t)v * oo;
-    const double *pq = in->vvop + ((size_t)p * v + q) * o * n;   /* vvop[p,q] : (o, n) */
-    const double *t2r = in->t2T + (size_t)r * voo;                /* t2T[r] : (v, o, o) */
-    const double *t2rq = t2r + (size_t)q * oo;                    /* t2T[r,q] : (o, o) */
-    const double *voop = in->vooo + (size_t)p * ooo;             /* vooo[p] : (o,o,o) */
-    memset(wtmp, 0, sizeof(double) * ooo);
-    for (int i = 0; i < o; i++) {
-        double *wi = wtmp + i * oo;
-        for (int f = 0; f < v; f++) {
-            const double x = pq[(size_t)i * n + o + f];
-            const double *t = t2r + (size_t)f * oo;
-            for (size_t jk = 0; jk < oo; jk++) wi[jk] += x * t[jk];
-        }
-        for (int j = 0; j < o; j++) {
-            double *wij = wi + j * o;
-            for (int m = 0; m < o; m++) {
-                const double x = voop[(i * o + j) * o + m];
-                const double *t = t2rq + (size_t)m * o;
-                for (int k = 0; k < o; k++) wij[k] -= x * t[k];
-            }
-        }
+    for (int x = 0; x < 2; x++) {
+        const double *pq = x ? pq1 : pq0;
+        for (int i = 0; i < o; i++)
+            memcpy(pack + (size_t)v * (x * o + i), pq + (size_t)i * n + o, sizeof(double) * v);
     }
-    const double *t2qp = in->t2T + ((size_t)q * v + p) * oo;     /* t2T[q,p] : (o,o) */
-    const double *t1r = in->t1T + (size_t)r * o;
-    const double *fr = in->fvo + (size_t)r * o;
-    for (int i = 0; i < o; i++)
-        for (int j = 0; j < o; j++)
-            for (int k = 0; k < o; k++) {
-                size_t d = sigma_index(s, i, j, k, o);
-                w0[d] += wtmp[(i * o + j) * o + k];
-                v0[d] += 0.5 * pq[(size_t)i * n + j] * t1r[k] + 0.5 * t2qp[i * o + j] * fr[k];
-            }
+}
+
+/* W_s and V_s of the two orderings (p,q,r) of a pair, scattered into w0/v0
+ * through sigma_s:
+ * W[i,j,k] = sum_f vvop[p,q,i,o+f] t2T[r,f,j,k] - sum_m vooo[p,i,j,m] t2T[r,q,m,k]
+ * V[i,j,k] = 1/2 vvop[p,q,i,j] t1T[r,k] + 1/2 t2T[q,p,i,j] fvo[r,k]
+ * As column-major products (row-major W[i][jk] is the column-major oo x o
+ * matrix Wc[jk,i]): [Wc_0 Wc_1] = t2T[r] (oo x v) . pack (v x 2o), then per
+ * ordering W[ij][k] -= sum_m vooo[p][ij][m] t2T[r,q][m][k] as the o x oo
+ * product t2T[r,q] (o x o) . vooo[p] (o x oo). */
+static void build_wv_pair(const T3In *in, const int *abc, int pr, double *w0, double *v0,
+                          double *wtmp, double *pack)
+{
+    const int o = in->o, v = in->v, n = in->n, oo = o * o;
+    const size_t voo = (size_t)v * oo, ooo = (size_t)oo * o;
+    const int r = abc[PERM_ABC[PAIRS[pr][0]][2]];
+    const double *t2r = in->t2T + (size_t)r * voo;                /* t2T[r] : (v, o, o) */
+    const double *pqs[2];
+    for (int x = 0; x < 2; x++) {
+        const int s = PAIRS[pr][x];
+        pqs[x] = vvop_block(in, abc[PERM_ABC[s][0]], abc[PERM_ABC[s][1]]);
+    }
+    pack_pair(in, pqs[0], pqs[1], pack);
+    gemm('N', 'N', oo, 2 * o, v, 1.0, t2r, oo, pack, v, 0.0, wtmp, oo);
+    for (int x = 0; x < 2; x++) {
+        const int s = PAIRS[pr][x];
+        const int p = abc[PERM_ABC[s][0]], q = abc[PERM_ABC[s][1]];
+        double *wt = wtmp + (size_t)x * ooo;
+        const double *t2rq = t2r + (size_t)q * oo;                /* t2T[r,q] : (o, o) */
+        const double *voop = in->vooo + (size_t)p * ooo;         /* vooo[p] : (o,o,o) */
+        gemm('N', 'N', o, oo, o, -1.0, t2rq, o, voop, o, 1.0, wt, o);
+        const double *pq = pqs[x];
+        const double *t2qp = in->t2T + ((size_t)q * v + p) * oo; /* t2T[q,p] : (o,o) */
+        const double *t1r = in->t1T + (size_t)r * o;
+        const double *fr = in->fvo + (size_t)r * o;
+        for (int i = 0; i < o; i++)
+            for (int j = 0; j < o; j++)
+                for (int k = 0; k < o; k++) {
+                    size_t d = sigma_index(s, i, j, k, o);
+                    w0[d] += wt[((size_t)i * o + j) * o + k];
+                    v0[d] += 0.5 * pq[(size_t)i * n + j] * t1r[k] + 0.5 * t2qp[i * o + j] * fr[k];
+                }
+    }
 }
 
 static inline double triple_weight(int a, int b, int c)
@@ -120,6 +193,13 @@ static inline double triple_weight(int a, int b, int c)
     return 1.0;
 }
 
+/* doubles of scratch job_w_z needs after w0 and z0 */
+static size_t job_buf_len(const T3In *in)
+{
+    const size_t ooo = (size_t)in->o * in->o * in->o;
+    return 3 * ooo + (size_t)2 * in->v * in->o;
+}
+
 /* w0 and z0 = r3(w0+v0) of the virtual triple (a,b,c), a >= b >= c. */
 static void job_w_z(const T3In *in, int a, int b, int c,
                     double *w0, double *z0, double *buf)
@@ -127,11 +207,11 @@ static void job_w_z(const T3In *in, int a, int b, int c,
     const int o = in->o;
     const size_t ooo = (size_t)o * o * o;
     const int abc[3] = {a, b, c};
-    double *v0 = buf, *wtmp = buf + ooo;
+    double *v0 = buf, *wtmp = buf + ooo, *pack = wtmp + 2 * ooo;
     memset(w0, 0, sizeof(double) * ooo);
     memset(v0, 0, sizeof(double) * ooo);
-    for (int s = 0; s < 6; s++)
-        build_wv(in, abc[PERM_ABC[s][0]], abc[PERM_ABC[s][1]], abc[PERM_ABC[s][2]], s, w0, v0, wtmp);
+    for (int pr = 0; pr < 3; pr++)
+        build_wv_pair(in, abc, pr, w0, v0, wtmp, pack);
     for (size_t x = 0; x < ooo; x++) v0[x] += w0[x];
     r3_apply(z0, v0, o);
 }
@@ -151,24 +231,30 @@ static void job_decode(long m, int *pa, int *pb, int *pc)
     *pa = a; *pb = b; *pc = (int)m;
 }
 
+/* largest job count whose slots stay below w */
+static long clamp_jobs(long job1, int w)
+{
+    return job1 > njobs_total(w) ? njobs_total(w) : job1;
+}
+
 /*
  * E(T) over the job range [job0, job1) of the a>=b>=c enumeration (the full
  * energy is job0 = 0, job1 = v(v+1)(v+2)/6).  A sub-range is what the bench
  * times as a bounded CPU sample.  Returns the partial E(T) (already x2).
  */
-double oracle_ccsd_t_energy_range(int nocc, int nvir, const double *mo_energy,
+double oracle_ccsd_t_energy_range(int nocc, int nvir, int vrow, const double *mo_energy,
                                   const double *t1T, const double *t2T,
                                   const double *vooo, const double *fvo,
                                   const double *vvop, long job0, long job1)
 {
-    T3In in = {nocc, nvir, nocc + nvir, mo_energy, t1T, t2T, vooo, fvo, vvop};
+    T3In in = {nocc, nvir, nocc + nvir, vrow, mo_energy, t1T, t2T, vooo, fvo, vvop};
     const int o = nocc;
     const size_t ooo = (size_t)o * o * o;
-    if (job1 > njobs_total(nvir)) job1 = njobs_total(nvir);
+    job1 = clamp_jobs(job1, vrow);
     double et = 0.0;
 #pragma omp parallel reduction(+ : et)
     {
-        double *w0 = malloc(sizeof(double) * ooo * 4);
+        double *w0 = malloc(sizeof(double) * (2 * ooo + job_buf_len(&in)));
         double *z0 = w0 + ooo, *buf = z0 + ooo;
 #pragma omp for schedule(dynamic, 4)
         for (long m = job0; m < job1; m++) {
@@ -192,46 +278,37 @@ double oracle_ccsd_t_energy_range(int nocc, int nvir, const double *mo_energy,
     return 2.0 * et;
 }
 
-double oracle_ccsd_t_energy(int nocc, int nvir, const double *mo_energy,
-                            const double *t1T, const double *t2T,
-                            const double *vooo, const double *fvo, const double *vvop)
-{
-    return oracle_ccsd_t_energy_range(nocc, nvir, mo_energy, t1T, t2T, vooo, fvo, vvop,
-                                      0, njobs_total(nvir));
-}
-
 /*
  * Reverse pass.  et_bar is the cotangent of E(T); the reference doubles it
  * first (ccsd_t.py:86), which is done here too.  All outputs are accumulated.
- * Small cotangents (mo_energy, t1T, fvo, vooo) use per-thread private copies
- * reduced at the end, as the reference does (ccsd_t_vjp.c:379-493).  The two
- * large ones (vvop_bar, t2T_bar) are accumulated in place under one lock per
- * (p,q) block of vvop and one per leading index of t2T, which keeps memory
- * bounded at the sizes the bench samples.
+ * Every cotangent except vvop_bar is accumulated in per-thread private
+ * copies reduced in thread order at the end, as the reference does
+ * (ccsd_t_vjp.c:379-493).  vvop_bar is accumulated per (p,q) block through a
+ * job-local buffer flushed under one lock per block.  e_out (may be NULL)
+ * receives the E(T) of the same jobs, from the w0, z0 and d3 the reverse pass
+ * forms anyway.
  */
-void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, const double *mo_energy,
+void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, int vrow, const double *mo_energy,
                                     const double *t1T, const double *t2T,
                                     const double *vooo, const double *fvo, const double *vvop,
                                     double et_bar,
                                     double *mo_energy_bar, double *t1T_bar, double *t2T_bar,
                                     double *vooo_bar, double *fvo_bar, double *vvop_bar,
-                                    long job0, long job1)
+                                    long job0, long job1, double *e_out)
 {
-    T3In in = {nocc, nvir, nocc + nvir, mo_energy, t1T, t2T, vooo, fvo, vvop};
-    const int o = nocc, v = nvir, n = nocc + nvir;
-    const size_t oo = (size_t)o * o, ooo = oo * o, voo = (size_t)v * oo;
-    const size_t len_me = n, len_t1 = (size_t)v * o;
-    const size_t len_vooo = (size_t)v * ooo, len_fvo = (size_t)v * o;
-    const size_t len_priv = len_me + len_t1 + len_vooo + len_fvo;
+    T3In in = {nocc, nvir, nocc + nvir, vrow, mo_energy, t1T, t2T, vooo, fvo, vvop};
+    const int o = nocc, v = nvir, n = nocc + nvir, w = vrow, oo = nocc * nocc;
+    const size_t ooo = (size_t)oo * o, voo = (size_t)v * oo;
+    const size_t len_me = n, len_t1 = (size_t)w * o;
+    const size_t len_vooo = (size_t)w * ooo, len_fvo = (size_t)w * o, len_t2 = (size_t)w * voo;
+    const size_t len_priv = len_me + len_t1 + len_vooo + len_fvo + len_t2;
     const double ebar = 2.0 * et_bar;
-    if (job1 > njobs_total(v)) job1 = njobs_total(v);
+    job1 = clamp_jobs(job1, w);
     int nthreads = 1;
 #ifdef _OPENMP
     nthreads = omp_get_max_threads();
-    omp_lock_t *lk_pq = malloc(sizeof(omp_lock_t) * (size_t)v * v);
-    omp_lock_t *lk_r = malloc(sizeof(omp_lock_t) * (size_t)v);
-    for (size_t x = 0; x < (size_t)v * v; x++) omp_init_lock(&lk_pq[x]);
-    for (int x = 0; x < v; x++) omp_init_lock(&lk_r[x]);
+    omp_lock_t *lk_pq = malloc(sizeof(omp_lock_t) * (size_t)w * w);
+    for (size_t x = 0; x < (size_t)w * w; x++) omp_init_lock(&lk_pq[x]);
 #define LOCK(l) omp_set_lock(&(l))
 #define UNLOCK(l) omp_unset_lock(&(l))
 #else
@@ -239,7 +316,8 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, const double *mo_energy,
 #define UNLOCK(l)
 #endif
     double **priv = calloc(nthreads, sizeof(double *));
-#pragma omp parallel
+    double etot = 0.0;
+#pragma omp parallel reduction(+ : etot)
     {
         int tid = 0;
 #ifdef _OPENMP
@@ -248,14 +326,13 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, const double *mo_energy,
         double *acc = calloc(len_priv, sizeof(double));
         priv[tid] = acc;
         double *me_b = acc, *t1_b = me_b + len_me;
-        double *vooo_b = t1_b + len_t1, *fvo_b = vooo_b + len_vooo;
-        double *w0 = malloc(sizeof(double) * ooo * 8);
+        double *vooo_b = t1_b + len_t1, *fvo_b = vooo_b + len_vooo, *t2_b = fvo_b + len_fvo;
+        double *w0 = malloc(sizeof(double) * ooo * 10);
         double *z0 = w0 + ooo, *d3 = z0 + ooo, *zb = d3 + ooo, *wb0 = zb + ooo, *vb0 = wb0 + ooo;
-        double *wbar = vb0 + ooo, *vbar = wbar + ooo;
-        double *buf = malloc(sizeof(double) * ooo * 2);
-        double *pqb_loc = malloc(sizeof(double) * o * n);
-        double *t2b_loc = malloc(sizeof(double) * voo);
-        double *t2qp_loc = malloc(sizeof(double) * oo);
+        double *wbar = vb0 + ooo, *vbar = wbar + 2 * ooo;            /* two orderings each */
+        double *buf = malloc(sizeof(double) * job_buf_len(&in));
+        double *pack = malloc(sizeof(double) * (size_t)4 * v * o);   /* pack, then particle result */
+        double *pqb_loc = malloc(sizeof(double) * 2 * o * n);
 #pragma omp for schedule(dynamic, 4)
         for (long m = job0; m < job1; m++) {
             int a, b, c;
@@ -264,119 +341,107 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, const double *mo_energy,
             job_w_z(&in, a, b, c, w0, z0, buf);
             const double eabc = mo_energy[o + a] + mo_energy[o + b] + mo_energy[o + c];
             const double fac = triple_weight(a, b, c);
-            double tsum = 0.0;
+            double tsum = 0.0, ejob = 0.0;
             for (int i = 0; i < o; i++)
                 for (int j = 0; j < o; j++)
                     for (int k = 0; k < o; k++) {
                         size_t x = ((size_t)i * o + j) * o + k;
                         double d = fac / (mo_energy[i] + mo_energy[j] + mo_energy[k] - eabc);
                         d3[x] = d;
+                        ejob += w0[x] * z0[x] * d;
                         zb[x] = ebar * w0[x] * d;
                         double d3b = ebar * w0[x] * z0[x];
                         double t = d3b * d * d / fac;
                         me_b[i] -= t; me_b[j] -= t; me_b[k] -= t;
                         tsum += t;
                     }
+            etot += ejob;
             me_b[o + a] += tsum; me_b[o + b] += tsum; me_b[o + c] += tsum;
             r3_apply(vb0, zb, o);                                   /* v0_bar = r3(z_bar) */
             for (size_t x = 0; x < ooo; x++) wb0[x] = ebar * z0[x] * d3[x] + vb0[x];
-            for (int s = 0; s < 6; s++) {
-                const int p = abc[PERM_ABC[s][0]], q = abc[PERM_ABC[s][1]], r = abc[PERM_ABC[s][2]];
-                for (int i = 0; i < o; i++)
-                    for (int j = 0; j < o; j++)
-                        for (int k = 0; k < o; k++) {
-                            size_t d = sigma_index(s, i, j, k, o);
-                            wbar[(i * o + j) * o + k] = wb0[d];
-                            vbar[(i * o + j) * o + k] = vb0[d];
-                        }
-                const double *pq = vvop + ((size_t)p * v + q) * o * n;
+            for (int pr = 0; pr < 3; pr++) {
+                const int r = abc[PERM_ABC[PAIRS[pr][0]][2]];
                 const double *t2r = t2T + (size_t)r * voo;
-                const double *t2rq = t2r + (size_t)q * oo;
-                const double *voop = vooo + (size_t)p * ooo;
-                double *voop_b = vooo_b + (size_t)p * ooo;
-                memset(pqb_loc, 0, sizeof(double) * o * n);
-                memset(t2b_loc, 0, sizeof(double) * voo);
-                memset(t2qp_loc, 0, sizeof(double) * oo);
-                /* particle: vvop_bar[p,q,i,o+f] += sum_jk t2T[r,f,j,k] Wb[i,j,k]
-                 *           t2T_bar[r,f,j,k]   += sum_i Wb[i,j,k] vvop[p,q,i,o+f] */
-                for (int i = 0; i < o; i++) {
-                    const double *wi = wbar + i * oo;
-                    for (int f = 0; f < v; f++) {
-                        const double *t = t2r + (size_t)f * oo;
-                        double *tb = t2b_loc + (size_t)f * oo;
-                        const double x = pq[(size_t)i * n + o + f];
-                        double acc1 = 0.0;
-                        for (size_t jk = 0; jk < oo; jk++) {
-                            acc1 += t[jk] * wi[jk];
-                            tb[jk] += x * wi[jk];
-                        }
-                        pqb_loc[(size_t)i * n + o + f] += acc1;
-                    }
-                }
-                /* hole: vooo_bar[p,i,j,m] -= sum_k t2T[r,q,m,k] Wb[i,j,k]
-                 *       t2T_bar[r,q,m,k]  -= sum_ij Wb[i,j,k] vooo[p,i,j,m] */
-                double *t2rq_loc = t2b_loc + (size_t)q * oo;
-                for (int i = 0; i < o; i++)
-                    for (int j = 0; j < o; j++) {
-                        const double *wij = wbar + (i * o + j) * o;
-                        for (int mm = 0; mm < o; mm++) {
-                            const double *t = t2rq + (size_t)mm * o;
-                            double *tb = t2rq_loc + (size_t)mm * o;
-                            const double x = voop[(i * o + j) * o + mm];
-                            double acc1 = 0.0;
+                double *t2r_b = t2_b + (size_t)r * voo;
+                const double *pqs[2];
+                for (int x = 0; x < 2; x++) {
+                    const int s = PAIRS[pr][x];
+                    pqs[x] = vvop_block(&in, abc[PERM_ABC[s][0]], abc[PERM_ABC[s][1]]);
+                    double *wb = wbar + x * ooo, *vb = vbar + x * ooo;
+                    for (int i = 0; i < o; i++)
+                        for (int j = 0; j < o; j++)
                             for (int k = 0; k < o; k++) {
-                                acc1 += t[k] * wij[k];
-                                tb[k] -= x * wij[k];
+                                size_t d = sigma_index(s, i, j, k, o);
+                                wb[((size_t)i * o + j) * o + k] = wb0[d];
+                                vb[((size_t)i * o + j) * o + k] = vb0[d];
                             }
-                            voop_b[(i * o + j) * o + mm] -= acc1;
+                }
+                memset(pqb_loc, 0, sizeof(double) * 2 * o * n);
+                /* particle, both orderings at once (Wb_x = wbar + x*ooo, column-major oo x o):
+                 *   vvop_bar[p,q,i,o+f] += sum_jk t2T[r,f,j,k] Wb[i,j,k]
+                 *     (column-major v x 2o: t2T[r]^T (v x oo) . [Wb_0 Wb_1] (oo x 2o))
+                 *   t2T_bar[r,f,j,k]   += sum_i Wb[i,j,k] vvop[p,q,i,o+f]
+                 *     (column-major oo x v: [Wb_0 Wb_1] (oo x 2o) . pack^T (2o x v)) */
+                double *pres = pack + (size_t)2 * v * o;
+                pack_pair(&in, pqs[0], pqs[1], pack);
+                gemm('T', 'N', v, 2 * o, oo, 1.0, t2r, oo, wbar, oo, 0.0, pres, v);
+                gemm('N', 'T', oo, v, 2 * o, 1.0, wbar, oo, pack, v, 1.0, t2r_b, oo);
+                for (int x = 0; x < 2; x++) {
+                    const int s = PAIRS[pr][x];
+                    const int p = abc[PERM_ABC[s][0]], q = abc[PERM_ABC[s][1]];
+                    const double *pq = pqs[x];
+                    const double *wb = wbar + x * ooo, *vb = vbar + x * ooo;
+                    double *pqb = pqb_loc + (size_t)x * o * n;
+                    for (int i = 0; i < o; i++)
+                        for (int f = 0; f < v; f++) pqb[(size_t)i * n + o + f] = pres[(size_t)v * (x * o + i) + f];
+                    const double *t2rq = t2r + (size_t)q * oo;
+                    const double *voop = vooo + (size_t)p * ooo;
+                    double *voop_b = vooo_b + (size_t)p * ooo;
+                    /* hole: vooo_bar[p,i,j,m] -= sum_k t2T[r,q,m,k] Wb[i,j,k]
+                     *   (column-major o x oo: t2T[r,q]^T (o x o) . Wb (o x oo))
+                     *       t2T_bar[r,q,m,k]  -= sum_ij Wb[i,j,k] vooo[p,i,j,m]
+                     *   (column-major o x o: Wb (o x oo) . vooo[p]^T (oo x o)) */
+                    gemm('T', 'N', o, oo, o, -1.0, t2rq, o, wb, o, 1.0, voop_b, o);
+                    gemm('N', 'T', o, o, oo, -1.0, wb, o, voop, o, 1.0, t2r_b + (size_t)q * oo, o);
+                    /* V part */
+                    const double *t1r = t1T + (size_t)r * o;
+                    const double *fr = fvo + (size_t)r * o;
+                    const double *t2qp = t2T + ((size_t)q * v + p) * oo;
+                    double *t2qp_b = t2_b + ((size_t)q * v + p) * oo;
+                    double *t1r_b = t1_b + (size_t)r * o;
+                    double *fr_b = fvo_b + (size_t)r * o;
+                    for (int i = 0; i < o; i++)
+                        for (int j = 0; j < o; j++) {
+                            const double *vij = vb + ((size_t)i * o + j) * o;
+                            double s1 = 0.0, s2 = 0.0;
+                            const double x1 = pq[(size_t)i * n + j], x2 = t2qp[i * o + j];
+                            for (int k = 0; k < o; k++) {
+                                s1 += vij[k] * t1r[k];
+                                s2 += vij[k] * fr[k];
+                                t1r_b[k] += 0.5 * vij[k] * x1;
+                                fr_b[k] += 0.5 * vij[k] * x2;
+                            }
+                            pqb[(size_t)i * n + j] += 0.5 * s1;
+                            t2qp_b[i * o + j] += 0.5 * s2;
                         }
-                    }
-                /* V part */
-                const double *t1r = t1T + (size_t)r * o;
-                const double *fr = fvo + (size_t)r * o;
-                const double *t2qp = t2T + ((size_t)q * v + p) * oo;
-                double *t1r_b = t1_b + (size_t)r * o;
-                double *fr_b = fvo_b + (size_t)r * o;
-                for (int i = 0; i < o; i++)
-                    for (int j = 0; j < o; j++) {
-                        const double *vij = vbar + (i * o + j) * o;
-                        double s1 = 0.0, s2 = 0.0;
-                        const double x1 = pq[(size_t)i * n + j], x2 = t2qp[i * o + j];
-                        for (int k = 0; k < o; k++) {
-                            s1 += vij[k] * t1r[k];
-                            s2 += vij[k] * fr[k];
-                            t1r_b[k] += 0.5 * vij[k] * x1;
-                            fr_b[k] += 0.5 * vij[k] * x2;
-                        }
-                        pqb_loc[(size_t)i * n + j] += 0.5 * s1;
-                        t2qp_loc[i * o + j] += 0.5 * s2;
-                    }
-                /* flush the job-local blocks into the shared cotangents */
-                double *pq_b = vvop_bar + ((size_t)p * v + q) * o * n;
-                LOCK(lk_pq[(size_t)p * v + q]);
-                for (size_t x = 0; x < (size_t)o * n; x++) pq_b[x] += pqb_loc[x];
-                UNLOCK(lk_pq[(size_t)p * v + q]);
-                double *t2r_b = t2T_bar + (size_t)r * voo;
-                LOCK(lk_r[r]);
-                for (size_t x = 0; x < voo; x++) t2r_b[x] += t2b_loc[x];
-                UNLOCK(lk_r[r]);
-                double *t2qp_b = t2T_bar + ((size_t)q * v + p) * oo;
-                LOCK(lk_r[q]);
-                for (size_t x = 0; x < oo; x++) t2qp_b[x] += t2qp_loc[x];
-                UNLOCK(lk_r[q]);
+                    /* flush the job-local block into the shared vvop_bar[p,q] */
+                    double *pq_b = vvop_bar + ((size_t)p * w + q) * o * n;
+                    LOCK(lk_pq[(size_t)p * w + q]);
+                    for (size_t y = 0; y < (size_t)o * n; y++) pq_b[y] += pqb[y];
+                    UNLOCK(lk_pq[(size_t)p * w + q]);
+                }
             }
         }
-        free(t2qp_loc);
-        free(t2b_loc);
+        free(pack);
         free(pqb_loc);
         free(buf);
         free(w0);
     }
     /* ordered reduction of the per-thread accumulators into the caller's buffers */
-    double *outs[4] = {mo_energy_bar, t1T_bar, vooo_bar, fvo_bar};
-    size_t lens[4] = {len_me, len_t1, len_vooo, len_fvo};
+    double *outs[5] = {mo_energy_bar, t1T_bar, vooo_bar, fvo_bar, t2T_bar};
+    size_t lens[5] = {len_me, len_t1, len_vooo, len_fvo, len_t2};
     size_t off = 0;
-    for (int x = 0; x < 4; x++) {
+    for (int x = 0; x < 5; x++) {
         double *dst = outs[x];
         const size_t len = lens[x];
 #pragma omp parallel for schedule(static)
@@ -390,26 +455,13 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, const double *mo_energy,
     }
     for (int t = 0; t < nthreads; t++) free(priv[t]);
     free(priv);
+    if (e_out) *e_out = 2.0 * etot;
 #ifdef _OPENMP
-    for (size_t x = 0; x < (size_t)v * v; x++) omp_destroy_lock(&lk_pq[x]);
-    for (int x = 0; x < v; x++) omp_destroy_lock(&lk_r[x]);
+    for (size_t x = 0; x < (size_t)w * w; x++) omp_destroy_lock(&lk_pq[x]);
     free(lk_pq);
-    free(lk_r);
 #endif
 #undef LOCK
 #undef UNLOCK
-}
-
-void oracle_ccsd_t_energy_vjp(int nocc, int nvir, const double *mo_energy,
-                              const double *t1T, const double *t2T,
-                              const double *vooo, const double *fvo, const double *vvop,
-                              double et_bar,
-                              double *mo_energy_bar, double *t1T_bar, double *t2T_bar,
-                              double *vooo_bar, double *fvo_bar, double *vvop_bar)
-{
-    oracle_ccsd_t_energy_vjp_range(nocc, nvir, mo_energy, t1T, t2T, vooo, fvo, vvop, et_bar,
-                                   mo_energy_bar, t1T_bar, t2T_bar, vooo_bar, fvo_bar, vvop_bar,
-                                   0, njobs_total(nvir));
 }
 
 long oracle_njobs(int nvir) { return njobs_total(nvir); }

@@ -10,10 +10,18 @@ used to pin its energy:
 * ``energy_dense6``: the dense six-index form (ccsd_t_slow.py:138-220) --
   ``t3 = P6(W+V)/D``, ``E = sum t3 * r3(P6(W)/D) * D / 3``.
 
-Both use ``t2T = t2.transpose(2,3,0,1)`` exactly as ccsd_t_slow.py does; for a
-t2 with the CCSD symmetry t2[i,j,a,b] = t2[j,i,b,a] this reads the same values
-as the fast path's ``t2.transpose(2,3,1,0)`` (ccsd_t.py:90).  Only use them
-with such t2.
+Both use ``t2T = t2.transpose(2,3,0,1)`` exactly as ccsd_t_slow.py does.  For
+a t2 with the CCSD symmetry t2[i,j,a,b] = t2[j,i,b,a] this reads the same values
+as the fast path's ``t2.transpose(2,3,1,0)`` (ccsd_t.py:90).  For any other t2
+the two paths differ: the particle term reads the same t2 element in both,
+while the hole and V terms read its pair partner.  Only use these two with such t2.
+Their maps of ovvv, ovoo and ovov (ccsd_t_slow.py:40-42) equal the fast path's
+(ccsd_t.py:98-106) element for element, so they pin those maps on any input.
+
+* ``energy_dense6_fastmap``: the dense six-index form written with the fast
+  path's own map ``t2T = t2.transpose(2,3,1,0)`` (ccsd_t.py:90) in each of the
+  terms of get_wv (ccsd_t.c:137-150).  It holds for any t2, and it pins the
+  element of t2 that each term reads.
 """
 import numpy as np
 
@@ -97,3 +105,21 @@ def energy_dense6(t1, t2, mo_energy, fock, ovoo, ovov, ovvv):
     t3 = _p6(w + v) / d3
     wd = _p6(w) / d3
     return float(np.einsum('ijkabc,ijkabc', t3, _r3_6d(wd) * d3) / 3.0)
+
+
+def energy_dense6_fastmap(t1, t2, mo_energy, fock, ovoo, ovov, ovvv):
+    """E = (1/3) sum P6(W+V) r3(P6(W)) / D, with the terms of get_wv
+    (ccsd_t.c:137-150) over all (a,b,c) and (i,j,k):
+        W = sum_f vvop[a,b,i,o+f] t2T[c,f,j,k] - sum_m vooo[a,i,j,m] t2T[c,b,m,k]
+        V = 1/2 vvop[a,b,i,j] t1T[c,k] + 1/2 t2T[b,a,i,j] fvo[c,k]
+    and t2T = t2.transpose(2,3,1,0) (ccsd_t.py:90)."""
+    nocc, nvir = t1.shape
+    vvov, vooo, vvoo, fvo = _prep(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+    t2T = np.asarray(t2).transpose(2, 3, 1, 0)
+    w = np.einsum('abif,cfjk->ijkabc', vvov, t2T)
+    w -= np.einsum('aijm,cbmk->ijkabc', vooo, t2T)
+    v = 0.5 * (np.einsum('abij,ck->ijkabc', vvoo, t1.T) + np.einsum('baij,ck->ijkabc', t2T, fvo))
+    eia = mo_energy[:nocc, None] - mo_energy[None, nocc:]
+    d3 = (eia[:, None, None, :, None, None] + eia[None, :, None, None, :, None]
+          + eia[None, None, :, None, None, :])
+    return float(np.einsum('ijkabc,ijkabc', _p6(w + v), _r3_6d(_p6(w)) / d3) / 3.0)
