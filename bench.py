@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
-"""(T) energy+VJP throughput on one MI355X.
+"""(T) energy+VJP throughput on one MI355X, FP32-MFMA against FP64.
 
-One step = one forward call (E(T), the custom_vjp forward) plus one backward
-call (all cotangents; the backward recomputes W/V instead of storing them) of
-the CCSD(T) perturbative-triples correction, on the synthetic single instance
-nocc=32, nvir=256 (seed 2; generator in pyscfad_amd/synthetic.py), with the
-inputs resident in HBM before the timed region starts.
+Workload (default): the synthetic single instance nocc=64, nvir=512 (seed 3;
+generator in pyscfad_amd/synthetic.py), inputs resident in HBM before the
+timed region.  One step = the forward call (E(T), the custom_vjp forward)
+plus the backward call (every cotangent; the backward recomputes W/V
+instead of storing them) over one batch of --batch-triples unique occupied
+triples; step s takes the s-th consecutive batch from --first-triple on.  A
+whole solve (45760 triples) takes about 25 min in FP32, so it is not timed
+here; the whole o=32/v=256 solve is (--nocc 32 --nvir 256 --batch-triples 0).
 
-metric   : TFLOP/s of energy+VJP credited as 3 * F_E, F_E = 2 o^3 v^3 (v + o)
+metric   : TFLOP/s of energy+VJP credited as 3 * F_E of the triples processed,
+           F_E = 2 v^3 (v + o) per distinct occupied ordering (2 o^3 v^3 (v+o)
+           for a whole instance)
 value    : that rate for the --precision path (default f32 MFMA)
-roofline : the grouped MFMA GEMM kernel (it executes 4 * F_E algorithmic flops
-           per step: forward W, backward W recompute, two adjoint contractions),
-           its device time measured with hipEvents around every launch during
-           the warm-up step, against the MI355X dense matrix peak of the
-           precision used
+f64      : the FP64 path timed on the same batches in the same process, and
+           the ratio of the two rates
+roofline : the grouped MFMA GEMM kernel, against the MI355X dense matrix peak
+           of the precision used; achieved = the credited 3 F_E over the
+           kernel's device time (hipEvents around every launch on the call's
+           stream, in the first warm-up step); the kernel also executes the
+           backward's W recompute (a fourth F_E), which is not credited
 cpu_baseline: the repository's FP64 C/OpenMP restatement of the reference
-           algorithm (oracle/), energy + VJP on a bounded slice of the
-           a>=b>=c virtual-triple jobs, on the host cores of the same machine
-The FP64 GPU path is run once as the accuracy reference of the lower
-precisions (|dE| and max relative cotangent error are reported).
+           algorithm (oracle/, products on scipy's BLAS), energy + VJP of a
+           bounded run of the a>=b>=c virtual-triple jobs of the same
+           instance, on the host cores of the same machine
+accuracy : the last timed batch in --precision against FP64: |dE| in Eh and
+           relative, max|d|/max|ref| per cotangent
 """
 import argparse
 import json
@@ -27,16 +35,21 @@ import sys
 import time
 
 MI355X_PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2516.6 / 3.0, "bf16x6": 2516.6 / 6.0}
+METRIC = "(T) energy+VJP TFLOP/s & s/solve, FP32-MFMA vs FP64 on MI355X at <=1e-6 Eh"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nocc", type=int, default=32)
-    ap.add_argument("--nvir", type=int, default=256)
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nocc", type=int, default=64)
+    ap.add_argument("--nvir", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--batch-triples", type=int, default=64,
+                    help="unique occupied triples per step (0: the whole instance per step)")
+    ap.add_argument("--first-triple", type=int, default=20000,
+                    help="first triple of the timed batches (the warm-up batches precede it)")
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16x3", "bf16x6"])
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 accuracy reference run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline sample time")
@@ -67,15 +80,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(inst, nocc, nvir, target_s):
-    """Oracle energy + VJP on a slice of the job list, timed on the host cores."""
-    import numpy as np
+def cpu_baseline(inst, nocc, nvir, target_s, vrow=16):
+    """Oracle energy + VJP on a bounded run of the instance's a>=b>=c jobs,
+    timed on the host cores.  Only the host arrays those jobs read are
+    built: the jobs [0, njobs(vrow)) have a, b, c < vrow, so their operands
+    are the first vrow virtual rows of t1T, t2T, vooo and fvo, the vrow x vrow
+    block of vvop, and ovvv[:, :vrow] (the contracted indices still run over
+    all nocc and nvir).  Each job's work is the same for every job
+    (12 o^3 (v + o) flops forward), so the rate stands for the whole list."""
     from oracle import ccsd_t_oracle as orc
-    host = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in inst.items()}
-    prep = orc.Prepared(**host)
-    del host
+    w = min(vrow, nvir)
+
+    def host(x):
+        return x.cpu().numpy() if hasattr(x, "cpu") else x
+
+    prep = orc.Prepared(t1=host(inst["t1"]), t2=host(inst["t2"][:, :, :w, :]), mo_energy=host(inst["mo_energy"]),
+                        fock=host(inst["fock"]), ovoo=host(inst["ovoo"][:, :w]),
+                        ovov=host(inst["ovov"][:, :w, :, :w]), ovvv=host(inst["ovvv"][:, :w]), vrow=w)
     flops_per_job = 12.0 * nocc ** 3 * (nvir + nocc)          # forward, one a>=b>=c job
-    njobs_all = orc.njobs(nvir)
+    nwin = prep.njobs()
 
     def run(j0, j1):
         t0 = time.perf_counter()
@@ -83,30 +106,33 @@ def cpu_baseline(inst, nocc, nvir, target_s):
         prep.energy_vjp(j0, j1)
         return time.perf_counter() - t0
 
-    j0 = njobs_all // 2
-    probe = 16
-    run(j0, j0 + 2)                      # first call allocates the accumulators
-    t_probe = run(j0, j0 + probe)
-    n = int(max(probe, min(njobs_all - j0, probe * target_s / max(t_probe, 1e-3))))
-    t = run(j0, j0 + n)
+    with orc.blas():
+        j0 = nwin // 4
+        probe = max(2, min(16, nwin // 8))
+        t_probe = run(j0, j0 + probe)
+        n = int(max(probe, min(nwin - j0, probe * target_s / max(t_probe, 1e-3))))
+        t = run(j0, j0 + n)
     rate = 3.0 * flops_per_job * n / t / 1e12
     return {"value": rate, "unit": "TFLOP/s", "cores": orc.num_threads(), "kind": "port",
-            "sample": f"energy+VJP of a>=b>=c jobs [{j0}, {j0 + n}) of {njobs_all} "
-                      f"(nocc={nocc}, nvir={nvir}) in {t:.1f} s, FP64, credited 3*F_E per job"}
+            "sample": f"energy+VJP of a>=b>=c jobs [{j0}, {j0 + n}) of the {orc.njobs(nvir)} of the nocc={nocc}, "
+                      f"nvir={nvir} instance (jobs with a, b, c < {w}, whose host arrays alone are built) in "
+                      f"{t:.1f} s, FP64 C/OpenMP with scipy's BLAS (one BLAS thread per OpenMP thread), "
+                      f"credited 3*F_E per job"}
 
 
-def measured_traffic(precision, nocc, nvir):
+def measured_traffic(precision, nocc, nvir, batch_triples):
     """HBM bytes per gemm_grouped launch from the committed FETCH_SIZE / WRITE_SIZE
     profile of this workload (rocprofv3 --pmc runs, calibrated on a copy kernel),
     or None when none was recorded for it."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "gemm_grouped_traffic.json")
     try:
-        rec = json.load(open(path)).get(precision)
+        recs = json.load(open(path))
     except (OSError, ValueError):
         return None, None
-    if not rec or rec.get("nocc") != nocc or rec.get("nvir") != nvir:
-        return None, None
-    return rec["bytes_per_launch"], rec.get("source")
+    for rec in recs.get("records", []):
+        if (rec.get("precision"), rec.get("nocc"), rec.get("nvir")) == (precision, nocc, nvir):
+            return rec["bytes_per_launch"], rec.get("source")
+    return None, None
 
 
 def slice_flops(nocc, nvir, t0, t1):
@@ -216,6 +242,17 @@ def max_rel(a, b):
     return (torch.max(d).item() / scale, elem)
 
 
+def triple_flops(nocc, nvir):
+    """Cumulative F_E over the unique triples in the library's order (i >= j >= k,
+    i outermost): entry t is the F_E of triples [0, t).  Each unique triple
+    stands for its distinct orderings (6, 3 or 1), each 2 v^3 (v + o)."""
+    import numpy as np
+    per = 2.0 * nvir ** 3 * (nvir + nocc)
+    w = [per * (1 if i == k else (3 if (i == j or j == k) else 6))
+         for i in range(nocc) for j in range(i + 1) for k in range(j + 1)]
+    return np.concatenate([[0.0], np.cumsum(w)])
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -239,24 +276,39 @@ def main():
     inst = make_instance(o, v, args.seed, device="cuda", df=args.df)
     torch.cuda.synchronize()
     torch.cuda.empty_cache()   # return the generator's temporaries before the workspace is allocated
-    torch.cuda.reset_peak_memory_stats()   # peak_hbm_GB: resident inputs + workspace + outputs
     keys = ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov") + (("Lov", "Lvv") if args.df else ("ovvv",))
     a = [inst[k] for k in keys]
     naux = inst["Lov"].shape[0] if args.df else 0
     ntrip = ccsd_t.num_triples(o)
-    if args.triples == "auto" and args.df:
-        t0, t1 = df_auto_slice(o, args.df_block)
-    elif args.triples:
-        t0, t1 = (int(x) for x in args.triples.split(":"))
+    W, K = max(args.warmup, 0), args.steps
+
+    # the batch of each step
+    whole = False
+    if args.triples:
+        t0, t1 = (df_auto_slice(o, args.df_block) if (args.triples == "auto" and args.df)
+                  else (int(x) for x in args.triples.split(":")))
+        batches = [(t0, t1)] * (W + K)
+    elif args.batch_triples > 0 and not args.df:
+        B = min(args.batch_triples, ntrip)
+        nb = W + K
+        start = max(0, min(args.first_triple - W * B, ntrip - nb * B))
+        batches = [(start + x * B, min(ntrip, start + (x + 1) * B)) for x in range(nb)]
+        if start + nb * B > ntrip:
+            # more steps than batches in the list: cycle through it
+            batches = [((x * B) % (ntrip - B + 1), (x * B) % (ntrip - B + 1) + B) for x in range(nb)]
     else:
-        t0, t1 = 0, ntrip
-    sl = (t0, t1)
-    df_builds = None
+        whole = True
+        batches = [(0, ntrip)] * (W + K)
+    warm, timed = batches[:W], batches[W:]
     if args.df:
-        fe, sb, fb = df_slice_info(o, v, args.df_block, t0, t1)
-        df_builds = (sb, fb)
+        def fe_of(sl):
+            return df_slice_info(o, v, args.df_block, sl[0], sl[1])[0]
     else:
-        fe = flops_energy(o, v) if (t0, t1) == (0, ntrip) else slice_flops(o, v, t0, t1)
+        cum = triple_flops(o, v)
+
+        def fe_of(sl):
+            return float(cum[sl[1]] - cum[sl[0]])
+
     if args.calibrate_copy:
         src = torch.empty(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -280,7 +332,7 @@ def main():
     f_energy = ccsd_t.energy_df if args.df else ccsd_t.energy
     f_vjp = ccsd_t.energy_vjp_df if args.df else ccsd_t.energy_vjp
 
-    def step(prec, timing=False):
+    def step(prec, sl, timing=False):
         _lib.set_timing(timing)
         e = f_energy(*a, precision=prec, triples=sl, workspace=workspace(prec))
         t_fwd = _lib.last_timing() if timing else None
@@ -292,110 +344,133 @@ def main():
         _lib.set_timing(False)
         return e, bars, t_fwd, t_bwd
 
+    def timed_run(prec, sls, label):
+        """Time the steps over the batches sls: (seconds per step, per-step
+        seconds, last energy, last cotangents)."""
+        log(f"[bench] timing {len(sls)} {label} step(s) ({prec})")
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(sls) + 1)]
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        evs[0].record()
+        e = bars = None
+        for x, sl in enumerate(sls):
+            bars = None
+            e, bars, _, _ = step(prec, sl)
+            evs[x + 1].record()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - c0) / len(sls)
+        per = [evs[x].elapsed_time(evs[x + 1]) / 1e3 for x in range(len(sls))]
+        return dt, per, e, bars
+
     # tile shapes from the autotuner (outside the timed region)
     tuned = None
+    compare = not args.no_compare and args.precision != "f64"
     if not args.no_autotune:
         log(f"[bench] autotuning tile shapes ({args.precision})")
         tuned = ccsd_t.autotune(o, v, args.precision)
-        if not args.no_compare and args.precision != "f64":
+        if compare:
             ccsd_t.autotune(o, v, "f64")
 
-    gemm_ms = None
-    fam = {}
-    for w in range(args.warmup):
-        log(f"[bench] warm-up step {w} ({args.precision}, kernel timing on)")
-        _, _, tf, tb = step(args.precision, timing=(w == 0))
+    torch.cuda.reset_peak_memory_stats()   # peak_hbm_GB: resident inputs + workspace + outputs
+    fam, fe_w = {}, None
+    for w, sl in enumerate(warm):
+        log(f"[bench] warm-up step {w} ({args.precision}, triples [{sl[0]},{sl[1]})" + (", kernel timing on)" if w == 0 else ")"))
+        _, bars, tf, tb = step(args.precision, sl, timing=(w == 0))
+        bars = None
         if w == 0:
             fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf}
-            gemm_ms = fam["gemm"][0]
-    torch.cuda.synchronize()
-    log(f"[bench] timing {args.steps} step(s)")
-    c0 = time.perf_counter()
-    for _ in range(args.steps):
-        e, bars, _, _ = step(args.precision)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - c0) / args.steps
-    peak_hbm = torch.cuda.max_memory_allocated() / 1e9   # before the FP64 reference step
+            fe_w = fe_of(sl)
+    dt, per32, e, bars = timed_run(args.precision, timed, "timed")
+    fe = sum(fe_of(sl) for sl in timed)
+    peak_hbm = torch.cuda.max_memory_allocated() / 1e9
     credit = 1.0 if args.energy_only else 3.0    # forward only: F_E; energy + VJP: 3 F_E
-    tflops = credit * fe / dt / 1e12
-    if gemm_ms is None:
-        _, _, tf, tb = step(args.precision, timing=True)
+    tflops = credit * fe / (dt * K) / 1e12
+    if fe_w is None:
+        _, bars_t, tf, tb = step(args.precision, timed[0], timing=True)
+        del bars_t
         fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf}
-        gemm_ms = fam["gemm"][0]
+        fe_w = fe_of(timed[0])
+    gemm_ms = fam["gemm"][0]
     peak = MI355X_PEAK_TFLOPS[args.precision]
     # density-fitted mode: the FP64 slab builds and folds run through the same
     # GEMM kernel but are not part of the credited contractions; their family
     # time (which also holds their clears and reductions) is taken out
     df_gemm_ms = fam.get("df", (0.0, 0))[0]
-    achieved = (1.0 if args.energy_only else 4.0) * fe / ((gemm_ms - df_gemm_ms) / 1e3) / 1e12
-    # algorithmic flops of each contraction: W build (forward + recompute)
-    # 2 F_E; particle-term adjoints (ovvv, t2) F_E v/(v+o) each; hole-term
-    # adjoints (ovoo, t2) F_E o/(v+o) each
-    part, hole = fe * v / (v + o), fe * o / (v + o)
-    sub_flops = {"gemm_w": 2.0 * fe, "gemm_b1": part, "gemm_b2": part, "gemm_b3": hole, "gemm_b4": hole}
+    achieved = credit * fe_w / ((gemm_ms - df_gemm_ms) / 1e3) / 1e12
+    # algorithmic flops of each contraction of the timed batch: W build
+    # (forward + recompute) 2 F_E, of which F_E is credited; particle-term
+    # adjoints (ovvv, t2) F_E v/(v+o) each; hole-term adjoints (ovoo, t2)
+    # F_E o/(v+o) each
+    part, hole = fe_w * v / (v + o), fe_w * o / (v + o)
+    sub_flops = {"gemm_w": 2.0 * fe_w, "gemm_b1": part, "gemm_b2": part, "gemm_b3": hole, "gemm_b4": hole}
     if args.energy_only:
-        sub_flops = {"gemm_w": fe}
-    traffic, traffic_src = measured_traffic(args.precision, o, v)
-    # a slice of the triples carries the whole per-call cost (input re-layout,
-    # accumulator clears, cotangent write-back, host planning); the full solve
-    # is projected as that fixed part plus the per-triple kernels scaled by flops
-    setup_ms = fam.get("prep", (0.0, 0))[0]
-    # density-fitted slab builds, folds and clears scale with the number of
-    # slab blocks built, not with the triples
-    df_s = fam.get("df", (0.0, 0))[0] / 1e3
-    per_triple_s = sum(fam.get(k, (0.0, 0))[0] for k in ("gemm", "epilogue", "vpart", "reduce")) / 1e3 - df_s
-    full = flops_energy(o, v)
-    projected_s = None
-    if (t0, t1) != (0, ntrip):
-        projected_s = max(dt - per_triple_s - df_s, 0.0) + per_triple_s * full / fe
-        if df_builds and df_builds[0] > 0:
-            projected_s += df_s * df_builds[1] / df_builds[0]
-    projected_tflops = (credit * full / projected_s / 1e12) if projected_s else None
+        sub_flops = {"gemm_w": fe_w}
     contraction = {k: {"ms": fam[k][0], "launches": fam[k][1],
-                       "tflops": (f / (fam[k][0] / 1e3) / 1e12) if fam[k][0] > 0 else None}
+                       "tflops_executed": (f / (fam[k][0] / 1e3) / 1e12) if fam[k][0] > 0 else None}
                    for k, f in sub_flops.items() if k in fam}
+    traffic, traffic_src = measured_traffic(args.precision, o, v, None if whole else timed[0][1] - timed[0][0])
+    full = flops_energy(o, v)
 
+    f64 = None
     accuracy = None
-    if args.energy_only and not args.no_compare and args.precision != "f64":
+    if compare:
+        # the FP64 path on the same batches; large cotangents wait in host
+        # memory while the FP64 workspace is live
+        last = timed[-1] if args.energy_only else None
+        if bars is not None and o * v ** 3 * 8 > 8e9:
+            bars = [b.cpu() for b in bars]
         wsp.clear()
         torch.cuda.empty_cache()
-        e64 = ccsd_t.energy(*a, precision="f64", triples=sl, workspace=workspace("f64"))
-        accuracy = {"dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64)}
-    elif not args.no_compare and args.precision != "f64":
-        # large instances: park the reduced-precision cotangents in host memory
-        # while the FP64 step runs, and compare there
-        on_host = o * v ** 3 * 8 > 8e9
-        if on_host:
-            bars = [b.cpu() for b in bars]
-            torch.cuda.empty_cache()
-        log("[bench] FP64 reference step for the accuracy report")
-        e64, bars64, _, _ = step("f64")
-        names = ("t1", "t2", "fock", "mo_energy", "ovoo", "ovov") + (("Lov", "Lvv") if args.df else ("ovvv",))
-        rel, elem = {}, {}
-        for n, b, r in zip(names, bars, bars64):
-            rel[n], elem[n] = max_rel_chunked(b, r)
-        accuracy = {"dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64),
-                    "max_rel": rel, "max_rel_elementwise": elem}
+        n64 = len(timed) if not whole else min(len(timed), 2)
+        log("[bench] FP64 warm-up step")
+        step("f64", warm[0] if warm else timed[0])
+        dt64, per64, e64, bars64 = timed_run("f64", timed[-n64:], "FP64")
+        t32_same = sum(per32[-n64:])
+        t64_same = sum(per64)
+        f64 = {"tflops": credit * sum(fe_of(sl) for sl in timed[-n64:]) / t64_same / 1e12,
+               "ms_per_step": dt64 * 1e3, "steps": n64,
+               "batches": f"the last {n64} of the timed batches",
+               "f32_over_f64": t64_same / t32_same if args.precision == "f32" else None,
+               "speedup_over_f64": t64_same / t32_same}
+        accuracy = {"batch": list(timed[-1]), "dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64),
+                    "dE_rel": abs(float(e) - float(e64)) / max(abs(float(e64)), 1e-300)}
+        if not args.energy_only:
+            names = ("t1", "t2", "fock", "mo_energy", "ovoo", "ovov") + (("Lov", "Lvv") if args.df else ("ovvv",))
+            rel, elem = {}, {}
+            for n, b, r in zip(names, bars, bars64):
+                rel[n], elem[n] = max_rel_chunked(b, r)
+            accuracy["max_rel"] = rel
+            accuracy["max_rel_elementwise"] = elem
+        del last
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.df:
         log(f"[bench] CPU baseline sample (~{args.cpu_seconds:.0f} s)")
         cpu = cpu_baseline(inst, o, v, args.cpu_seconds)
 
+    wl = (f"single (T) instance nocc={o} nvir={v} seed={args.seed}"
+          + (f", density-fitted (naux={naux}, occupied blocks of {args.df_block})" if args.df else ""))
+    if whole:
+        wl += f", all {ntrip} unique occupied triples per step"
+    elif args.triples:
+        wl += f", unique triples [{timed[0][0]},{timed[0][1]}) of {ntrip} per step" + (" (blocked order)" if args.df else "")
+    else:
+        wl += (f", {timed[0][1] - timed[0][0]} unique occupied triples per step: consecutive batches "
+               f"[{timed[0][0]},{timed[-1][1]}) of the {ntrip}")
     out = {
-        "metric": ("(T) energy TFLOP/s (F_E credited)" if args.energy_only
-                   else "(T) energy+VJP TFLOP/s (3*F_E credited)"),
+        "metric": METRIC if not args.energy_only else "(T) energy TFLOP/s (F_E credited)",
         "value": tflops,
         "unit": "TFLOP/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": W,
         "ms_per_step": dt * 1e3,
-        "s_per_solve": dt,
-        "setup_ms": setup_ms,
+        "s_per_solve": dt if whole else None,
+        "projected_full_solve_s": None if whole else credit * full / (tflops * 1e12),
+        "projection_note": (None if whole else
+                            "not measured: the whole triple list at the measured rate (each batch here "
+                            "also pays the per-call setup once)"),
         "peak_hbm_GB": peak_hbm,
-        "projected_full_solve_s": projected_s,
-        "projected_full_solve_tflops": projected_tflops,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -403,22 +478,20 @@ def main():
                   "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)",
                   "bf16x6": "bf16x6 (three-part bf16 split, FP32 accumulate, FP64 denominators and sums)"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": f"single (T) instance nocc={o} nvir={v} seed={args.seed}"
-                               + (f", density-fitted (naux={naux}, occupied blocks of {args.df_block})" if args.df else "")
-                               + ("" if (t0, t1) == (0, ntrip) else
-                                  f", unique triples [{t0},{t1}) of {ntrip}" + (" (blocked order)" if args.df else "")),
-                   "precision": args.precision, "E_T": float(e)},
+        "config": {"workload": wl, "precision": args.precision, "E_T_last_step": float(e)},
+        "f64": f64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic, "traffic_unit": "bytes per launch",
                      "traffic_source": traffic_src,
-                     "kernel": "gemm_grouped", "gemm_ms_per_step": gemm_ms,
+                     "kernel": "gemm_grouped",
+                     "credited": f"{credit:.0f} F_E of the first warm-up batch over its gemm_grouped device time "
+                                 "(the backward's W recompute runs in the same kernel and is not credited)",
+                     "gemm_ms_per_step": gemm_ms,
                      "gemm_launches_per_step": fam.get("gemm", (0, 0))[1],
                      "tiles": tuned,
-                     "family_ms": {k: val[0] for k, val in fam.items() if not k.startswith("gemm_") and k != "prep"},
+                     "family_ms": {k: val[0] for k, val in fam.items() if not k.startswith("gemm_")},
                      "contractions": contraction},
         "cpu_baseline": cpu,
-        "df_slab_blocks": ({"slice": df_builds[0], "full_solve": df_builds[1], "seconds_in_slice": df_s}
-                           if df_builds else None),
         "accuracy_vs_f64": accuracy,
     }
     print(json.dumps(out))
@@ -469,7 +542,8 @@ def main_batched(args):
     _lib.set_timing(False)
     gemm_ms = fam["gemm"][0]
     peak = MI355X_PEAK_TFLOPS[args.precision]
-    achieved = 4.0 * fe / (gemm_ms / 1e3) / 1e12
+    # credited 3 F_E over the GEMM device time (the W recompute is executed, not credited)
+    achieved = 3.0 * fe / (gemm_ms / 1e3) / 1e12
 
     log("[bench] 64 sequential FP64 single-instance calls (default tiles)")
     ws64 = torch.empty(max(ccsd_t.workspace_bytes(o, v, "f64") for o, v, _ in shapes), dtype=torch.uint8,

@@ -213,20 +213,32 @@ int pyscfad_amd_last_timing(double *ms, long long *launches, int n);
  * reference registers its GPU handlers through typed XLA FFI
  * (pyscfadlib/pyscfadlib/cuda/solver_kernels.cc:144-157); the legacy form is
  * used here because it needs no XLA headers.
- *   energy:     buffers = {mo_energy, t1, t2, fock, ovoo, ovov, ovvv, e_out}
- *   energy_vjp: buffers = {mo_energy, t1, t2, fock, ovoo, ovov, ovvv,
- *                          t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar}
+ *   energy:     buffers = {mo_energy, t1, t2, fock, ovoo, ovov, ovvv, e_out[, workspace]}
+ *   energy_vjp: buffers = {mo_energy, t1, t2, fock, ovoo, ovov, ovvv, et_bar,
+ *                          t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar[, workspace]}
+ * et_bar is a device FP64 scalar operand (the backward's traced cotangent,
+ * applied on the stream).  With workspace_bytes > 0 the last buffer is a
+ * caller scratch result of that size (at least
+ * pyscfad_amd_ccsd_t_workspace_bytes, 256-byte aligned), so a call makes no
+ * device allocation; with 0 the library allocates and frees its own, which
+ * synchronises the stream.
  * The legacy ABI has no error channel: failures are printed on stderr.
  */
 typedef struct {
     int nocc, nvir, precision, ovvv_packed;
-    double et_bar; /* energy_vjp only */
+    unsigned long long workspace_bytes;
 } PyscfadAmdXlaOpaque;
 
 void pyscfad_amd_xla_ccsd_t_energy(void *stream, void **buffers, const char *opaque, size_t opaque_len);
 void pyscfad_amd_xla_ccsd_t_energy_vjp(void *stream, void **buffers, const char *opaque, size_t opaque_len);
 
-/* ABI version, currently 1. */
+/* Counters of the batched entry point's graph cache since the library was
+ * loaded (no counterpart in the reference; for tests and diagnostics): calls
+ * served by replaying a cached graph, and calls that captured new graphs.
+ * Either pointer may be NULL. */
+void pyscfad_amd_graph_cache_stats(long long *replays, long long *captures);
+
+/* ABI version, currently 2 (2: et_bar operand and workspace result of the XLA handlers). */
 int pyscfad_amd_abi_version(void);
 
 #ifdef __cplusplus

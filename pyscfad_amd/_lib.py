@@ -33,6 +33,7 @@ EXPORTS = (
     "pyscfad_amd_last_timing",
     "pyscfad_amd_xla_ccsd_t_energy",
     "pyscfad_amd_xla_ccsd_t_energy_vjp",
+    "pyscfad_amd_graph_cache_stats",
     "pyscfad_amd_abi_version",
 )
 
@@ -98,6 +99,8 @@ def load():
         fn = getattr(lib, name)
         fn.restype = None
         fn.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, sz]
+    lib.pyscfad_amd_graph_cache_stats.restype = None
+    lib.pyscfad_amd_graph_cache_stats.argtypes = [ctypes.POINTER(ctypes.c_longlong)] * 2
     lib.pyscfad_amd_abi_version.restype = i
     lib.pyscfad_amd_abi_version.argtypes = []
     _lib = lib
@@ -127,7 +130,14 @@ def last_timing():
     return {FAMILIES[f]: (ms[f], nl[f]) for f in range(min(m, len(FAMILIES)))}
 
 
+def graph_cache_stats():
+    """(replays, captures) of the batched entry point's graph cache."""
+    r, c = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    load().pyscfad_amd_graph_cache_stats(ctypes.byref(r), ctypes.byref(c))
+    return r.value, c.value
+
+
 class XlaOpaque(ctypes.Structure):
     """PyscfadAmdXlaOpaque of include/pyscfad_amd.h (the opaque bytes of the XLA custom calls)."""
     _fields_ = [("nocc", ctypes.c_int), ("nvir", ctypes.c_int), ("precision", ctypes.c_int),
-                ("ovvv_packed", ctypes.c_int), ("et_bar", ctypes.c_double)]
+                ("ovvv_packed", ctypes.c_int), ("workspace_bytes", ctypes.c_ulonglong)]

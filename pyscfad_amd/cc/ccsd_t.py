@@ -19,8 +19,11 @@ reference:
   cotangent of ovvv comes back as the cotangents of Lov and Lvv.
 
 Both passes run on the GPU through the C ABI of ``libpyscfad_amd.so``;
-arrays may be NumPy (copied to the device) or torch tensors (used in place
-when already contiguous FP64 on the device).  ``precision`` selects the
+arrays may be NumPy (copied to the device), torch tensors (used in place when
+already contiguous FP64 on the device) or any DLPack array (a JAX device
+array is taken without a copy).  ``stream=`` runs a call on another torch
+stream: it waits for the caller's current stream first, and the current
+stream waits for it afterwards, so results can be used right away on either.  ``precision`` selects the
 arithmetic of the W/V contractions: 'f64' (default, the drop-in), 'f32',
 'bf16x3', or 'bf16x6' (FP32-level accuracy from three-part bf16 splits);
 denominators, the E(T) sum and cotangent accumulation stay FP64.
@@ -39,12 +42,62 @@ def _device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
+def _as_torch(x):
+    """x as a torch tensor: itself, a zero-copy DLPack view (JAX and NumPy
+    arrays), or a NumPy copy for anything else array-like."""
+    if isinstance(x, torch.Tensor):
+        return x
+    if hasattr(x, "__dlpack__"):
+        try:
+            return torch.from_dlpack(x)
+        except (RuntimeError, TypeError, BufferError):
+            pass
+    return torch.from_numpy(numpy.ascontiguousarray(x, dtype=numpy.float64))
+
+
 def _dev(x):
     """Contiguous FP64 device tensor view/copy of x."""
-    dev = _device()
-    if isinstance(x, torch.Tensor):
-        return x.to(device=dev, dtype=torch.float64).contiguous()
-    return torch.as_tensor(numpy.ascontiguousarray(x, dtype=numpy.float64)).to(dev)
+    return _as_torch(x).to(device=_device(), dtype=torch.float64).contiguous()
+
+
+class _OnStream:
+    """Runs a call's allocations, copies and kernels on ``stream`` (default:
+    the current stream).  On another stream the call first waits for the
+    current one; caller tensors read in place are recorded on it (the caching
+    allocator keeps them until it is done); afterwards the current stream
+    waits for it and the outputs are recorded on the current stream."""
+
+    def __init__(self, stream):
+        _device()
+        self.cur = torch.cuda.current_stream()
+        self.s = self.cur if stream is None else stream
+        self.side = self.s != self.cur
+        self._ctx = None
+
+    def __enter__(self):
+        if self.side:
+            self.s.wait_stream(self.cur)
+            self._ctx = torch.cuda.stream(self.s)
+            self._ctx.__enter__()
+        return self
+
+    def uses(self, *tensors):
+        if self.side:
+            for t in tensors:
+                if isinstance(t, torch.Tensor) and t.is_cuda:
+                    t.record_stream(self.s)
+
+    def outputs(self, *tensors):
+        if self.side:
+            for t in tensors:
+                if isinstance(t, torch.Tensor) and t.is_cuda:
+                    t.record_stream(self.cur)
+
+    def __exit__(self, *exc):
+        if self.side:
+            self._ctx.__exit__(*exc)
+            self.cur.wait_stream(self.s)
+        return False
 
 
 def _ptr(t):
@@ -96,14 +149,16 @@ def energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISIO
     occupied triples (a shard of the work).  ``workspace``: an optional device
     tensor of at least ``workspace_bytes(..., with_vjp=False)`` bytes, 256-byte
     aligned; without it the library allocates and frees its own."""
-    nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
-    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
-    s = stream if stream is not None else torch.cuda.current_stream()
     lib = _lib.load()
-    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
-    _lib.check(lib.pyscfad_amd_ccsd_t_energy_slice(
-        nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
-        int(packed), _prec(precision), int(t0), int(t1_), _ptr(out), *_ws(workspace), s.cuda_stream))
+    t0, t1_ = triples if triples is not None else (0, num_triples(numpy.shape(t1)[0]))
+    with _OnStream(stream) as on:
+        nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+        on.uses(t1d, t2d, me, fk, oo, ov, vv, workspace)
+        out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+        _lib.check(lib.pyscfad_amd_ccsd_t_energy_slice(
+            nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
+            int(packed), _prec(precision), int(t0), int(t1_), _ptr(out), *_ws(workspace), on.s.cuda_stream))
+        on.outputs(out)
     return out
 
 
@@ -111,19 +166,20 @@ def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, precision=
                stream=None, return_energy=False, triples=None, workspace=None):
     """Cotangents of E(T) as FP64 device tensors, in the order and layouts of
     the reference's _ccsd_t_energy_vjp: (t1, t2, fock, mo_energy, ovoo, ovov, ovvv)."""
-    nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
-    z = lambda a: torch.empty_like(a)
-    bars = [z(t1d), z(t2d), z(fk), z(me), z(oo), z(ov), z(vv)]
-    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
-    s = stream if stream is not None else torch.cuda.current_stream()
     lib = _lib.load()
     if hasattr(et_bar, "item"):
         et_bar = et_bar.item()
-    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
-    _lib.check(lib.pyscfad_amd_ccsd_t_energy_vjp_slice(
-        nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
-        int(packed), float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars],
-        _ptr(out), *_ws(workspace), s.cuda_stream))
+    t0, t1_ = triples if triples is not None else (0, num_triples(numpy.shape(t1)[0]))
+    with _OnStream(stream) as on:
+        nocc, nvir, t1d, t2d, me, fk, oo, ov, vv, packed = _inputs(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
+        on.uses(t1d, t2d, me, fk, oo, ov, vv, workspace)
+        bars = [torch.empty_like(a) for a in (t1d, t2d, fk, me, oo, ov, vv)]
+        out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+        _lib.check(lib.pyscfad_amd_ccsd_t_energy_vjp_slice(
+            nocc, nvir, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(vv),
+            int(packed), float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars],
+            _ptr(out), *_ws(workspace), on.s.cuda_stream))
+        on.outputs(out, *bars)
     if return_energy:
         return tuple(bars), out
     return tuple(bars)
@@ -177,17 +233,22 @@ def batched(problems, precision=DEFAULT_PRECISION, with_vjp=True, stream=None, w
     tensors (new values allowed), the same ``workspace`` and ``outputs`` = the
     previous result replays the cached graph."""
     lib = _lib.load()
-    arr, keep = _instances(problems, precision, with_vjp, outputs)
-    need = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(problems), arr, _prec(precision), int(with_vjp))
-    if need == 0:
-        raise _lib.T3Error(lib.pyscfad_amd_last_error().decode())
-    if workspace is None:
-        workspace = torch.empty(need, dtype=torch.uint8, device=keep[0][0].device)
-    s = stream if stream is not None else torch.cuda.current_stream()
-    _lib.check(lib.pyscfad_amd_ccsd_t_batched(len(problems), arr, _prec(precision), int(with_vjp),
-                                              *_ws(workspace), s.cuda_stream))
-    # the workspace must outlive the enqueued work
-    s.synchronize()
+    with _OnStream(stream) as on:
+        arr, keep = _instances(problems, precision, with_vjp, outputs)
+        need = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(problems), arr, _prec(precision), int(with_vjp))
+        if need == 0:
+            raise _lib.T3Error(lib.pyscfad_amd_last_error().decode())
+        if workspace is None:
+            workspace = torch.empty(need, dtype=torch.uint8, device=keep[0][0].device)
+        for k in keep:
+            on.uses(*k[:7])
+        on.uses(workspace)
+        # the workspace of this call stays allocated until the stream is past it
+        workspace.record_stream(on.s)
+        _lib.check(lib.pyscfad_amd_ccsd_t_batched(len(problems), arr, _prec(precision), int(with_vjp),
+                                                  *_ws(workspace), on.s.cuda_stream))
+        for k in keep:
+            on.outputs(k[7], *(k[8] if with_vjp else ()))
     return [(k[7], tuple(k[8]) if with_vjp else None) for k in keep]
 
 
@@ -262,13 +323,16 @@ def energy_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=DEFAULT_P
     """E(T) with ovvv = Lov^T Lvv (pyscfad/cc/dfccsd.py:91) built slab by slab
     on the device, never stored whole.  ``triples`` selects a slice of the
     unique triples in this variant's blocked order."""
-    nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv = _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)
-    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
-    s = stream if stream is not None else torch.cuda.current_stream()
-    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
-    _lib.check(_lib.load().pyscfad_amd_ccsd_t_df_energy(
-        nocc, nvir, naux, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(lo), _ptr(lv),
-        _prec(precision), int(t0), int(t1_), _ptr(out), *_ws(workspace), s.cuda_stream))
+    t0, t1_ = triples if triples is not None else (0, num_triples(numpy.shape(t1)[0]))
+    with _OnStream(stream) as on:
+        nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv = _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov,
+                                                                         Lov, Lvv)
+        on.uses(t1d, t2d, me, fk, oo, ov, lo, lv, workspace)
+        out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+        _lib.check(_lib.load().pyscfad_amd_ccsd_t_df_energy(
+            nocc, nvir, naux, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(lo), _ptr(lv),
+            _prec(precision), int(t0), int(t1_), _ptr(out), *_ws(workspace), on.s.cuda_stream))
+        on.outputs(out)
     return out
 
 
@@ -279,22 +343,30 @@ def energy_vjp_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, et_bar=1.0, pre
     factors the cotangent of ovvv carried through ovvv = Lov^T Lvv
     (Lov_bar in the shape Lov was given in)."""
     lov_shape = tuple(Lov.shape)
-    nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv = _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)
-    z = lambda a: torch.empty_like(a)
-    bars = [z(t1d), z(t2d), z(fk), z(me), z(oo), z(ov), z(lo), z(lv)]
-    out = torch.zeros((), dtype=torch.float64, device=t1d.device)
-    s = stream if stream is not None else torch.cuda.current_stream()
     if hasattr(et_bar, "item"):
         et_bar = et_bar.item()
-    t0, t1_ = triples if triples is not None else (0, num_triples(nocc))
-    _lib.check(_lib.load().pyscfad_amd_ccsd_t_df_energy_vjp(
-        nocc, nvir, naux, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(lo), _ptr(lv),
-        float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars], _ptr(out),
-        *_ws(workspace), s.cuda_stream))
+    t0, t1_ = triples if triples is not None else (0, num_triples(numpy.shape(t1)[0]))
+    with _OnStream(stream) as on:
+        nocc, nvir, naux, t1d, t2d, me, fk, oo, ov, lo, lv = _df_inputs(t1, t2, mo_energy, fock, ovoo, ovov,
+                                                                         Lov, Lvv)
+        on.uses(t1d, t2d, me, fk, oo, ov, lo, lv, workspace)
+        bars = [torch.empty_like(a) for a in (t1d, t2d, fk, me, oo, ov, lo, lv)]
+        out = torch.zeros((), dtype=torch.float64, device=t1d.device)
+        _lib.check(_lib.load().pyscfad_amd_ccsd_t_df_energy_vjp(
+            nocc, nvir, naux, _ptr(me), _ptr(t1d), _ptr(t2d), _ptr(fk), _ptr(oo), _ptr(ov), _ptr(lo), _ptr(lv),
+            float(et_bar), _prec(precision), int(t0), int(t1_), *[_ptr(b) for b in bars], _ptr(out),
+            *_ws(workspace), on.s.cuda_stream))
+        on.outputs(out, *bars)
     bars[6] = bars[6].reshape(lov_shape)
     if return_energy:
         return tuple(bars), out
     return tuple(bars)
+
+
+def _like(t, ref):
+    """t on the device and in the dtype of the autograd input ref (autograd
+    requires a gradient to match its input; inputs may live on the host)."""
+    return t.to(device=ref.device, dtype=ref.dtype)
 
 
 class CCSDTEnergyDF(torch.autograd.Function):
@@ -305,14 +377,15 @@ class CCSDTEnergyDF(torch.autograd.Function):
     def forward(ctx, t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=DEFAULT_PRECISION):
         ctx.save_for_backward(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv)
         ctx.precision = precision
-        return energy_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=precision)
+        return _like(energy_df(t1, t2, mo_energy, fock, ovoo, ovov, Lov, Lvv, precision=precision), t1)
 
     @staticmethod
     def backward(ctx, gbar):
         t1, t2, me, fk, oo, ov, lo, lv = ctx.saved_tensors
         t1b, t2b, fkb, meb, oob, ovb, lob, lvb = energy_vjp_df(t1, t2, me, fk, oo, ov, lo, lv, et_bar=float(gbar),
                                                               precision=ctx.precision)
-        return t1b, t2b, meb, fkb, oob, ovb, lob, lvb, None
+        grads = (t1b, t2b, meb, fkb, oob, ovb, lob, lvb)
+        return tuple(_like(g, x) for g, x in zip(grads, (t1, t2, me, fk, oo, ov, lo, lv))) + (None,)
 
 
 def _ccsd_t_energy_vjp(eris, t1, t2, et_bar, max_memory=None, precision=DEFAULT_PRECISION):
@@ -332,14 +405,15 @@ class CCSDTEnergy(torch.autograd.Function):
     def forward(ctx, t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=DEFAULT_PRECISION):
         ctx.save_for_backward(t1, t2, mo_energy, fock, ovoo, ovov, ovvv)
         ctx.precision = precision
-        return energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=precision)
+        return _like(energy(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, precision=precision), t1)
 
     @staticmethod
     def backward(ctx, gbar):
         t1, t2, me, fk, oo, ov, vv = ctx.saved_tensors
         t1b, t2b, fkb, meb, oob, ovb, vvb = energy_vjp(t1, t2, me, fk, oo, ov, vv,
                                                         et_bar=float(gbar), precision=ctx.precision)
-        return t1b, t2b, meb, fkb, oob, ovb, vvb, None
+        grads = (t1b, t2b, meb, fkb, oob, ovb, vvb)
+        return tuple(_like(g, x) for g, x in zip(grads, (t1, t2, me, fk, oo, ov, vv))) + (None,)
 
 
 class ERIs:
@@ -356,52 +430,58 @@ class ERIs:
             setattr(self, k, val)
 
 
+COTANGENT_LEAVES = ("fock", "mo_energy", "ovoo", "ovov", "ovvv")
+
+
+def eris_cotangent_leaves(paths_and_leaves, named):
+    """Cotangent leaves for a flattened eris pytree (the list of (path, leaf)
+    pairs of jax.tree_util.tree_flatten_with_path): a leaf whose path starts
+    at an attribute named in ``named`` gets that cotangent, every other leaf
+    zeros of its own shape.  This is the pytree contract of the reference's
+    backward (pyscfad/cc/ccsd_t.py:52-77): leaves are matched by their
+    GetAttrKey name, and oooo, vvvv, Lvv or any other leaf gets zeros."""
+    return [named[path[0].name] if path and getattr(path[0], "name", None) in named
+            else numpy.zeros(numpy.shape(leaf)) for path, leaf in paths_and_leaves]
+
+
 def kernel(mycc, eris, t1=None, t2=None, verbose=None, precision=DEFAULT_PRECISION):
     """E(T) of ``mycc``'s amplitudes (pyscfad/cc/ccsd_t.py:28).
 
     With JAX importable this is a ``custom_vjp`` over (eris, t1, t2) whose
     backward returns cotangents for the leaves fock, mo_energy, ovoo, ovov and
-    ovvv by name and zeros for every other leaf (ccsd_t.py:48-77).
+    ovvv by name and zeros for every other leaf (ccsd_t.py:48-77).  Without
+    JAX it returns E(T) as a float.  Device arrays reach the library through
+    DLPack without a host copy.
     """
     if t1 is None:
         t1 = mycc.t1
     if t2 is None:
         t2 = mycc.t2
+
+    def _e(eris, t1, t2):
+        return float(energy(t1, t2, eris.mo_energy, eris.fock, eris.ovoo, eris.ovov, eris.ovvv,
+                            precision=precision))
+
     try:
-        import jax
         from jax import custom_vjp
         from jax.tree_util import tree_flatten_with_path, tree_unflatten
     except ImportError:
-        e = energy(t1, t2, eris.mo_energy, eris.fock, eris.ovoo, eris.ovov, eris.ovvv, precision=precision)
-        return float(e)
+        return _e(eris, t1, t2)
 
     @custom_vjp
     def _ccsd_t_kernel(eris, t1, t2):
-        e = energy(numpy.asarray(t1), numpy.asarray(t2), numpy.asarray(eris.mo_energy),
-                   numpy.asarray(eris.fock), numpy.asarray(eris.ovoo), numpy.asarray(eris.ovov),
-                   numpy.asarray(eris.ovvv), precision=precision)
-        return float(e)
+        return _e(eris, t1, t2)
 
     def _fwd(eris, t1, t2):
         return _ccsd_t_kernel(eris, t1, t2), (eris, t1, t2)
 
     def _bwd(res, ybar):
         eris, t1, t2 = res
-        path_vals, treedef = tree_flatten_with_path(eris)
-        keys = [item[0][0].name for item in path_vals]
-        shapes = [item[1].shape for item in path_vals]
         t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar = _ccsd_t_energy_vjp(
-            eris, numpy.asarray(t1), numpy.asarray(t2), float(ybar), precision=precision)
-        key_to_bar = {"fock": fock_bar, "mo_energy": mo_energy_bar, "ovoo": ovoo_bar,
-                      "ovov": ovov_bar, "ovvv": ovvv_bar}
-        leaves = [None] * len(keys)
-        for k, val in key_to_bar.items():
-            leaves[keys.index(k)] = val
-        for idx, leaf in enumerate(leaves):
-            if leaf is None:
-                leaves[idx] = numpy.zeros(shapes[idx])
-        return tree_unflatten(treedef, leaves), t1_bar, t2_bar
+            eris, t1, t2, float(ybar), precision=precision)
+        flat, treedef = tree_flatten_with_path(eris)
+        named = dict(zip(COTANGENT_LEAVES, (fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar)))
+        return tree_unflatten(treedef, eris_cotangent_leaves(flat, named)), t1_bar, t2_bar
 
     _ccsd_t_kernel.defvjp(_fwd, _bwd)
-    del jax
     return _ccsd_t_kernel(eris, t1, t2)

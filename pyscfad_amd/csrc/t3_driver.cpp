@@ -61,6 +61,7 @@ hipError_t t3_prep_df(double* LovP, double* LvvF, const double* Lov, const doubl
 hipError_t t3_finalize_df(double* Lov_bar, double* Lvv_bar, const double* LovBar, const double* LvvBar, int naux,
                           int o, int v, int V, int NP, hipStream_t s);
 hipError_t t3_copy_f64(double* dst, const double* src, int n, hipStream_t s);
+hipError_t t3_scale_f64(double* x, long long n, const double* scale, hipStream_t s);
 hipError_t t3_zero_f64(double* p, long long n, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2hacc,
@@ -1341,8 +1342,10 @@ struct Events4 {
 int autotune(int nocc, int nvir, int precision, int* cfg_out, hipStream_t st);
 bool autotune_enabled();
 
+// batch > 0 fixes the triples per batch (else the tuned or default choice);
+// allow_tune = false never starts the autotuner (which allocates).
 int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes, void* stream,
-        long long t_begin = 0, long long t_end = -1) {
+        long long t_begin = 0, long long t_end = -1, int batch = -1, bool allow_tune = true) {
   g_last_error.clear();
   int rc = check_problem(p, precision);
   if (rc) return rc;
@@ -1351,7 +1354,7 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
   if (t_begin < 0 || t_begin > t_end || t_end > ntrip)
     return fail(PYSCFAD_AMD_ERR_ARGS, "triple range must satisfy 0 <= begin <= end <= nocc(nocc+1)(nocc+2)/6");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if (autotune_enabled()) {
+  if (allow_tune && autotune_enabled()) {
     const Dims d = make_dims(p.nocc, p.nvir, precision);
     if (tuned_cfg(precision, FAM_GEMM_W, d.V, d.Op) < 0 && (rc = autotune(p.nocc, p.nvir, precision, nullptr, st)))
       return rc;
@@ -1362,7 +1365,7 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
   c.t_begin = t_begin;
   c.t_end = t_end;
   bind(c, p);
-  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end, -1, p.naux);
+  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end, batch, p.naux);
   bool own = false;
   if (workspace) {
     if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
@@ -1683,6 +1686,7 @@ constexpr int MAX_GRAPH_NODES = 2048;
 
 std::mutex g_graph_mu;
 std::vector<GraphEntry> g_graphs;   // least recently used first
+long long g_graph_replays = 0, g_graph_captures = 0;   // guarded by g_graph_mu
 constexpr size_t MAX_GRAPHS = 4;
 
 void destroy_entry(GraphEntry& g) {
@@ -1731,14 +1735,27 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
   char* ws = reinterpret_cast<char*>(workspace);
   reset_timing();
 
-  std::vector<char> key(sizeof(int) * 3 + sizeof(void*) + sizeof(size_t) + sizeof(PyscfadAmdInstance) * n);
+  // the cache key holds every field of the call, written one by one: the
+  // struct's padding bytes (whatever the caller's memory held) never enter it
+  std::vector<char> key;
+  auto put = [&key](const void* src, size_t nb) {
+    const char* b = reinterpret_cast<const char*>(src);
+    key.insert(key.end(), b, b + nb);
+  };
   {
-    char* k = key.data();
     const int head[3] = {n, precision, with_vjp};
-    std::memcpy(k, head, sizeof(head)); k += sizeof(head);
-    std::memcpy(k, &workspace, sizeof(void*)); k += sizeof(void*);
-    std::memcpy(k, &workspace_bytes, sizeof(size_t)); k += sizeof(size_t);
-    std::memcpy(k, inst, sizeof(PyscfadAmdInstance) * n);
+    put(head, sizeof(head));
+    put(&workspace, sizeof(void*));
+    put(&workspace_bytes, sizeof(size_t));
+    for (int x = 0; x < n; x++) {
+      const PyscfadAmdInstance& I = inst[x];
+      const int dims[3] = {I.nocc, I.nvir, I.ovvv_packed};
+      put(dims, sizeof(dims));
+      put(&I.et_bar, sizeof(double));
+      const void* ptrs[15] = {I.mo_energy, I.t1, I.t2, I.fock, I.ovoo, I.ovov, I.ovvv, I.t1_bar, I.t2_bar,
+                              I.fock_bar, I.mo_energy_bar, I.ovoo_bar, I.ovov_bar, I.ovvv_bar, I.e_out};
+      put(ptrs, sizeof(ptrs));
+    }
   }
   std::lock_guard<std::mutex> lock(g_graph_mu);
   if (!g_timing) {
@@ -1749,6 +1766,7 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
       g_graphs.push_back(hit);
       const hipError_t e = launch_entry(hit, st);
       if (e != hipSuccess) return fail(PYSCFAD_AMD_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+      g_graph_replays++;
       return PYSCFAD_AMD_OK;
     }
   }
@@ -1856,6 +1874,7 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
     g_graphs.erase(g_graphs.begin());
   }
   g_graphs.push_back(g);
+  g_graph_captures++;
   return PYSCFAD_AMD_OK;
 }
 
@@ -1985,36 +2004,53 @@ const char* pyscfad_amd_last_error(void) { return g_last_error.c_str(); }
 // handler signature needs no XLA headers.  buffers[] holds the operands then
 // the results, opaque a PyscfadAmdXlaOpaque.  The legacy ABI has no error
 // channel: a failure is reported on stderr and the results are left unset.
-void pyscfad_amd_xla_ccsd_t_energy(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
-  PyscfadAmdXlaOpaque op{};
+namespace {
+bool xla_opaque(const char* opaque, size_t opaque_len, PyscfadAmdXlaOpaque& op, const char* who) {
   if (!opaque || opaque_len < sizeof(op)) {
-    std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy: opaque of %zu bytes, need %zu\n", opaque_len, sizeof(op));
-    return;
+    std::fprintf(stderr, "%s: opaque of %zu bytes, need %zu\n", who, opaque_len, sizeof(op));
+    return false;
   }
   std::memcpy(&op, opaque, sizeof(op));
+  return true;
+}
+}  // namespace
+
+void pyscfad_amd_xla_ccsd_t_energy(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+  PyscfadAmdXlaOpaque op{};
+  if (!xla_opaque(opaque, opaque_len, op, "pyscfad_amd_xla_ccsd_t_energy")) return;
   const double* in[7];
   for (int x = 0; x < 7; x++) in[x] = reinterpret_cast<const double*>(buffers[x]);
+  void* ws = op.workspace_bytes ? buffers[8] : nullptr;
   const int rc = pyscfad_amd_ccsd_t_energy(op.nocc, op.nvir, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
                                            op.ovvv_packed, op.precision, reinterpret_cast<double*>(buffers[7]),
-                                           nullptr, 0, stream);
+                                           ws, (size_t)op.workspace_bytes, stream);
   if (rc) std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy: error %d: %s\n", rc, g_last_error.c_str());
 }
 
+// The cotangent of E(T) is the device scalar buffers[7] (traced inside a jitted
+// backward): the VJP is linear in it, so the cotangents are formed for 1 and
+// then scaled by it on the stream.
 void pyscfad_amd_xla_ccsd_t_energy_vjp(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
   PyscfadAmdXlaOpaque op{};
-  if (!opaque || opaque_len < sizeof(op)) {
-    std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy_vjp: opaque of %zu bytes, need %zu\n", opaque_len,
-                 sizeof(op));
-    return;
-  }
-  std::memcpy(&op, opaque, sizeof(op));
+  if (!xla_opaque(opaque, opaque_len, op, "pyscfad_amd_xla_ccsd_t_energy_vjp")) return;
   const double* in[7];
   double* out[7];
   for (int x = 0; x < 7; x++) in[x] = reinterpret_cast<const double*>(buffers[x]);
-  for (int x = 0; x < 7; x++) out[x] = reinterpret_cast<double*>(buffers[7 + x]);
-  const int rc = pyscfad_amd_ccsd_t_energy_vjp(op.nocc, op.nvir, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
-                                               op.ovvv_packed, op.et_bar, op.precision, out[0], out[1], out[2],
-                                               out[3], out[4], out[5], out[6], nullptr, nullptr, 0, stream);
+  const double* et_bar = reinterpret_cast<const double*>(buffers[7]);
+  for (int x = 0; x < 7; x++) out[x] = reinterpret_cast<double*>(buffers[8 + x]);
+  void* ws = op.workspace_bytes ? buffers[15] : nullptr;
+  int rc = pyscfad_amd_ccsd_t_energy_vjp(op.nocc, op.nvir, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
+                                         op.ovvv_packed, 1.0, op.precision, out[0], out[1], out[2], out[3], out[4],
+                                         out[5], out[6], nullptr, ws, (size_t)op.workspace_bytes, stream);
+  if (!rc) {
+    const long long o = op.nocc, v = op.nvir, n = o + v;
+    const long long len[7] = {o * v, o * o * v * v, n * n, n, o * v * o * o, o * v * o * v,
+                              op.ovvv_packed ? o * v * (v * (v + 1) / 2) : o * v * v * v};
+    for (int x = 0; x < 7 && !rc; x++) {
+      const hipError_t e = t3_scale_f64(out[x], len[x], et_bar, reinterpret_cast<hipStream_t>(stream));
+      if (e != hipSuccess) rc = fail(PYSCFAD_AMD_ERR_HIP, std::string("scale: ") + hipGetErrorString(e));
+    }
+  }
   if (rc) std::fprintf(stderr, "pyscfad_amd_xla_ccsd_t_energy_vjp: error %d: %s\n", rc, g_last_error.c_str());
 }
 
@@ -2029,6 +2065,12 @@ int pyscfad_amd_last_timing(double* ms, long long* launches, int n) {
   return m;
 }
 
-int pyscfad_amd_abi_version(void) { return 1; }
+void pyscfad_amd_graph_cache_stats(long long* replays, long long* captures) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  if (replays) *replays = g_graph_replays;
+  if (captures) *captures = g_graph_captures;
+}
+
+int pyscfad_amd_abi_version(void) { return 2; }
 
 }  // extern "C"

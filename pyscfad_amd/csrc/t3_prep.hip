@@ -709,6 +709,14 @@ __global__ void zero_f64(double* __restrict__ p, long long n) {
     p[e] = 0.0;
 }
 
+// x[e] *= *scale, the factor read from device memory (a traced cotangent)
+__global__ void scale_f64(double* __restrict__ x, long long n, const double* __restrict__ scale) {
+  const double f = *scale;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x)
+    x[e] *= f;
+}
+
 __global__ void copy_f64(double* __restrict__ dst, const double* __restrict__ src, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) dst[e] = src[e];
@@ -726,6 +734,12 @@ extern "C" hipError_t t3_zero(void* p, size_t bytes, hipStream_t s) {
 extern "C" hipError_t t3_zero_f64(double* p, long long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(zero_f64, grid_for(n), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t t3_scale_f64(double* x, long long n, const double* scale, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scale_f64, grid_for(n), dim3(256), 0, s, x, n, scale);
   return hipGetLastError();
 }
 

@@ -24,6 +24,13 @@ For an instance with ``nocc = o``, ``nvir = v``, ``n = o + v``:
     exercise the t2*fvo term.
 Draw order: occupied energies, virtual energies, L, t1, t2 noise, fvo.
 
+``integral_scale = s`` multiplies L by s and the t2 noise by s^2 (the same
+draws): the integrals and t2 shrink by s^2.  With the generator as stated
+(s = 1) the integrals grow with n -- |E(T)| is about 3e8 Eh at nocc=32,
+nvir=256 -- and s = 1/sqrt(n) gives amplitudes and an E(T) of molecular size
+(|t2| ~ 1e-2, |E(T)| ~ 1e-2 Eh), the scale at which absolute accuracy in Eh
+is meaningful.
+
 Large instances can build the integral slices with torch on a GPU
 (``device='cuda'``); the random draws are identical either way.
 """
@@ -49,7 +56,7 @@ def _draws(nocc, nvir, seed, fvo_scale):
     return mo_energy, L, t1, t2_noise, fvo
 
 
-def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None, df=False):
+def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None, df=False, integral_scale=1.0):
     """Return a dict with t1, t2, mo_energy, fock, ovoo, ovov, ovvv (FP64).
 
     With ``device=None`` everything is NumPy on the host.  With a torch device
@@ -61,6 +68,9 @@ def make_instance(nocc, nvir, seed, fvo_scale=0.0, packed=True, device=None, df=
     """
     o, v = nocc, nvir
     mo_energy, L, t1, t2_noise, fvo = _draws(o, v, seed, fvo_scale)
+    if integral_scale != 1.0:
+        L *= integral_scale
+        t2_noise *= integral_scale ** 2
     n = o + v
     fock = np.diag(mo_energy)
     if fvo is not None:

@@ -25,7 +25,7 @@ def test_library_exports_all_symbols():
     lib = _lib.load()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.pyscfad_amd_abi_version() == 1
+    assert lib.pyscfad_amd_abi_version() == 2
 
 
 def test_workspace_bytes():

@@ -47,7 +47,7 @@ def test_f32_agrees_with_f64(gpu_lib):
     e32, b32 = _run(d, "f32")
     assert abs(e32 - e64) <= 1e-6 * abs(e64)
     for x, y in zip(b32, b64):
-        assert np.abs(x - y).max() <= 1e-5 * np.abs(y).max()
+        assert np.abs(x - y).max() <= 1e-6 * np.abs(y).max()
 
 
 def test_triple_slices_partition(gpu_lib):

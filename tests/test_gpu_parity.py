@@ -62,27 +62,39 @@ def test_vjp_f64(gpu_lib, o, v, seed, fs, packed):
         assert err <= max(REL_TOL * scale, ABS_FLOOR), (name, err, scale)
 
 
-def test_xla_legacy_handlers(gpu_lib):
-    """The XLA custom-call entry points give the same results as the direct ABI."""
+@pytest.mark.parametrize("with_ws", [False, True])
+def test_xla_legacy_handlers(gpu_lib, with_ws):
+    """The XLA custom-call entry points give the results of the direct ABI:
+    et_bar arrives as a device scalar operand (a traced cotangent), and with
+    a caller workspace result the call allocates nothing."""
     import ctypes
     import torch
     from pyscfad_amd import _lib
     from pyscfad_amd.cc import ccsd_t
-    d = _inst(4, 21, 8, 0.02, True)
+    o, v = 4, 21
+    d = _inst(o, v, 8, 0.02, True)
     keys = ("mo_energy", "t1", "t2", "fock", "ovoo", "ovov", "ovvv")
     ins = [torch.as_tensor(d[k]).cuda().contiguous() for k in keys]
-    e_out = torch.zeros((), dtype=torch.float64, device="cuda")
-    op = _lib.XlaOpaque(4, 21, _lib.PREC_F64, 1, 0.5)
     stream = torch.cuda.current_stream().cuda_stream
-    bufs = (ctypes.c_void_p * 8)(*[t.data_ptr() for t in ins], e_out.data_ptr())
+    nws = ccsd_t.workspace_bytes(o, v, "f64", with_vjp=True) if with_ws else 0
+    ws = [torch.empty(nws, dtype=torch.uint8, device="cuda")] if with_ws else []
+    op = _lib.XlaOpaque(o, v, _lib.PREC_F64, 1, nws)
+    e_out = torch.zeros((), dtype=torch.float64, device="cuda")
+    bufs = (ctypes.c_void_p * (8 + len(ws)))(*[t.data_ptr() for t in ins + [e_out] + ws])
     gpu_lib.pyscfad_amd_xla_ccsd_t_energy(stream, bufs, bytes(op), ctypes.sizeof(op))
     torch.cuda.synchronize()
     assert abs(float(e_out) - orc.energy(**d)) <= E_TOL
-    outs = [torch.empty_like(ins[i]) for i in (1, 2, 3, 0, 4, 5, 6)]   # t1,t2,fock,mo_energy,ovoo,ovov,ovvv
-    bufs = (ctypes.c_void_p * 14)(*[t.data_ptr() for t in ins], *[t.data_ptr() for t in outs])
-    gpu_lib.pyscfad_amd_xla_ccsd_t_energy_vjp(stream, bufs, bytes(op), ctypes.sizeof(op))
-    torch.cuda.synchronize()
-    ref = ccsd_t.energy_vjp(*[d[k] for k in ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")],
-                            et_bar=0.5)
-    for a, b in zip(outs, ref):
-        assert torch.equal(a, b)
+    args = [d[k] for k in ("t1", "t2", "mo_energy", "fock", "ovoo", "ovov", "ovvv")]
+    for et_bar in (0.5, 0.3):
+        outs = [torch.empty_like(ins[i]) for i in (1, 2, 3, 0, 4, 5, 6)]   # t1,t2,fock,mo_energy,ovoo,ovov,ovvv
+        eb = torch.tensor(et_bar, dtype=torch.float64, device="cuda")
+        bufs = (ctypes.c_void_p * (15 + len(ws)))(*[t.data_ptr() for t in ins + [eb] + outs + ws])
+        gpu_lib.pyscfad_amd_xla_ccsd_t_energy_vjp(stream, bufs, bytes(op), ctypes.sizeof(op))
+        torch.cuda.synchronize()
+        ref = ccsd_t.energy_vjp(*args, et_bar=et_bar)
+        for a, b in zip(outs, ref):
+            if et_bar == 0.5:
+                # a power-of-two factor scales every step exactly
+                assert torch.equal(a, b)
+            else:
+                assert torch.allclose(a, b, rtol=1e-13, atol=1e-15 * float(torch.abs(b).max()))

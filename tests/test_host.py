@@ -1,0 +1,50 @@
+"""Host-side logic of the drop-in that needs no GPU: the pytree contract of
+the backward (pyscfad/cc/ccsd_t.py:52-77) on a stub flatten result, and the
+loud failure of the product path without a GPU."""
+import types
+
+import numpy as np
+import pytest
+
+
+def _key(name):
+    # jax.tree_util.GetAttrKey exposes the attribute as .name
+    return types.SimpleNamespace(name=name)
+
+
+def test_eris_cotangent_leaves_by_name():
+    from pyscfad_amd.cc import ccsd_t
+    leaves = {"fock": np.ones((3, 3)), "mo_energy": np.ones(3), "oooo": np.ones((1, 1, 1, 1)),
+              "ovoo": np.ones((1, 2, 1, 1)), "ovov": np.ones((1, 2, 1, 2)), "ovvv": np.ones((1, 2, 3)),
+              "vvvv": np.ones((2, 2, 2, 2)), "Lvv": np.ones((5, 3))}
+    flat = [((_key(k),), val) for k, val in leaves.items()]
+    named = {k: np.full(np.shape(leaves[k]), float(x + 2)) for x, k in enumerate(ccsd_t.COTANGENT_LEAVES)}
+    out = ccsd_t.eris_cotangent_leaves(flat, named)
+    assert len(out) == len(flat)
+    for (path, leaf), got in zip(flat, out):
+        name = path[0].name
+        assert np.shape(got) == np.shape(leaf), name
+        if name in named:
+            assert got is named[name]
+        else:
+            assert not np.any(got), name
+
+
+def test_eris_cotangent_leaves_missing_cotangent_leaf():
+    """A leaf that the backward has no cotangent for is zero-filled, and a
+    cotangent with no matching leaf is dropped."""
+    from pyscfad_amd.cc import ccsd_t
+    flat = [((_key("fock"),), np.ones((2, 2))), ((_key("extra"),), np.ones(4))]
+    out = ccsd_t.eris_cotangent_leaves(flat, {"fock": np.eye(2), "ovvv": np.ones(3)})
+    assert np.array_equal(out[0], np.eye(2)) and np.array_equal(out[1], np.zeros(4))
+
+
+def test_product_path_fails_loudly_without_gpu():
+    import torch
+    from pyscfad_amd import _lib
+    from pyscfad_amd.cc import ccsd_t
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(_lib.T3Error):
+        ccsd_t.energy(np.zeros((1, 1)), np.zeros((1, 1, 1, 1)), np.zeros(2), np.zeros((2, 2)),
+                      np.zeros((1, 1, 1, 1)), np.zeros((1, 1, 1, 1)), np.zeros((1, 1, 1)))
