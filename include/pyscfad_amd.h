@@ -73,6 +73,29 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double *mo_energy, c
                                   double *ovov_bar, double *ovvv_bar, double *e_out,
                                   void *workspace, size_t workspace_bytes, void *stream);
 
+/*
+ * pyscfad_amd_ccsd_t_energy_vjp without a workspace argument and without any
+ * device allocation.  No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch is one
+ * file-scope device array of 1 GiB that each GPU the library runs on holds
+ * until the process ends; the call picks the largest number of triples per
+ * batch (up to the usual choice) whose plan fits it, and rejects an instance
+ * that does not fit even at one triple per batch with
+ * PYSCFAD_AMD_ERR_WORKSPACE (FP64 energy+VJP fits up to about nocc=16,
+ * nvir=128).  Arguments, outputs and results are those of
+ * pyscfad_amd_ccsd_t_energy_vjp (equal to rounding: the batch size may
+ * differ).  All calls on one GPU share its array: calls that overlap on one
+ * GPU (other host threads, other streams) race and must be serialised by
+ * the caller.  The autotuner is never started from here.
+ */
+int pyscfad_amd_ccsd_t_energy_vjp_noalloc(int nocc, int nvir, const double *mo_energy, const double *t1,
+                                          const double *t2, const double *fock, const double *ovoo,
+                                          const double *ovov, const double *ovvv, int ovvv_packed,
+                                          double et_bar, int precision, double *t1_bar, double *t2_bar,
+                                          double *fock_bar, double *mo_energy_bar, double *ovoo_bar,
+                                          double *ovov_bar, double *ovvv_bar, double *e_out, void *stream);
+
 /* Workspace needed by the calls above (with_vjp = 0: energy, 1: energy_vjp);
  * 0 for invalid arguments.  No counterpart in the reference, whose C kernel
  * mallocs its own scratch (ccsd_t_vjp.c:352-444); this follows the XLA FFI

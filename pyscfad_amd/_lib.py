@@ -18,6 +18,7 @@ PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3, "bf16x6":
 EXPORTS = (
     "pyscfad_amd_ccsd_t_energy",
     "pyscfad_amd_ccsd_t_energy_vjp",
+    "pyscfad_amd_ccsd_t_energy_vjp_noalloc",
     "pyscfad_amd_ccsd_t_workspace_bytes",
     "pyscfad_amd_ccsd_t_num_triples",
     "pyscfad_amd_ccsd_t_energy_slice",
@@ -65,6 +66,8 @@ def load():
     lib.pyscfad_amd_ccsd_t_energy.argtypes = [i, i] + [dp] * 7 + [i, i, dp, vp, sz, vp]
     lib.pyscfad_amd_ccsd_t_energy_vjp.restype = i
     lib.pyscfad_amd_ccsd_t_energy_vjp.argtypes = [i, i] + [dp] * 7 + [i, d, i] + [dp] * 8 + [vp, sz, vp]
+    lib.pyscfad_amd_ccsd_t_energy_vjp_noalloc.restype = i
+    lib.pyscfad_amd_ccsd_t_energy_vjp_noalloc.argtypes = [i, i] + [dp] * 7 + [i, d, i] + [dp] * 8 + [vp]
     lib.pyscfad_amd_ccsd_t_workspace_bytes.restype = sz
     lib.pyscfad_amd_ccsd_t_workspace_bytes.argtypes = [i, i, i, i]
     ll = ctypes.c_longlong

@@ -62,6 +62,8 @@ hipError_t t3_finalize_df(double* Lov_bar, double* Lvv_bar, const double* LovBar
                           int o, int v, int V, int NP, hipStream_t s);
 hipError_t t3_copy_f64(double* dst, const double* src, int n, hipStream_t s);
 hipError_t t3_scale_f64(double* x, long long n, const double* scale, hipStream_t s);
+hipError_t t3_noalloc_scratch(void** ptr, size_t* bytes);
+size_t t3_noalloc_scratch_bytes(void);
 hipError_t t3_zero_f64(double* p, long long n, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2hacc,
@@ -1914,6 +1916,37 @@ int pyscfad_amd_ccsd_t_energy_vjp(int nocc, int nvir, const double* mo_energy, c
   return run(make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar, t1_bar,
                           t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out),
              precision, workspace, workspace_bytes, stream);
+}
+
+int pyscfad_amd_ccsd_t_energy_vjp_noalloc(int nocc, int nvir, const double* mo_energy, const double* t1,
+                                          const double* t2, const double* fock, const double* ovoo,
+                                          const double* ovov, const double* ovvv, int ovvv_packed, double et_bar,
+                                          int precision, double* t1_bar, double* t2_bar, double* fock_bar,
+                                          double* mo_energy_bar, double* ovoo_bar, double* ovov_bar,
+                                          double* ovvv_bar, double* e_out, void* stream) {
+  g_last_error.clear();
+  const Problem p = make_problem(nocc, nvir, mo_energy, t1, t2, fock, ovoo, ovov, ovvv, ovvv_packed, true, et_bar,
+                                 t1_bar, t2_bar, fock_bar, mo_energy_bar, ovoo_bar, ovov_bar, ovvv_bar, e_out);
+  int rc = check_problem(p, precision);
+  if (rc) return rc;
+  // the largest batch (up to the usual choice, halving) whose plan fits the scratch
+  const size_t cap = t3_noalloc_scratch_bytes();
+  int B = choose_batch(make_dims(nocc, nvir, precision));
+  size_t need = 0;
+  for (;; B /= 2) {
+    need = plan_bytes(nocc, nvir, precision, true, nullptr, nullptr, 0, -1, B);
+    if (need <= cap || B == 1) break;
+  }
+  if (need > cap)
+    return fail(PYSCFAD_AMD_ERR_WORKSPACE, "nocc=" + std::to_string(nocc) + ", nvir=" + std::to_string(nvir) +
+                                               " needs " + std::to_string(need) +
+                                               " bytes of scratch even at one triple per batch; the allocation-free "
+                                               "entry point holds " + std::to_string(cap));
+  void* scratch = nullptr;
+  size_t bytes = 0;
+  const hipError_t e = t3_noalloc_scratch(&scratch, &bytes);
+  if (e != hipSuccess) return fail(PYSCFAD_AMD_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+  return run(p, precision, scratch, bytes, stream, 0, -1, B, false);
 }
 
 int pyscfad_amd_ccsd_t_autotune(int nocc, int nvir, int precision, int* cfg_out, void* stream) {
