@@ -16,7 +16,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "pyscfad_amd", "_build")
-OBJS = [os.path.join(BUILD, f) for f in ("t3_gemm.hip.o", "t3_epilogue.hip.o", "t3_prep.hip.o")]
+OBJS = [os.path.join(BUILD, f) for f in ("t3_gemm.hip.o", "t3_epilogue.hip.o", "t3_prep.hip.o", "t3_noalloc.hip.o")]
 HIPCC = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
 
 
