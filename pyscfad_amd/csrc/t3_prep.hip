@@ -15,45 +15,92 @@ __device__ __forceinline__ long long tri_index(long long f, long long y) {
   return f >= y ? f * (f + 1) / 2 + y : y * (y + 1) / 2 + f;
 }
 
+// ---------------- tiled re-layouts (32 x 32 tiles through LDS) ----------------
+// A block of 256 threads takes one 32 x 32 tile of the (row, col) plane of a
+// run of TR_SLABS consecutive slabs; thread (tx = t % 32, ty = t / 32) moves
+// the elements (ty + 8 q, tx), q = 0..3, so every global access is 32
+// consecutive elements of one row.  Tiles whose source is the transpose of
+// their destination go through LDS (row pitch 33: conflict-free both ways).
+// Block b takes tile b % ntiles of slab run b / ntiles.
+constexpr int TR_T = 32, TR_SLABS = 8;
+
+// ovvvT[p][x][y][f] = ovvv[p,x,f,y] for one slab s = p * V + x (zero past v):
+// packed input pair (f, y) at tri(max, min) of the slab's v(v+1)/2 pairs.
 template <class TP>
-__global__ void prep_ovvv(TP* __restrict__ out, const double* __restrict__ ovvv, int o, int v, int V,
-                          int packed) {
-  const long long total = (long long)o * V * V * V;
+__global__ void __launch_bounds__(256) prep_ovvv(TP* __restrict__ out, const double* __restrict__ ovvv, int o,
+                                                 int v, int V, int packed) {
+  __shared__ double tile[TR_T][TR_T + 1];
+  const int nt = V / TR_T, ntiles = nt * nt;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int tile_id = (int)(blockIdx.x % (unsigned)ntiles);
+  const long long run = blockIdx.x / (unsigned)ntiles;
+  const int Y0 = (tile_id / nt) * TR_T, F0 = (tile_id % nt) * TR_T;
   const long long npair = (long long)v * (v + 1) / 2;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const int f = (int)(e % V);
-    const int y = (int)((e / V) % V);
-    const int x = (int)((e / ((long long)V * V)) % V);
-    const int p = (int)(e / ((long long)V * V * V));
-    double val = 0.0;
-    if (x < v && y < v && f < v) {
-      const long long px = (long long)p * v + x;
-      val = packed ? ovvv[px * npair + tri_index(f, y)] : ovvv[(px * v + f) * v + y];
+  const long long nslab = (long long)o * V;
+  for (int sl = 0; sl < TR_SLABS; sl++) {
+    const long long slab = run * TR_SLABS + sl;
+    if (slab >= nslab) break;
+    const int x = (int)(slab % V);
+    const long long p = slab / V;
+    TP* dst = out + slab * V * V;
+    if (x >= v) {
+      for (int q = 0; q < 4; q++) dst[(long long)(Y0 + ty + 8 * q) * V + F0 + tx] = (TP)0.0;
+      continue;
     }
-    out[e] = (TP)val;
+    const double* src = ovvv + (p * v + x) * (packed ? npair : (long long)v * v);
+    if (packed && F0 < Y0) {
+      // below the diagonal (f < y): pair (y, f), consecutive in f
+      for (int q = 0; q < 4; q++) {
+        const int y = Y0 + ty + 8 * q, f = F0 + tx;
+        const double val = (y < v && f < v) ? src[(long long)y * (y + 1) / 2 + f] : 0.0;
+        dst[(long long)y * V + f] = (TP)val;
+      }
+      continue;
+    }
+    // tile[fi][yi] = ovvv[p,x,F0+fi,Y0+yi]: pair (f, y) (f >= y above the
+    // diagonal, consecutive in y) or dense [f][y]
+    for (int q = 0; q < 4; q++) {
+      const int f = F0 + ty + 8 * q, y = Y0 + tx;
+      double val = 0.0;
+      if (f < v && y < v) val = packed ? src[tri_index(f, y)] : src[(long long)f * v + y];
+      tile[ty + 8 * q][tx] = val;
+    }
+    __syncthreads();
+    for (int q = 0; q < 4; q++) dst[(long long)(Y0 + ty + 8 * q) * V + F0 + tx] = (TP)tile[tx][ty + 8 * q];
+    __syncthreads();
   }
 }
 
-// t2T[r][q][z][f] = t2[r,q,z,f];  t2H[r][m][y][z] = t2[r,m,z,y]
+// t2T[r][q][z][f] = t2[r,q,z,f];  t2H[r][m][y][z] = t2[r,m,z,y]  (per (r, q) slab, zero padded)
 template <class TP>
-__global__ void prep_t2(TP* __restrict__ t2T, TP* __restrict__ t2H, const double* __restrict__ t2,
-                        int o, int v, int Op, int V) {
-  const long long total = (long long)Op * Op * V * V;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const int c1 = (int)(e % V);
-    const int c0 = (int)((e / V) % V);
-    const int q = (int)((e / ((long long)V * V)) % Op);
-    const int r = (int)(e / ((long long)V * V * Op));
-    double a = 0.0, b = 0.0;
-    if (r < o && q < o && c0 < v && c1 < v) {
-      const long long base = ((long long)r * o + q) * v;
-      a = t2[(base + c0) * v + c1];
-      b = t2[(base + c1) * v + c0];
+__global__ void __launch_bounds__(256) prep_t2(TP* __restrict__ t2T, TP* __restrict__ t2H,
+                                               const double* __restrict__ t2, int o, int v, int Op, int V) {
+  __shared__ double tile[TR_T][TR_T + 1];
+  const int nt = V / TR_T, ntiles = nt * nt;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int tile_id = (int)(blockIdx.x % (unsigned)ntiles);
+  const long long run = blockIdx.x / (unsigned)ntiles;
+  const int R0 = (tile_id / nt) * TR_T, C0 = (tile_id % nt) * TR_T;
+  const long long nslab = (long long)Op * Op;
+  for (int sl = 0; sl < TR_SLABS; sl++) {
+    const long long slab = run * TR_SLABS + sl;
+    if (slab >= nslab) break;
+    const int q = (int)(slab % Op), r = (int)(slab / Op);
+    const bool live = r < o && q < o;
+    const double* src = t2 + ((long long)r * o + q) * v * v;
+    TP* dT = t2T + slab * V * V;
+    TP* dH = t2H + slab * V * V;
+    // tile[zi][fi] = t2[r,q,R0+zi,C0+fi]
+    for (int k = 0; k < 4; k++) {
+      const int z = R0 + ty + 8 * k, f = C0 + tx;
+      const double val = (live && z < v && f < v) ? src[(long long)z * v + f] : 0.0;
+      tile[ty + 8 * k][tx] = val;
+      dT[(long long)z * V + f] = (TP)val;
     }
-    t2T[e] = (TP)a;
-    t2H[e] = (TP)b;
+    __syncthreads();
+    // t2H row y = C0 + yi, column z = R0 + zi
+    for (int k = 0; k < 4; k++) dH[(long long)(C0 + ty + 8 * k) * V + R0 + tx] = (TP)tile[tx][ty + 8 * k];
+    __syncthreads();
   }
 }
 
@@ -393,39 +440,89 @@ vpart_scatter(const VpSlab* __restrict__ slabs, int nslab, const VpContrib* __re
 
 // ---------------- finalisation into the reference's cotangent layouts --------------
 // (ccsd_t.py:179-192)
-__global__ void fin_ovvv(double* __restrict__ out, const double* __restrict__ acc, int o, int v, int V,
-                         int packed) {
+__global__ void __launch_bounds__(256) fin_ovvv(double* __restrict__ out, const double* __restrict__ acc, int o,
+                                                int v, int V, int packed) {
+  // packed: out[p,x, pair (f, y)] = acc[p][x][y][f] + acc[p][x][f][y] (f > y) or
+  // acc[p][x][f][f]; tiles (F0 rows, Y0 columns) on or below the diagonal only.
+  // dense: out[p,x,f,y] = acc[p][x][y][f].
+  __shared__ double tile[TR_T][TR_T + 1];
+  const int nt = (v + TR_T - 1) / TR_T;
+  const int ntiles = packed ? nt * (nt + 1) / 2 : nt * nt;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int tile_id = (int)(blockIdx.x % (unsigned)ntiles);
+  const long long run = blockIdx.x / (unsigned)ntiles;
+  int tf, tyy;
+  if (packed) {
+    tf = 0;
+    while ((tf + 1) * (tf + 2) / 2 <= tile_id) tf++;
+    tyy = tile_id - tf * (tf + 1) / 2;
+  } else {
+    tf = tile_id / nt;
+    tyy = tile_id % nt;
+  }
+  const int F0 = tf * TR_T, Y0 = tyy * TR_T;
   const long long npair = (long long)v * (v + 1) / 2;
-  const long long total = packed ? (long long)o * v * npair : (long long)o * v * v * v;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    if (packed) {
-      const long long pr = e % npair;
-      const long long px = e / npair;
-      const int x = (int)(px % v), p = (int)(px / v);
-      long long f = (long long)((sqrt(8.0 * (double)pr + 1.0) - 1.0) / 2.0);
-      while (f * (f + 1) / 2 > pr) f--;
-      while ((f + 1) * (f + 2) / 2 <= pr) f++;
-      const long long y = pr - f * (f + 1) / 2;
-      const long long base = ((long long)p * V + x) * V;
-      const double a = acc[(base + y) * V + f];
-      out[e] = (f == y) ? a : a + acc[(base + f) * V + y];
-    } else {
-      const int y = (int)(e % v), f = (int)((e / v) % v), x = (int)((e / ((long long)v * v)) % v);
-      const int p = (int)(e / ((long long)v * v * v));
-      out[e] = acc[((((long long)p * V + x) * V + y) * V) + f];
+  const long long nslab = (long long)o * v;
+  for (int sl = 0; sl < TR_SLABS; sl++) {
+    const long long slab = run * TR_SLABS + sl;
+    if (slab >= nslab) break;
+    const int x = (int)(slab % v);
+    const long long p = slab / v;
+    const double* a = acc + (p * V + x) * V * V;
+    // tile[yi][fi] = acc[p][x][Y0+yi][F0+fi]
+    for (int q = 0; q < 4; q++) {
+      const int y = Y0 + ty + 8 * q, f = F0 + tx;
+      tile[ty + 8 * q][tx] = (y < v && f < v) ? a[(long long)y * V + f] : 0.0;
     }
+    __syncthreads();
+    if (packed) {
+      double* dst = out + slab * npair;
+      for (int q = 0; q < 4; q++) {
+        const int f = F0 + ty + 8 * q, y = Y0 + tx;
+        if (f < v && y <= f) {
+          const double t = tile[tx][ty + 8 * q];
+          dst[(long long)f * (f + 1) / 2 + y] = f == y ? t : t + a[(long long)f * V + y];
+        }
+      }
+    } else {
+      double* dst = out + slab * v * v;
+      for (int q = 0; q < 4; q++) {
+        const int f = F0 + ty + 8 * q, y = Y0 + tx;
+        if (f < v && y < v) dst[(long long)f * v + y] = tile[tx][ty + 8 * q];
+      }
+    }
+    __syncthreads();
   }
 }
 
-// t2_bar[i,j,a,b] += t2hacc[i][j][b][a] (the rest of t2_bar was accumulated in place)
-__global__ void fin_t2(double* __restrict__ out, const double* __restrict__ t2hacc, int o, int v, int Op, int V) {
-  const long long total = (long long)o * o * v * v;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const int b = (int)(e % v), a = (int)((e / v) % v);
-    const int j = (int)((e / ((long long)v * v)) % o), i = (int)(e / ((long long)v * v * o));
-    out[e] += t2hacc[((long long)i * Op + j) * V * V + (long long)b * V + a];
+// t2_bar[i,j,a,b] += t2hacc[i][j][b][a] (the rest of t2_bar was accumulated in
+// place), per (i, j) slab
+__global__ void __launch_bounds__(256) fin_t2(double* __restrict__ out, const double* __restrict__ t2hacc, int o,
+                                              int v, int Op, int V) {
+  __shared__ double tile[TR_T][TR_T + 1];
+  const int nt = (v + TR_T - 1) / TR_T, ntiles = nt * nt;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int tile_id = (int)(blockIdx.x % (unsigned)ntiles);
+  const long long run = blockIdx.x / (unsigned)ntiles;
+  const int A0 = (tile_id / nt) * TR_T, B0 = (tile_id % nt) * TR_T;
+  const long long nslab = (long long)o * o;
+  for (int sl = 0; sl < TR_SLABS; sl++) {
+    const long long slab = run * TR_SLABS + sl;
+    if (slab >= nslab) break;
+    const int j = (int)(slab % o), i = (int)(slab / o);
+    const double* h = t2hacc + ((long long)i * Op + j) * V * V;
+    // tile[bi][ai] = t2hacc[i][j][B0+bi][A0+ai]
+    for (int q = 0; q < 4; q++) {
+      const int b = B0 + ty + 8 * q, a = A0 + tx;
+      tile[ty + 8 * q][tx] = (a < v && b < v) ? h[(long long)b * V + a] : 0.0;
+    }
+    __syncthreads();
+    double* dst = out + slab * v * v;
+    for (int q = 0; q < 4; q++) {
+      const int a = A0 + ty + 8 * q, b = B0 + tx;
+      if (a < v && b < v) dst[(long long)a * v + b] += tile[tx][ty + 8 * q];
+    }
+    __syncthreads();
   }
 }
 
@@ -476,19 +573,26 @@ static inline dim3 grid_for(long long n) {
   return dim3((unsigned)b);
 }
 
+// grid of the tiled re-layouts: every tile of every run of TR_SLABS slabs
+static inline dim3 tile_grid(long long ntiles, long long nslab) {
+  const long long runs = (nslab + TR_SLABS - 1) / TR_SLABS;
+  return dim3((unsigned)std::max(1LL, ntiles * runs));
+}
+
 extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void* t2H, void* ovooT,
                                      double* eps_occ, double* eps_vir, double* fvoP, const double* ovvv,
                                      int packed, const double* t2, const double* ovoo, const double* mo_energy,
                                      const double* fock, int o, int v, int Op, int V, hipStream_t s) {
-  const long long n_ovvv = (long long)o * V * V * V, n_t2 = (long long)Op * Op * V * V,
-                  n_ovoo = (long long)o * o * V * Op;
+  const long long n_ovoo = (long long)o * o * V * Op;
+  const dim3 g_ovvv = tile_grid((long long)(V / TR_T) * (V / TR_T), (long long)o * V);
+  const dim3 g_t2 = tile_grid((long long)(V / TR_T) * (V / TR_T), (long long)Op * Op);
   if (tp_is_f64) {
-    if (ovvv) hipLaunchKernelGGL(prep_ovvv<double>, grid_for(n_ovvv), dim3(256), 0, s, (double*)ovvvT, ovvv, o, v, V, packed);
-    hipLaunchKernelGGL(prep_t2<double>, grid_for(n_t2), dim3(256), 0, s, (double*)t2T, (double*)t2H, t2, o, v, Op, V);
+    if (ovvv) hipLaunchKernelGGL(prep_ovvv<double>, g_ovvv, dim3(256), 0, s, (double*)ovvvT, ovvv, o, v, V, packed);
+    hipLaunchKernelGGL(prep_t2<double>, g_t2, dim3(256), 0, s, (double*)t2T, (double*)t2H, t2, o, v, Op, V);
     hipLaunchKernelGGL(prep_ovoo<double>, grid_for(n_ovoo), dim3(256), 0, s, (double*)ovooT, ovoo, o, v, V, Op);
   } else {
-    if (ovvv) hipLaunchKernelGGL(prep_ovvv<float>, grid_for(n_ovvv), dim3(256), 0, s, (float*)ovvvT, ovvv, o, v, V, packed);
-    hipLaunchKernelGGL(prep_t2<float>, grid_for(n_t2), dim3(256), 0, s, (float*)t2T, (float*)t2H, t2, o, v, Op, V);
+    if (ovvv) hipLaunchKernelGGL(prep_ovvv<float>, g_ovvv, dim3(256), 0, s, (float*)ovvvT, ovvv, o, v, V, packed);
+    hipLaunchKernelGGL(prep_t2<float>, g_t2, dim3(256), 0, s, (float*)t2T, (float*)t2H, t2, o, v, Op, V);
     hipLaunchKernelGGL(prep_ovoo<float>, grid_for(n_ovoo), dim3(256), 0, s, (float*)ovooT, ovoo, o, v, V, Op);
   }
   const long long ne = (long long)o * V > V ? (long long)o * V : V;
@@ -533,10 +637,11 @@ extern "C" hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_b
                                   const double* t2hacc, const double* ovoo_acc, const double* t1_acc,
                                   const double* fvo_acc, const double* eps_bar, int o, int v, int Op, int V,
                                   hipStream_t s) {
-  const long long npair = (long long)v * (v + 1) / 2;
-  const long long n_ovvv = packed ? (long long)o * v * npair : (long long)o * v * v * v;
-  if (ovvv_bar) hipLaunchKernelGGL(fin_ovvv, grid_for(n_ovvv), dim3(256), 0, s, ovvv_bar, ovvv_acc, o, v, V, packed);
-  hipLaunchKernelGGL(fin_t2, grid_for((long long)o * o * v * v), dim3(256), 0, s, t2_bar, t2hacc, o, v, Op, V);
+  const long long ntv = (v + TR_T - 1) / TR_T;
+  if (ovvv_bar)
+    hipLaunchKernelGGL(fin_ovvv, tile_grid(packed ? ntv * (ntv + 1) / 2 : ntv * ntv, (long long)o * v), dim3(256), 0, s,
+                       ovvv_bar, ovvv_acc, o, v, V, packed);
+  hipLaunchKernelGGL(fin_t2, tile_grid(ntv * ntv, (long long)o * o), dim3(256), 0, s, t2_bar, t2hacc, o, v, Op, V);
   const long long n = o + v;
   const long long nsmall = (long long)o * v * o * o + (long long)o * v + n * n + n;
   hipLaunchKernelGGL(fin_small, grid_for(nsmall), dim3(256), 0, s, ovoo_bar, t1_bar, fock_bar, mo_energy_bar,
