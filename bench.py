@@ -310,9 +310,13 @@ def main():
             return float(cum[sl[1]] - cum[sl[0]])
 
     if args.calibrate_copy:
-        src = torch.empty(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        # two plain copies of the given size for the FETCH_SIZE / WRITE_SIZE
+        # scale: the runtime's blit kernel, then an elementwise kernel with
+        # ordinary vector loads and stores (the kind the GEMM issues)
+        src = torch.ones(args.calibrate_copy * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 

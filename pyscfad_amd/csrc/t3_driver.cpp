@@ -1626,26 +1626,46 @@ bool autotune_enabled() {
 }
 
 // ---------------------------------------------------------------------------
-// Batched problems: the instances run one after another in stream order and
-// share one workspace (its region part sized for the largest instance); every
-// instance's descriptor blob has its own place after that.  The whole
-// sequence -- descriptor upload from pinned host memory, then every
-// instance's kernels -- is captured once into a hipGraph and replayed with a
-// single hipGraphLaunch; graphs are cached by the exact call (shapes,
+// Batched problems.  The instances are dealt to up to MAX_BATCH_STREAMS
+// groups, largest F_E first to the group with the least work so far; a
+// group's instances run one after another in stream order on one region part
+// of the workspace (sized for the largest instance), and the groups run on
+// streams of their own, so small instances' launches overlap on the GPU.
+// Every instance's descriptor blob has its own place after the region parts.
+// Each group's sequence of kernels is captured once into hipGraphs; a call
+// uploads the descriptors from pinned host memory on the caller's stream,
+// forks the group streams from it, replays each group's graphs on its stream
+// and joins them back.  Graphs are cached by the exact call (shapes,
 // pointers, precision, workspace).
+constexpr int MAX_BATCH_STREAMS = 4;
+
+int batch_streams(int n) {
+  int g = MAX_BATCH_STREAMS;
+  if (const char* env = std::getenv("PYSCFAD_AMD_BATCH_STREAMS")) {
+    const int x = std::atoi(env);
+    if (x >= 1 && x <= 16) g = x;
+  }
+  return std::max(1, std::min(g, n));
+}
 
 struct BatchPlan {
   std::vector<Problem> probs;
   std::vector<Ctx> ctx;
+  std::vector<int> group;           // group of each instance
+  std::vector<size_t> blob_off;     // descriptor blob of each instance, from the workspace base
+  int ngroups = 1;
   size_t region_bytes = 0, blob_bytes = 0;
-  size_t need() const { return region_bytes + blob_bytes; }
+  size_t need() const { return region_bytes * ngroups + blob_bytes; }
 };
 
 int plan_batch(int n, const PyscfadAmdInstance* inst, int precision, int with_vjp, BatchPlan& bp) {
   if (n < 1 || !inst) return fail(PYSCFAD_AMD_ERR_ARGS, "need at least one instance");
   bp.probs.resize(n);
   bp.ctx.resize(n);
+  bp.group.assign(n, 0);
+  bp.blob_off.assign(n, 0);
   std::vector<size_t> blob(n);
+  std::vector<double> fe(n);
   for (int x = 0; x < n; x++) {
     const PyscfadAmdInstance& I = inst[x];
     Problem& p = bp.probs[x];
@@ -1666,23 +1686,43 @@ int plan_batch(int n, const PyscfadAmdInstance* inst, int precision, int with_vj
     c.B = B;
     c.r = r;
     bind(c, p);
+    fe[x] = 2.0 * (double)p.nocc * p.nocc * p.nocc * (double)p.nvir * p.nvir * p.nvir * (p.nvir + p.nocc);
   }
-  size_t off = bp.region_bytes;
+  bp.region_bytes = (size_t)round_up((long long)bp.region_bytes, 256);
+  bp.ngroups = batch_streams(n);
+  // largest first to the least loaded group (ties: the lower group)
+  std::vector<int> order(n);
+  for (int x = 0; x < n; x++) order[x] = x;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fe[a] > fe[b]; });
+  std::vector<double> load(bp.ngroups, 0.0);
+  for (int x : order) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    bp.group[x] = g;
+    load[g] += fe[x];
+  }
+  size_t off = bp.region_bytes * bp.ngroups;
   for (int x = 0; x < n; x++) {
-    bp.ctx[x].r.desc = off;
+    // the plan of instance x is built against its group's region part; its
+    // descriptor offset is taken from that base
+    bp.blob_off[x] = off;
+    bp.ctx[x].r.desc = off - bp.region_bytes * bp.group[x];
     off += blob[x];
   }
-  bp.blob_bytes = off - bp.region_bytes;
+  bp.blob_bytes = off - bp.region_bytes * bp.ngroups;
   return PYSCFAD_AMD_OK;
 }
 
-// A captured call: a chain of graphs, each of at most MAX_GRAPH_NODES kernel
-// nodes (a single graph for small batches), launched back to back.
+// A captured call: per group, a chain of graphs, each of at most
+// MAX_GRAPH_NODES kernel nodes (a single graph for small groups), launched
+// back to back on the group's stream.
 struct GraphEntry {
   std::vector<char> key;
-  std::vector<hipGraph_t> graphs;
-  std::vector<hipGraphExec_t> execs;
-  void* host_blobs = nullptr;   // pinned; read by the first graph's upload node
+  std::vector<std::vector<hipGraph_t>> graphs;        // per group
+  std::vector<std::vector<hipGraphExec_t>> execs;     // per group
+  std::vector<hipStream_t> streams;                   // one per group (default priority)
+  std::vector<hipEvent_t> events;                     // [0] fork, [1 + g] join of group g
+  void* host_blobs = nullptr;   // pinned; uploaded on the caller's stream before the fork
+  size_t blob_at = 0, blob_bytes = 0;
 };
 constexpr int MAX_GRAPH_NODES = 2048;
 
@@ -1692,17 +1732,28 @@ long long g_graph_replays = 0, g_graph_captures = 0;   // guarded by g_graph_mu
 constexpr size_t MAX_GRAPHS = 4;
 
 void destroy_entry(GraphEntry& g) {
-  for (auto x : g.execs) (void)hipGraphExecDestroy(x);
-  for (auto x : g.graphs) (void)hipGraphDestroy(x);
+  for (auto& v : g.execs)
+    for (auto x : v) (void)hipGraphExecDestroy(x);
+  for (auto& v : g.graphs)
+    for (auto x : v) (void)hipGraphDestroy(x);
+  for (auto x : g.events) (void)hipEventDestroy(x);
+  for (auto x : g.streams) (void)hipStreamDestroy(x);
   if (g.host_blobs) (void)hipHostFree(g.host_blobs);
   g = GraphEntry{};
 }
 
-hipError_t launch_entry(const GraphEntry& g, hipStream_t st) {
-  for (auto x : g.execs) {
-    const hipError_t e = hipGraphLaunch(x, st);
-    if (e != hipSuccess) return e;
+hipError_t launch_entry(const GraphEntry& g, char* ws, hipStream_t st) {
+  hipError_t e = hipMemcpyAsync(ws + g.blob_at, g.host_blobs, g.blob_bytes, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  if ((e = hipEventRecord(g.events[0], st)) != hipSuccess) return e;
+  for (size_t gi = 0; gi < g.streams.size(); gi++) {
+    if ((e = hipStreamWaitEvent(g.streams[gi], g.events[0], 0)) != hipSuccess) return e;
+    for (auto x : g.execs[gi])
+      if ((e = hipGraphLaunch(x, g.streams[gi])) != hipSuccess) return e;
+    if ((e = hipEventRecord(g.events[1 + gi], g.streams[gi])) != hipSuccess) return e;
   }
+  for (size_t gi = 0; gi < g.streams.size(); gi++)
+    if ((e = hipStreamWaitEvent(st, g.events[1 + gi], 0)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -1745,7 +1796,7 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
     key.insert(key.end(), b, b + nb);
   };
   {
-    const int head[3] = {n, precision, with_vjp};
+    const int head[4] = {n, precision, with_vjp, bp.ngroups};
     put(head, sizeof(head));
     put(&workspace, sizeof(void*));
     put(&workspace_bytes, sizeof(size_t));
@@ -1766,29 +1817,32 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
       GraphEntry hit = g_graphs[x];
       g_graphs.erase(g_graphs.begin() + x);
       g_graphs.push_back(hit);
-      const hipError_t e = launch_entry(hit, st);
+      const hipError_t e = launch_entry(hit, ws, st);
       if (e != hipSuccess) return fail(PYSCFAD_AMD_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e));
       g_graph_replays++;
       return PYSCFAD_AMD_OK;
     }
   }
 
-  // plans against the real workspace, blobs into one pinned buffer
+  // plans against the real workspace (each instance on its group's region
+  // part), blobs into one pinned buffer
   GraphEntry g;
   g.key = key;
+  g.blob_at = bp.region_bytes * bp.ngroups;
+  g.blob_bytes = bp.blob_bytes;
   if (hipHostMalloc(&g.host_blobs, std::max<size_t>(bp.blob_bytes, 256), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     return fail(PYSCFAD_AMD_ERR_OOM, "pinned host buffer for the descriptors");
   }
   for (int x = 0; x < n; x++) {
     Ctx& c = bp.ctx[x];
-    c.ws = ws;
+    c.ws = ws + bp.region_bytes * bp.group[x];
     c.build_plan();
     if (!c.plan_error.empty()) {
       destroy_entry(g);
       return fail(PYSCFAD_AMD_ERR_ARGS, "internal plan error: " + c.plan_error);
     }
-    std::memcpy(static_cast<char*>(g.host_blobs) + (c.r.desc - bp.region_bytes), c.blob.data.data(),
+    std::memcpy(static_cast<char*>(g.host_blobs) + (bp.blob_off[x] - g.blob_at), c.blob.data.data(),
                 c.blob.data.size());
   }
   auto hip_fail = [&](hipError_t e, const char* where) {
@@ -1797,8 +1851,8 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
   };
   hipError_t e;
   if (g_timing) {
-    // eager, with per-launch events, for measurement
-    e = hipMemcpyAsync(ws + bp.region_bytes, g.host_blobs, bp.blob_bytes, hipMemcpyHostToDevice, st);
+    // eager and in order on the caller's stream, with per-launch events, for measurement
+    e = hipMemcpyAsync(ws + g.blob_at, g.host_blobs, bp.blob_bytes, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
     for (int x = 0; x < n; x++) {
       Events4 ev;
@@ -1810,65 +1864,82 @@ int run_batched(int n, const PyscfadAmdInstance* inst, int precision, int with_v
     destroy_entry(g);
     return PYSCFAD_AMD_OK;
   }
-  // The call as a list of pieces (instance setup, runs of plan launches,
-  // instance finish), cut into graphs of at most MAX_GRAPH_NODES kernels:
-  // the graph runtime walks a graph's dependency chain, and one chain per
-  // call (hundreds of thousands of nodes for 64 instances) is kept out of it.
+  // Per group: its instances as a list of pieces (instance setup, runs of
+  // plan launches, instance finish), cut into graphs of at most
+  // MAX_GRAPH_NODES kernels: the graph runtime walks a graph's dependency
+  // chain, and one chain per call (hundreds of thousands of nodes for 64
+  // instances) is kept out of it.
   struct Piece { int inst, kind; size_t a, b; int nodes; };   // kind 0 setup, 1 launches [a,b), 2 finish
-  std::vector<std::vector<Piece>> groups(1);
-  int in_group = 1;   // the descriptor upload
-  auto add = [&](const Piece& pc) {
-    if (in_group + pc.nodes > MAX_GRAPH_NODES && !groups.back().empty()) {
-      groups.emplace_back();
-      in_group = 0;
-    }
-    groups.back().push_back(pc);
-    in_group += pc.nodes;
-  };
-  for (int x = 0; x < n; x++) {
-    add(Piece{x, 0, 0, 0, SETUP_KERNELS});
-    const auto& ls = bp.ctx[x].launches;
-    size_t a = 0;
-    while (a < ls.size()) {
-      size_t b = a;
-      int k = 0;
-      const int room = std::max(MAX_GRAPH_NODES - in_group, 0);
-      const int cap = room > 0 ? room : MAX_GRAPH_NODES;
-      while (b < ls.size() && k + Ctx::kernels_of(ls[b]) <= cap) k += Ctx::kernels_of(ls[b++]);
-      if (b == a) k += Ctx::kernels_of(ls[b++]);
-      add(Piece{x, 1, a, b, k});
-      a = b;
-    }
-    add(Piece{x, 2, 0, 0, FINISH_KERNELS});
-  }
+  g.graphs.resize(bp.ngroups);
+  g.execs.resize(bp.ngroups);
   // capture on a private stream (the caller's may be the null stream, which
-  // cannot be captured); the graphs are launched on the caller's stream
+  // cannot be captured); the graphs are launched on the group streams
   hipStream_t cs = nullptr;
   if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "stream");
-  for (size_t gi = 0; gi < groups.size() && e == hipSuccess; gi++) {
-    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) break;
-    if (gi == 0) e = hipMemcpyAsync(ws + bp.region_bytes, g.host_blobs, bp.blob_bytes, hipMemcpyHostToDevice, cs);
-    for (const Piece& pc : groups[gi]) {
-      if (e != hipSuccess) break;
-      Ctx& c = bp.ctx[pc.inst];
-      const Problem& p = bp.probs[pc.inst];
-      if (pc.kind == 0) e = enqueue_setup(c, p, cs);
-      else if (pc.kind == 1) e = c.execute(cs, pc.a, pc.b);
-      else e = enqueue_finish(c, p, cs);
+  for (int gr = 0; gr < bp.ngroups && e == hipSuccess; gr++) {
+    std::vector<std::vector<Piece>> pieces(1);
+    int in_graph = 0;
+    auto add = [&](const Piece& pc) {
+      if (in_graph + pc.nodes > MAX_GRAPH_NODES && !pieces.back().empty()) {
+        pieces.emplace_back();
+        in_graph = 0;
+      }
+      pieces.back().push_back(pc);
+      in_graph += pc.nodes;
+    };
+    for (int x = 0; x < n; x++) {
+      if (bp.group[x] != gr) continue;
+      add(Piece{x, 0, 0, 0, SETUP_KERNELS});
+      const auto& ls = bp.ctx[x].launches;
+      size_t a = 0;
+      while (a < ls.size()) {
+        size_t b = a;
+        int k = 0;
+        const int room = std::max(MAX_GRAPH_NODES - in_graph, 0);
+        const int cap = room > 0 ? room : MAX_GRAPH_NODES;
+        while (b < ls.size() && k + Ctx::kernels_of(ls[b]) <= cap) k += Ctx::kernels_of(ls[b++]);
+        if (b == a) k += Ctx::kernels_of(ls[b++]);
+        add(Piece{x, 1, a, b, k});
+        a = b;
+      }
+      add(Piece{x, 2, 0, 0, FINISH_KERNELS});
     }
-    hipGraph_t graph = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(cs, &graph);
-    if (e == hipSuccess) e = e2;
-    if (graph) g.graphs.push_back(graph);
-    if (e != hipSuccess) break;
-    hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e == hipSuccess) g.execs.push_back(exec);
+    for (size_t gi = 0; gi < pieces.size() && e == hipSuccess; gi++) {
+      if (pieces[gi].empty()) continue;
+      e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) break;
+      for (const Piece& pc : pieces[gi]) {
+        if (e != hipSuccess) break;
+        Ctx& c = bp.ctx[pc.inst];
+        const Problem& p = bp.probs[pc.inst];
+        if (pc.kind == 0) e = enqueue_setup(c, p, cs);
+        else if (pc.kind == 1) e = c.execute(cs, pc.a, pc.b);
+        else e = enqueue_finish(c, p, cs);
+      }
+      hipGraph_t graph = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+      if (e == hipSuccess) e = e2;
+      if (graph) g.graphs[gr].push_back(graph);
+      if (e != hipSuccess) break;
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e == hipSuccess) g.execs[gr].push_back(exec);
+    }
   }
   (void)hipStreamDestroy(cs);
   if (e != hipSuccess) return hip_fail(e, "graph capture");
-  if ((e = launch_entry(g, st)) != hipSuccess) return hip_fail(e, "graph launch");
+  for (int gr = 0; gr < bp.ngroups && e == hipSuccess; gr++) {
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) g.streams.push_back(s);
+  }
+  for (int x = 0; x <= bp.ngroups && e == hipSuccess; x++) {
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) g.events.push_back(ev);
+  }
+  if (e != hipSuccess) return hip_fail(e, "streams and events");
+  if ((e = launch_entry(g, ws, st)) != hipSuccess) return hip_fail(e, "graph launch");
   if (g_graphs.size() >= MAX_GRAPHS) {
     // the evicted graph may still be running: its pinned upload source must outlive it
     (void)hipDeviceSynchronize();

@@ -75,16 +75,22 @@ def _fake_instances(shapes):
     return arr
 
 
-def test_batched_workspace_bytes():
+def test_batched_workspace_bytes(monkeypatch):
     lib = _lib.load()
-    shapes = [(4, 40), (9, 73), (6, 120)]
+    shapes = [(4, 40), (9, 73), (6, 120), (5, 33), (7, 41)]
     arr = _fake_instances(shapes)
     for prec in (0, 1):
         for vjp in (0, 1):
-            need = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(shapes), arr, prec, vjp)
             single = max(lib.pyscfad_amd_ccsd_t_workspace_bytes(o, v, prec, vjp) for o, v in shapes)
-            # instances share the region part; each keeps its own descriptors
-            assert single <= need < sum(lib.pyscfad_amd_ccsd_t_workspace_bytes(o, v, prec, vjp) for o, v in shapes)
+            total = sum(lib.pyscfad_amd_ccsd_t_workspace_bytes(o, v, prec, vjp) for o, v in shapes)
+            # the instances of one stream group share a region part; each keeps its own descriptors
+            monkeypatch.setenv("PYSCFAD_AMD_BATCH_STREAMS", "1")
+            one = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(shapes), arr, prec, vjp)
+            assert single <= one < total
+            # four groups (the default): four region parts of the largest instance's size
+            monkeypatch.delenv("PYSCFAD_AMD_BATCH_STREAMS")
+            need = lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(len(shapes), arr, prec, vjp)
+            assert one < need <= 4 * single + one
     assert lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(0, arr, 1, 1) == 0
     bad = _fake_instances([(4, 40), (0, 10)])
     assert lib.pyscfad_amd_ccsd_t_batched_workspace_bytes(2, bad, 1, 1) == 0
