@@ -338,13 +338,18 @@ __device__ __forceinline__ TileCoord find_tile(const GemmProb* __restrict__ prob
     local = bid - probs[lo].block0;
   }
   const GemmProb& P = probs[lo];
-  // a launch has fewer than 2^31 tiles (checked by the plan): 32-bit arithmetic
+  // a launch has fewer than 2^31 tiles (checked by the plan): 32-bit arithmetic.
+  // The output tiles of one k-split are consecutive (n, then m, then split):
+  // the blocks that run together on an XCD read the same K range of the A
+  // rows and B columns, which their L2 then serves once.  (With the split
+  // fastest, the o=64/v=512 t2 particle adjoint fetched every A panel once
+  // per n-tile and every B panel once per m-tile: 38.7 GB per triple.)
   unsigned u = (unsigned)local;
   TileCoord tc;
   tc.p = lo;
-  tc.split = (int)(u % (unsigned)P.ksplit); u /= (unsigned)P.ksplit;
   tc.tn = (int)(u % (unsigned)P.tiles_n); u /= (unsigned)P.tiles_n;
   tc.tm = (int)(u % (unsigned)P.tiles_m); u /= (unsigned)P.tiles_m;
+  tc.split = (int)(u % (unsigned)P.ksplit); u /= (unsigned)P.ksplit;
   tc.b = (int)u;
   return tc;
 }
