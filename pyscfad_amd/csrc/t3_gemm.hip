@@ -6,9 +6,10 @@
 //                 (hi*hi, lo*hi, hi*lo), FP32 accumulation (Henry, Tang and
 //                 Heinecke, ARITH 2019, "bf16x3").
 //   PREC_BF16X6 : the same with a three-way split x = hi + mid + lo (24
-//                 significant bits) and six products per K-step (hi*hi,
-//                 hi*mid, mid*hi, hi*lo, lo*hi, mid*mid; the dropped terms are
-//                 below 2^-32 relative), FP32 accumulation: FP32-level accuracy
+//                 significant bits; |mid| <= 2^-9 |x|, |lo| <= 2^-18 |x|) and
+//                 six products per K-step (hi*hi, hi*mid, mid*hi, hi*lo,
+//                 lo*hi, mid*mid; the dropped mid*lo, lo*mid are at most
+//                 2^-27 relative, lo*lo 2^-36), FP32 accumulation: FP32-level accuracy
 //                 from bf16 matrix cores (Henry, Tang and Heinecke, "bf16x6").
 //
 // Every contraction of the (T) energy and its VJP is expressed as a list of
