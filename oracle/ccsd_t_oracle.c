@@ -15,6 +15,9 @@
  *   - E(T) = 2 * sum w0 z0 d3                  (lno_ccsd_t.c:6-29 energy form;
  *                                               factor 2 as in ccsd_t_slow.py:124)
  *   - the reverse pass                         (ccsd_t_vjp.c:55-117, 139-269, 271-329)
+ *   - the LNO variant with an occupied projector mat: its own job list and
+ *     energy form, the same W, V and adjoint of W and V
+ *                                              (lno_ccsd_t.c:6-138, lno_ccsd_t_vjp.c:9-566)
  *
  * The four products of every ordering (the forward particle and hole terms,
  * ccsd_t.c:137-142, and the adjoints of ccsd_t_vjp.c:139-167, 205-226) are
@@ -216,6 +219,145 @@ static void job_w_z(const T3In *in, int a, int b, int c,
     r3_apply(z0, v0, o);
 }
 
+#ifdef _OPENMP
+#define LOCK_T omp_lock_t
+#define LOCK(l) omp_set_lock(&(l))
+#define UNLOCK(l) omp_unset_lock(&(l))
+#else
+#define LOCK_T int
+#define LOCK(l) ((void)(l))
+#define UNLOCK(l) ((void)(l))
+#endif
+
+/* The adjoint of one job's W and V through the reference's products
+ * (ccsd_t_vjp.c:139-269), given wb0 = dE/dw0 and vb0 = dE/dv0 of the job.
+ * t1_b, vooo_b, fvo_b, t2_b are the calling thread's private accumulators;
+ * vvop_bar is shared, its (p,q) blocks flushed under lk_pq[p*w+q].
+ * Scratch: wbar (4 ooo: the two orderings' W and V adjoints), pack (4 v o),
+ * pqb_loc (2 o n). */
+static void job_adjoint(const T3In *in, const int *abc, const double *wb0, const double *vb0, double *wbar,
+                        double *pack, double *pqb_loc, double *t1_b, double *vooo_b, double *fvo_b,
+                        double *t2_b, double *vvop_bar, LOCK_T *lk_pq)
+{
+    const int o = in->o, v = in->v, n = in->n, w = in->w, oo = o * o;
+    const size_t ooo = (size_t)oo * o, voo = (size_t)v * oo;
+    const double *t1T = in->t1T, *t2T = in->t2T, *vooo = in->vooo, *fvo = in->fvo;
+    double *vbar = wbar + 2 * ooo;
+    for (int pr = 0; pr < 3; pr++) {
+        const int r = abc[PERM_ABC[PAIRS[pr][0]][2]];
+        const double *t2r = t2T + (size_t)r * voo;
+        double *t2r_b = t2_b + (size_t)r * voo;
+        const double *pqs[2];
+        for (int x = 0; x < 2; x++) {
+            const int s = PAIRS[pr][x];
+            pqs[x] = vvop_block(in, abc[PERM_ABC[s][0]], abc[PERM_ABC[s][1]]);
+            double *wb = wbar + x * ooo, *vb = vbar + x * ooo;
+            for (int i = 0; i < o; i++)
+                for (int j = 0; j < o; j++)
+                    for (int k = 0; k < o; k++) {
+                        size_t d = sigma_index(s, i, j, k, o);
+                        wb[((size_t)i * o + j) * o + k] = wb0[d];
+                        vb[((size_t)i * o + j) * o + k] = vb0[d];
+                    }
+        }
+        memset(pqb_loc, 0, sizeof(double) * 2 * o * n);
+        /* particle, both orderings at once (Wb_x = wbar + x*ooo, column-major oo x o):
+         *   vvop_bar[p,q,i,o+f] += sum_jk t2T[r,f,j,k] Wb[i,j,k]
+         *     (column-major v x 2o: t2T[r]^T (v x oo) . [Wb_0 Wb_1] (oo x 2o))
+         *   t2T_bar[r,f,j,k]   += sum_i Wb[i,j,k] vvop[p,q,i,o+f]
+         *     (column-major oo x v: [Wb_0 Wb_1] (oo x 2o) . pack^T (2o x v)) */
+        double *pres = pack + (size_t)2 * v * o;
+        pack_pair(in, pqs[0], pqs[1], pack);
+        gemm('T', 'N', v, 2 * o, oo, 1.0, t2r, oo, wbar, oo, 0.0, pres, v);
+        gemm('N', 'T', oo, v, 2 * o, 1.0, wbar, oo, pack, v, 1.0, t2r_b, oo);
+        for (int x = 0; x < 2; x++) {
+            const int s = PAIRS[pr][x];
+            const int p = abc[PERM_ABC[s][0]], q = abc[PERM_ABC[s][1]];
+            const double *pq = pqs[x];
+            const double *wb = wbar + x * ooo, *vb = vbar + x * ooo;
+            double *pqb = pqb_loc + (size_t)x * o * n;
+            for (int i = 0; i < o; i++)
+                for (int f = 0; f < v; f++) pqb[(size_t)i * n + o + f] = pres[(size_t)v * (x * o + i) + f];
+            const double *t2rq = t2r + (size_t)q * oo;
+            const double *voop = vooo + (size_t)p * ooo;
+            double *voop_b = vooo_b + (size_t)p * ooo;
+            /* hole: vooo_bar[p,i,j,m] -= sum_k t2T[r,q,m,k] Wb[i,j,k]
+             *   (column-major o x oo: t2T[r,q]^T (o x o) . Wb (o x oo))
+             *       t2T_bar[r,q,m,k]  -= sum_ij Wb[i,j,k] vooo[p,i,j,m]
+             *   (column-major o x o: Wb (o x oo) . vooo[p]^T (oo x o)) */
+            gemm('T', 'N', o, oo, o, -1.0, t2rq, o, wb, o, 1.0, voop_b, o);
+            gemm('N', 'T', o, o, oo, -1.0, wb, o, voop, o, 1.0, t2r_b + (size_t)q * oo, o);
+            /* V part */
+            const double *t1r = t1T + (size_t)r * o;
+            const double *fr = fvo + (size_t)r * o;
+            const double *t2qp = t2T + ((size_t)q * v + p) * oo;
+            double *t2qp_b = t2_b + ((size_t)q * v + p) * oo;
+            double *t1r_b = t1_b + (size_t)r * o;
+            double *fr_b = fvo_b + (size_t)r * o;
+            for (int i = 0; i < o; i++)
+                for (int j = 0; j < o; j++) {
+                    const double *vij = vb + ((size_t)i * o + j) * o;
+                    double s1 = 0.0, s2 = 0.0;
+                    const double x1 = pq[(size_t)i * n + j], x2 = t2qp[i * o + j];
+                    for (int k = 0; k < o; k++) {
+                        s1 += vij[k] * t1r[k];
+                        s2 += vij[k] * fr[k];
+                        t1r_b[k] += 0.5 * vij[k] * x1;
+                        fr_b[k] += 0.5 * vij[k] * x2;
+                    }
+                    pqb[(size_t)i * n + j] += 0.5 * s1;
+                    t2qp_b[i * o + j] += 0.5 * s2;
+                }
+            /* flush the job-local block into the shared vvop_bar[p,q] */
+            double *pq_b = vvop_bar + ((size_t)p * w + q) * o * n;
+            LOCK(lk_pq[(size_t)p * w + q]);
+            for (size_t y = 0; y < (size_t)o * n; y++) pq_b[y] += pqb[y];
+            UNLOCK(lk_pq[(size_t)p * w + q]);
+        }
+    }
+}
+
+static LOCK_T *locks_new(size_t n)
+{
+    LOCK_T *lk = malloc(sizeof(LOCK_T) * (n ? n : 1));
+#ifdef _OPENMP
+    for (size_t x = 0; x < n; x++) omp_init_lock(&lk[x]);
+#endif
+    return lk;
+}
+
+static void locks_free(LOCK_T *lk, size_t n)
+{
+#ifdef _OPENMP
+    for (size_t x = 0; x < n; x++) omp_destroy_lock(&lk[x]);
+#endif
+    (void)n;
+    free(lk);
+}
+
+
+/* Ordered reduction of per-thread accumulators (thread t's block priv[t],
+ * the outputs' segments back to back) into the caller's buffers, thread 0
+ * first, as omp_dsum_reduce_inplace does (omp_reduce.c:6-24); frees them. */
+static void reduce_private(double **priv, int nthreads, double *const *outs, const size_t *lens, int nout)
+{
+    size_t off = 0;
+    for (int x = 0; x < nout; x++) {
+        double *dst = outs[x];
+        const size_t len = lens[x];
+#pragma omp parallel for schedule(static)
+        for (size_t e = 0; e < len; e++) {
+            double sacc = 0.0;
+            for (int t = 0; t < nthreads; t++)
+                if (priv[t]) sacc += priv[t][off + e];
+            dst[e] += sacc;
+        }
+        off += len;
+    }
+    for (int t = 0; t < nthreads; t++) free(priv[t]);
+    free(priv);
+}
+
 /* number of jobs a >= b >= c with a < v */
 static long njobs_total(int v) { return (long)v * (v + 1) * (v + 2) / 6; }
 
@@ -307,14 +449,8 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, int vrow, const double *
     int nthreads = 1;
 #ifdef _OPENMP
     nthreads = omp_get_max_threads();
-    omp_lock_t *lk_pq = malloc(sizeof(omp_lock_t) * (size_t)w * w);
-    for (size_t x = 0; x < (size_t)w * w; x++) omp_init_lock(&lk_pq[x]);
-#define LOCK(l) omp_set_lock(&(l))
-#define UNLOCK(l) omp_unset_lock(&(l))
-#else
-#define LOCK(l)
-#define UNLOCK(l)
 #endif
+    LOCK_T *lk_pq = locks_new((size_t)w * w);
     double **priv = calloc(nthreads, sizeof(double *));
     double etot = 0.0;
 #pragma omp parallel reduction(+ : etot)
@@ -329,7 +465,7 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, int vrow, const double *
         double *vooo_b = t1_b + len_t1, *fvo_b = vooo_b + len_vooo, *t2_b = fvo_b + len_fvo;
         double *w0 = malloc(sizeof(double) * ooo * 10);
         double *z0 = w0 + ooo, *d3 = z0 + ooo, *zb = d3 + ooo, *wb0 = zb + ooo, *vb0 = wb0 + ooo;
-        double *wbar = vb0 + ooo, *vbar = wbar + 2 * ooo;            /* two orderings each */
+        double *wbar = vb0 + ooo;                                    /* W and V adjoints, two orderings */
         double *buf = malloc(sizeof(double) * job_buf_len(&in));
         double *pack = malloc(sizeof(double) * (size_t)4 * v * o);   /* pack, then particle result */
         double *pqb_loc = malloc(sizeof(double) * 2 * o * n);
@@ -359,110 +495,171 @@ void oracle_ccsd_t_energy_vjp_range(int nocc, int nvir, int vrow, const double *
             me_b[o + a] += tsum; me_b[o + b] += tsum; me_b[o + c] += tsum;
             r3_apply(vb0, zb, o);                                   /* v0_bar = r3(z_bar) */
             for (size_t x = 0; x < ooo; x++) wb0[x] = ebar * z0[x] * d3[x] + vb0[x];
-            for (int pr = 0; pr < 3; pr++) {
-                const int r = abc[PERM_ABC[PAIRS[pr][0]][2]];
-                const double *t2r = t2T + (size_t)r * voo;
-                double *t2r_b = t2_b + (size_t)r * voo;
-                const double *pqs[2];
-                for (int x = 0; x < 2; x++) {
-                    const int s = PAIRS[pr][x];
-                    pqs[x] = vvop_block(&in, abc[PERM_ABC[s][0]], abc[PERM_ABC[s][1]]);
-                    double *wb = wbar + x * ooo, *vb = vbar + x * ooo;
-                    for (int i = 0; i < o; i++)
-                        for (int j = 0; j < o; j++)
-                            for (int k = 0; k < o; k++) {
-                                size_t d = sigma_index(s, i, j, k, o);
-                                wb[((size_t)i * o + j) * o + k] = wb0[d];
-                                vb[((size_t)i * o + j) * o + k] = vb0[d];
-                            }
-                }
-                memset(pqb_loc, 0, sizeof(double) * 2 * o * n);
-                /* particle, both orderings at once (Wb_x = wbar + x*ooo, column-major oo x o):
-                 *   vvop_bar[p,q,i,o+f] += sum_jk t2T[r,f,j,k] Wb[i,j,k]
-                 *     (column-major v x 2o: t2T[r]^T (v x oo) . [Wb_0 Wb_1] (oo x 2o))
-                 *   t2T_bar[r,f,j,k]   += sum_i Wb[i,j,k] vvop[p,q,i,o+f]
-                 *     (column-major oo x v: [Wb_0 Wb_1] (oo x 2o) . pack^T (2o x v)) */
-                double *pres = pack + (size_t)2 * v * o;
-                pack_pair(&in, pqs[0], pqs[1], pack);
-                gemm('T', 'N', v, 2 * o, oo, 1.0, t2r, oo, wbar, oo, 0.0, pres, v);
-                gemm('N', 'T', oo, v, 2 * o, 1.0, wbar, oo, pack, v, 1.0, t2r_b, oo);
-                for (int x = 0; x < 2; x++) {
-                    const int s = PAIRS[pr][x];
-                    const int p = abc[PERM_ABC[s][0]], q = abc[PERM_ABC[s][1]];
-                    const double *pq = pqs[x];
-                    const double *wb = wbar + x * ooo, *vb = vbar + x * ooo;
-                    double *pqb = pqb_loc + (size_t)x * o * n;
-                    for (int i = 0; i < o; i++)
-                        for (int f = 0; f < v; f++) pqb[(size_t)i * n + o + f] = pres[(size_t)v * (x * o + i) + f];
-                    const double *t2rq = t2r + (size_t)q * oo;
-                    const double *voop = vooo + (size_t)p * ooo;
-                    double *voop_b = vooo_b + (size_t)p * ooo;
-                    /* hole: vooo_bar[p,i,j,m] -= sum_k t2T[r,q,m,k] Wb[i,j,k]
-                     *   (column-major o x oo: t2T[r,q]^T (o x o) . Wb (o x oo))
-                     *       t2T_bar[r,q,m,k]  -= sum_ij Wb[i,j,k] vooo[p,i,j,m]
-                     *   (column-major o x o: Wb (o x oo) . vooo[p]^T (oo x o)) */
-                    gemm('T', 'N', o, oo, o, -1.0, t2rq, o, wb, o, 1.0, voop_b, o);
-                    gemm('N', 'T', o, o, oo, -1.0, wb, o, voop, o, 1.0, t2r_b + (size_t)q * oo, o);
-                    /* V part */
-                    const double *t1r = t1T + (size_t)r * o;
-                    const double *fr = fvo + (size_t)r * o;
-                    const double *t2qp = t2T + ((size_t)q * v + p) * oo;
-                    double *t2qp_b = t2_b + ((size_t)q * v + p) * oo;
-                    double *t1r_b = t1_b + (size_t)r * o;
-                    double *fr_b = fvo_b + (size_t)r * o;
-                    for (int i = 0; i < o; i++)
-                        for (int j = 0; j < o; j++) {
-                            const double *vij = vb + ((size_t)i * o + j) * o;
-                            double s1 = 0.0, s2 = 0.0;
-                            const double x1 = pq[(size_t)i * n + j], x2 = t2qp[i * o + j];
-                            for (int k = 0; k < o; k++) {
-                                s1 += vij[k] * t1r[k];
-                                s2 += vij[k] * fr[k];
-                                t1r_b[k] += 0.5 * vij[k] * x1;
-                                fr_b[k] += 0.5 * vij[k] * x2;
-                            }
-                            pqb[(size_t)i * n + j] += 0.5 * s1;
-                            t2qp_b[i * o + j] += 0.5 * s2;
-                        }
-                    /* flush the job-local block into the shared vvop_bar[p,q] */
-                    double *pq_b = vvop_bar + ((size_t)p * w + q) * o * n;
-                    LOCK(lk_pq[(size_t)p * w + q]);
-                    for (size_t y = 0; y < (size_t)o * n; y++) pq_b[y] += pqb[y];
-                    UNLOCK(lk_pq[(size_t)p * w + q]);
-                }
-            }
+            job_adjoint(&in, abc, wb0, vb0, wbar, pack, pqb_loc, t1_b, vooo_b, fvo_b, t2_b, vvop_bar,
+                        lk_pq);
         }
         free(pack);
         free(pqb_loc);
         free(buf);
         free(w0);
     }
-    /* ordered reduction of the per-thread accumulators into the caller's buffers */
     double *outs[5] = {mo_energy_bar, t1T_bar, vooo_bar, fvo_bar, t2T_bar};
-    size_t lens[5] = {len_me, len_t1, len_vooo, len_fvo, len_t2};
-    size_t off = 0;
-    for (int x = 0; x < 5; x++) {
-        double *dst = outs[x];
-        const size_t len = lens[x];
-#pragma omp parallel for schedule(static)
-        for (size_t e = 0; e < len; e++) {
-            double sacc = 0.0;
-            for (int t = 0; t < nthreads; t++)
-                if (priv[t]) sacc += priv[t][off + e];
-            dst[e] += sacc;
-        }
-        off += len;
-    }
-    for (int t = 0; t < nthreads; t++) free(priv[t]);
-    free(priv);
+    const size_t lens[5] = {len_me, len_t1, len_vooo, len_fvo, len_t2};
+    reduce_private(priv, nthreads, outs, lens, 5);
     if (e_out) *e_out = 2.0 * etot;
-#ifdef _OPENMP
-    for (size_t x = 0; x < (size_t)w * w; x++) omp_destroy_lock(&lk_pq[x]);
-    free(lk_pq);
-#endif
-#undef LOCK
-#undef UNLOCK
+    locks_free(lk_pq, (size_t)w * w);
 }
+
+/* ---------------- LNO (T): the occupied projector mat ----------------
+ * The impurity (T) of the LNO method (pyscfadlib/.../vjp/cc/lno_ccsd_t.c,
+ * lno_ccsd_t_vjp.c; pyscfad/lno/ccsd_t.py:70-237):
+ *   jobs: every a, then b, then c <= b            (lno_ccsd_t.c:74-94)
+ *   w0, z0 as above                               (lno_ccsd_t.c:32-63)
+ *   wm[q,j,k] = sum_r mat[r,q] w0[r,j,k]           (lno_ccsd_t.c:16-18)
+ *   E = 2/3 sum_jobs g_bc sum_ijk wm z0 / D,  g_bc = 1/2 if b == c else 1
+ *                                       (lno_ccsd_t.c:20-28, 65-69; x 2/3 at lno/ccsd_t.py:111)
+ *   reverse: lno_ccsd_t_vjp.c:55-117 (z0_bar, d3_bar, wm_bar, mat_bar, w0_bar),
+ *   then the canonical adjoint of W and V (job_adjoint).
+ * mat is a general o x o matrix (row-major); the reference forms it as
+ * ulo^T ulo (lno/ccsd_t.py:41), but neither pass assumes its symmetry. */
+
+static long lno_njobs_total(int v) { return (long)v * v * (v + 1) / 2; }
+
+static void lno_job_decode(long m, int v, int *pa, int *pb, int *pc)
+{
+    const long per = (long)v * (v + 1) / 2;
+    *pa = (int)(m / per);
+    m %= per;
+    int b = 0;
+    while ((long)(b + 1) * (b + 2) / 2 <= m) b++;
+    *pb = b;
+    *pc = (int)(m - (long)b * (b + 1) / 2);
+}
+
+/* E over the LNO job range [job0, job1) (all: job1 = v v (v+1) / 2), x 2/3 applied */
+double oracle_lno_ccsd_t_energy_range(int nocc, int nvir, const double *mat, const double *mo_energy,
+                                      const double *t1T, const double *t2T, const double *vooo,
+                                      const double *fvo, const double *vvop, long job0, long job1)
+{
+    T3In in = {nocc, nvir, nocc + nvir, nvir, mo_energy, t1T, t2T, vooo, fvo, vvop};
+    const int o = nocc, oo = nocc * nocc;
+    const size_t ooo = (size_t)oo * o;
+    if (job1 > lno_njobs_total(nvir)) job1 = lno_njobs_total(nvir);
+    double et = 0.0;
+#pragma omp parallel reduction(+ : et)
+    {
+        double *w0 = malloc(sizeof(double) * (3 * ooo + job_buf_len(&in)));
+        double *z0 = w0 + ooo, *wm = z0 + ooo, *buf = wm + ooo;
+#pragma omp for schedule(dynamic, 4)
+        for (long m = job0; m < job1; m++) {
+            int a, b, c;
+            lno_job_decode(m, nvir, &a, &b, &c);
+            job_w_z(&in, a, b, c, w0, z0, buf);
+            /* column-major: Wm (oo x o) = W (oo x o) . mat^T-as-stored, i.e. wm[q][p] = sum_r w0[r][p] mat[r][q] */
+            gemm('N', 'T', oo, o, o, 1.0, w0, oo, mat, o, 0.0, wm, oo);
+            const double eabc = mo_energy[o + a] + mo_energy[o + b] + mo_energy[o + c];
+            const double g = (b == c) ? 0.5 : 1.0;
+            double e = 0.0;
+            for (int i = 0; i < o; i++)
+                for (int j = 0; j < o; j++)
+                    for (int k = 0; k < o; k++) {
+                        size_t x = ((size_t)i * o + j) * o + k;
+                        e += g * wm[x] * z0[x] / (mo_energy[i] + mo_energy[j] + mo_energy[k] - eabc);
+                    }
+            et += e;
+        }
+        free(w0);
+    }
+    return 2.0 / 3.0 * et;
+}
+
+/* Reverse pass of the LNO energy over the job range; et_bar is scaled by 2/3
+ * first (lno/ccsd_t.py:129).  Outputs are accumulated; e_out (may be NULL)
+ * gets the E of the same jobs. */
+void oracle_lno_ccsd_t_energy_vjp_range(int nocc, int nvir, const double *mat, const double *mo_energy,
+                                        const double *t1T, const double *t2T, const double *vooo,
+                                        const double *fvo, const double *vvop, double et_bar,
+                                        double *mat_bar, double *mo_energy_bar, double *t1T_bar,
+                                        double *t2T_bar, double *vooo_bar, double *fvo_bar, double *vvop_bar,
+                                        long job0, long job1, double *e_out)
+{
+    T3In in = {nocc, nvir, nocc + nvir, nvir, mo_energy, t1T, t2T, vooo, fvo, vvop};
+    const int o = nocc, v = nvir, n = nocc + nvir, oo = nocc * nocc;
+    const size_t ooo = (size_t)oo * o, voo = (size_t)v * oo;
+    const size_t len_mat = (size_t)oo, len_me = n, len_t1 = (size_t)v * o;
+    const size_t len_vooo = (size_t)v * ooo, len_fvo = (size_t)v * o, len_t2 = (size_t)v * voo;
+    const size_t len_priv = len_mat + len_me + len_t1 + len_vooo + len_fvo + len_t2;
+    const double ebar = 2.0 / 3.0 * et_bar;
+    if (job1 > lno_njobs_total(v)) job1 = lno_njobs_total(v);
+    int nthreads = 1;
+#ifdef _OPENMP
+    nthreads = omp_get_max_threads();
+#endif
+    LOCK_T *lk_pq = locks_new((size_t)v * v);
+    double **priv = calloc(nthreads, sizeof(double *));
+    double etot = 0.0;
+#pragma omp parallel reduction(+ : etot)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *acc = calloc(len_priv, sizeof(double));
+        priv[tid] = acc;
+        double *mat_b = acc, *me_b = mat_b + len_mat, *t1_b = me_b + len_me;
+        double *vooo_b = t1_b + len_t1, *fvo_b = vooo_b + len_vooo, *t2_b = fvo_b + len_fvo;
+        double *w0 = malloc(sizeof(double) * ooo * 11);
+        double *z0 = w0 + ooo, *wm = z0 + ooo, *zb = wm + ooo, *wmb = zb + ooo, *wb0 = wmb + ooo;
+        double *vb0 = wb0 + ooo, *wbar = vb0 + ooo;
+        double *buf = malloc(sizeof(double) * job_buf_len(&in));
+        double *pack = malloc(sizeof(double) * (size_t)4 * v * o);
+        double *pqb_loc = malloc(sizeof(double) * 2 * o * n);
+#pragma omp for schedule(dynamic, 4)
+        for (long m = job0; m < job1; m++) {
+            int a, b, c;
+            lno_job_decode(m, v, &a, &b, &c);
+            const int abc[3] = {a, b, c};
+            job_w_z(&in, a, b, c, w0, z0, buf);
+            gemm('N', 'T', oo, o, o, 1.0, w0, oo, mat, o, 0.0, wm, oo);
+            const double eabc = mo_energy[o + a] + mo_energy[o + b] + mo_energy[o + c];
+            const double g = (b == c) ? 0.5 : 1.0;
+            double tsum = 0.0, ejob = 0.0;
+            for (int i = 0; i < o; i++)
+                for (int j = 0; j < o; j++)
+                    for (int k = 0; k < o; k++) {
+                        size_t x = ((size_t)i * o + j) * o + k;
+                        const double d = g / (mo_energy[i] + mo_energy[j] + mo_energy[k] - eabc);
+                        ejob += wm[x] * z0[x] * d;
+                        zb[x] = ebar * wm[x] * d;
+                        wmb[x] = ebar * z0[x] * d;
+                        const double t = ebar * wm[x] * z0[x] * d * d / g;
+                        me_b[i] -= t; me_b[j] -= t; me_b[k] -= t;
+                        tsum += t;
+                    }
+            etot += ejob;
+            me_b[o + a] += tsum; me_b[o + b] += tsum; me_b[o + c] += tsum;
+            /* mat_bar[r][q] += sum_jk wmb[q,jk] w0[r,jk]; w0_bar[r] = sum_q mat[r,q] wmb[q]
+             * (lno_ccsd_t_vjp.c:98-110, as column-major products) */
+            gemm('T', 'N', o, o, oo, 1.0, wmb, oo, w0, oo, 1.0, mat_b, o);
+            gemm('N', 'N', oo, o, o, 1.0, wmb, oo, mat, o, 0.0, wb0, oo);
+            r3_apply(vb0, zb, o);                                   /* v0_bar = r3(z0_bar) */
+            for (size_t x = 0; x < ooo; x++) wb0[x] += vb0[x];
+            job_adjoint(&in, abc, wb0, vb0, wbar, pack, pqb_loc, t1_b, vooo_b, fvo_b, t2_b, vvop_bar,
+                        lk_pq);
+        }
+        free(pack);
+        free(pqb_loc);
+        free(buf);
+        free(w0);
+    }
+    double *outs[6] = {mat_bar, mo_energy_bar, t1T_bar, vooo_bar, fvo_bar, t2T_bar};
+    const size_t lens[6] = {len_mat, len_me, len_t1, len_vooo, len_fvo, len_t2};
+    reduce_private(priv, nthreads, outs, lens, 6);
+    if (e_out) *e_out = 2.0 / 3.0 * etot;
+    locks_free(lk_pq, (size_t)v * v);
+}
+
+long oracle_lno_njobs(int nvir) { return lno_njobs_total(nvir); }
 
 long oracle_njobs(int nvir) { return njobs_total(nvir); }
 

@@ -35,6 +35,13 @@ def _load():
             [i, i, i] + [dp] * 6 + [ctypes.c_double] + [dp] * 6 + [lg, lg, dp])
         lib.oracle_njobs.restype = lg
         lib.oracle_njobs.argtypes = [i]
+        lib.oracle_lno_ccsd_t_energy_range.restype = ctypes.c_double
+        lib.oracle_lno_ccsd_t_energy_range.argtypes = [i, i] + [dp] * 7 + [lg, lg]
+        lib.oracle_lno_ccsd_t_energy_vjp_range.restype = None
+        lib.oracle_lno_ccsd_t_energy_vjp_range.argtypes = (
+            [i, i] + [dp] * 7 + [ctypes.c_double] + [dp] * 7 + [lg, lg, dp])
+        lib.oracle_lno_njobs.restype = lg
+        lib.oracle_lno_njobs.argtypes = [i]
         lib.oracle_num_threads.restype = i
         lib.oracle_set_dgemm.restype = None
         lib.oracle_set_dgemm.argtypes = [ctypes.c_void_p]
@@ -227,4 +234,63 @@ def energy_vjp(t1, t2, mo_energy, fock, ovoo, ovov, ovvv, et_bar=1.0, job_range=
         "ovov": np.ascontiguousarray(vvop_b[:, :, :, :nocc].transpose(2, 0, 3, 1)),
         "ovvv": pack_ovvv_bar(ovvv_full_b) if packed else np.ascontiguousarray(ovvv_full_b),
     }
+    return (out, e.value) if return_energy else out
+
+
+# ---------------------------------------------------------------------------
+# LNO (T) with an occupied projector (pyscfad/lno/ccsd_t.py:70-237)
+
+def lno_njobs(nvir):
+    """Jobs (a, b >= c) of the LNO job list (lno_ccsd_t.c:73-94)."""
+    return _load().oracle_lno_njobs(nvir)
+
+
+def lno_host_arrays(t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv):
+    """The host arrays of pyscfad/lno/ccsd_t.py:78-93 (inputs already in the
+    t1T (v,o), t2T (v,v,o,o), fvo (v,o) forms that function takes)."""
+    nvir, nocc = np.shape(t1T)
+    nmo = nocc + nvir
+    t1T = np.ascontiguousarray(t1T, dtype=np.float64)
+    t2T = np.ascontiguousarray(t2T, dtype=np.float64)
+    mo_energy = np.ascontiguousarray(mo_energy, dtype=np.float64)
+    fvo = np.ascontiguousarray(fvo, dtype=np.float64)
+    vooo = np.ascontiguousarray(np.asarray(ovoo, dtype=np.float64).transpose(1, 0, 3, 2))
+    vvop = np.empty((nvir, nvir, nocc, nmo))
+    vvop[:, :, :, :nocc] = np.asarray(ovov, dtype=np.float64).transpose(1, 3, 0, 2)
+    vvop[:, :, :, nocc:] = unpack_ovvv(ovvv, nvir).transpose(1, 3, 0, 2)
+    return t1T, t2T, mo_energy, fvo, vooo, vvop
+
+
+def lno_energy(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, job_range=None):
+    """E of the LNO (T) (the value of _ccsd_t_energy, lno/ccsd_t.py:70-113,
+    including its factor 2/3).  ``mat`` is any (o, o) matrix."""
+    nvir, nocc = np.shape(t1T)
+    t1T, t2T, me, fvo, vooo, vvop = lno_host_arrays(t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv)
+    mat = np.ascontiguousarray(mat, dtype=np.float64)
+    j0, j1 = job_range if job_range is not None else (0, lno_njobs(nvir))
+    return _load().oracle_lno_ccsd_t_energy_range(nocc, nvir, _ptr(mat), _ptr(me), _ptr(t1T), _ptr(t2T),
+                                                  _ptr(vooo), _ptr(fvo), _ptr(vvop), j0, j1)
+
+
+def lno_energy_vjp(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, et_bar=1.0, job_range=None,
+                   return_energy=False):
+    """Cotangents of the LNO E, in the order and layouts of
+    _ccsd_t_energy_bwd (lno/ccsd_t.py:121-235): (mat_bar, t1T_bar, t2T_bar,
+    mo_energy_bar, fvo_bar, ovoo_bar, ovov_bar, ovvv_bar), ovvv_bar packed if
+    the input was packed.  With ``return_energy`` also E: (tuple, E)."""
+    nvir, nocc = np.shape(t1T)
+    t1T, t2T, me, fvo, vooo, vvop = lno_host_arrays(t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv)
+    mat = np.ascontiguousarray(mat, dtype=np.float64)
+    bars = [np.zeros_like(x) for x in (mat, me, t1T, t2T, vooo, fvo, vvop)]
+    j0, j1 = job_range if job_range is not None else (0, lno_njobs(nvir))
+    e = ctypes.c_double(0.0)
+    _load().oracle_lno_ccsd_t_energy_vjp_range(
+        nocc, nvir, _ptr(mat), _ptr(me), _ptr(t1T), _ptr(t2T), _ptr(vooo), _ptr(fvo), _ptr(vvop),
+        float(et_bar), *[_ptr(b) for b in bars], j0, j1, ctypes.byref(e))
+    mat_b, me_b, t1T_b, t2T_b, vooo_b, fvo_b, vvop_b = bars
+    ovvv_full_b = vvop_b[:, :, :, nocc:].transpose(2, 0, 3, 1)
+    out = (mat_b, t1T_b, t2T_b, me_b, fvo_b,
+           np.ascontiguousarray(vooo_b.transpose(1, 0, 3, 2)),
+           np.ascontiguousarray(vvop_b[:, :, :, :nocc].transpose(2, 0, 3, 1)),
+           pack_ovvv_bar(ovvv_full_b) if np.ndim(ovvv) == 3 else np.ascontiguousarray(ovvv_full_b))
     return (out, e.value) if return_energy else out

@@ -123,3 +123,25 @@ def energy_dense6_fastmap(t1, t2, mo_energy, fock, ovoo, ovov, ovvv):
     d3 = (eia[:, None, None, :, None, None] + eia[None, :, None, None, :, None]
           + eia[None, None, :, None, None, :])
     return float(np.einsum('ijkabc,ijkabc', _p6(w + v), _r3_6d(_p6(w)) / d3) / 3.0)
+
+
+def lno_energy_dense6(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv):
+    """The LNO (T) energy (lno_ccsd_t.c:6-29, 96-138, x 2/3 at
+    lno/ccsd_t.py:111) as a dense six-index sum over all (a,b,c) and (i,j,k):
+        E = (1/3) sum_{abc,ijk} (sum_l mat[l,i] w0[l,j,k]) z0[i,j,k] / D
+    with w0 = P6(W), z0 = r3(P6(W + V)) and W, V the terms of get_wv
+    (ccsd_t.c:137-150) on the t2T the LNO caller passes (lno/ccsd_t.py:46)."""
+    nvir, nocc = np.shape(t1T)
+    vvov = unpack_ovvv(ovvv, nvir).transpose(1, 3, 0, 2)
+    vooo = np.asarray(ovoo).transpose(1, 0, 3, 2)
+    vvoo = np.asarray(ovov).transpose(1, 3, 0, 2)
+    w = np.einsum('abif,cfjk->ijkabc', vvov, t2T)
+    w -= np.einsum('aijm,cbmk->ijkabc', vooo, t2T)
+    v = 0.5 * (np.einsum('abij,ck->ijkabc', vvoo, t1T) + np.einsum('baij,ck->ijkabc', t2T, fvo))
+    eia = mo_energy[:nocc, None] - mo_energy[None, nocc:]
+    d3 = (eia[:, None, None, :, None, None] + eia[None, :, None, None, :, None]
+          + eia[None, None, :, None, None, :])
+    w0 = _p6(w)
+    z0 = _r3_6d(_p6(w + v))
+    wm = np.einsum('li,ljkabc->ijkabc', mat, w0)
+    return float(np.einsum('ijkabc,ijkabc', wm, z0 / d3) / 3.0)
