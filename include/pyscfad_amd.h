@@ -208,6 +208,42 @@ int pyscfad_amd_ccsd_t_df_energy_vjp(int nocc, int nvir, int naux, const double 
                                      double *Lov_bar, double *Lvv_bar, double *e_out, void *workspace,
                                      size_t workspace_bytes, void *stream);
 
+/*
+ * LNO (T): the impurity (T) correction of the LNO method, with an occupied
+ * projector.  Replaces the custom_vjp _ccsd_t_energy of pyscfad/lno/ccsd_t.py:
+ * the energy replaces its forward (lno/ccsd_t.py:70-113, the C kernel
+ * lnoccsdt_contract, pyscfadlib/pyscfadlib/vjp/cc/lno_ccsd_t.c:96-138, and
+ * the factor 2/3 at lno/ccsd_t.py:111); the VJP replaces its backward
+ * (lno/ccsd_t.py:121-235 with lnoccsdt_energy_vjp, lno_ccsd_t_vjp.c:400-566).
+ * Inputs are the arguments of _ccsd_t_energy, in its order and layouts:
+ *     mat (o,o)   t1T (v,o)   t2T (v,v,o,o)   mo_energy (n)   fvo (v,o)
+ *     ovoo (o,v,o,o)   ovov (o,v,o,v)   ovvv packed (o,v,v(v+1)/2) if ovvv_packed
+ *     (as lno/ccsd_t.py:62-68 reads it), else (o,v,v,v)
+ * E = 2/3 sum_{a, b>=c} g_bc sum_ijk (sum_l mat[l,i] w0[l,j,k]) z0[i,j,k] / D
+ * (g_bc = 1/2 for b == c, lno_ccsd_t.c:16-28, 65-69); with mat = I it is the
+ * E(T) of pyscfad_amd_ccsd_t_energy.  mat is any o x o matrix (the reference
+ * forms ulo^T ulo, lno/ccsd_t.py:42); mat_bar is not symmetrised.
+ * Outputs (DEVICE, FP64, overwritten) are the tuple the backward returns
+ * (lno/ccsd_t.py:235): mat_bar (o,o), t1T_bar (v,o), t2T_bar (v,v,o,o),
+ * mo_energy_bar (n), fvo_bar (v,o), ovoo_bar, ovov_bar, ovvv_bar (packed like
+ * the input).  e_out: the energy (may be NULL for the VJP).  nocc <= 128.
+ * The workspace (NULL: the library allocates) must hold
+ * pyscfad_amd_lno_ccsd_t_workspace_bytes(nocc, nvir, precision, with_vjp)
+ * bytes; that function returns 0 for invalid arguments.
+ */
+size_t pyscfad_amd_lno_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp);
+int pyscfad_amd_lno_ccsd_t_energy(int nocc, int nvir, const double *mat, const double *t1T, const double *t2T,
+                                  const double *mo_energy, const double *fvo, const double *ovoo,
+                                  const double *ovov, const double *ovvv, int ovvv_packed, int precision,
+                                  double *e_out, void *workspace, size_t workspace_bytes, void *stream);
+int pyscfad_amd_lno_ccsd_t_energy_vjp(int nocc, int nvir, const double *mat, const double *t1T, const double *t2T,
+                                      const double *mo_energy, const double *fvo, const double *ovoo,
+                                      const double *ovov, const double *ovvv, int ovvv_packed, double et_bar,
+                                      int precision, double *mat_bar, double *t1T_bar, double *t2T_bar,
+                                      double *mo_energy_bar, double *fvo_bar, double *ovoo_bar, double *ovov_bar,
+                                      double *ovvv_bar, double *e_out, void *workspace, size_t workspace_bytes,
+                                      void *stream);
+
 /* Text of the last error of the calling thread ("" if none). */
 const char *pyscfad_amd_last_error(void);
 

@@ -29,6 +29,9 @@ EXPORTS = (
     "pyscfad_amd_ccsd_t_df_workspace_bytes",
     "pyscfad_amd_ccsd_t_df_energy",
     "pyscfad_amd_ccsd_t_df_energy_vjp",
+    "pyscfad_amd_lno_ccsd_t_workspace_bytes",
+    "pyscfad_amd_lno_ccsd_t_energy",
+    "pyscfad_amd_lno_ccsd_t_energy_vjp",
     "pyscfad_amd_last_error",
     "pyscfad_amd_set_timing",
     "pyscfad_amd_last_timing",
@@ -92,6 +95,12 @@ def load():
     lib.pyscfad_amd_ccsd_t_df_energy_vjp.restype = i
     lib.pyscfad_amd_ccsd_t_df_energy_vjp.argtypes = ([i, i, i] + [dp] * 8 + [d, i, ll, ll] + [dp] * 9
                                                      + [vp, sz, vp])
+    lib.pyscfad_amd_lno_ccsd_t_workspace_bytes.restype = sz
+    lib.pyscfad_amd_lno_ccsd_t_workspace_bytes.argtypes = [i, i, i, i]
+    lib.pyscfad_amd_lno_ccsd_t_energy.restype = i
+    lib.pyscfad_amd_lno_ccsd_t_energy.argtypes = [i, i] + [dp] * 8 + [i, i, dp, vp, sz, vp]
+    lib.pyscfad_amd_lno_ccsd_t_energy_vjp.restype = i
+    lib.pyscfad_amd_lno_ccsd_t_energy_vjp.argtypes = [i, i] + [dp] * 8 + [i, d, i] + [dp] * 9 + [vp, sz, vp]
     lib.pyscfad_amd_last_error.restype = ctypes.c_char_p
     lib.pyscfad_amd_last_error.argtypes = []
     lib.pyscfad_amd_set_timing.restype = None

@@ -16,7 +16,7 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpyscfad_amd.so")
 ARCH = os.environ.get("PYSCFAD_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["t3_gemm.hip", "t3_epilogue.hip", "t3_prep.hip", "t3_noalloc.hip"]
+HIP_SOURCES = ["t3_gemm.hip", "t3_epilogue.hip", "t3_prep.hip", "t3_noalloc.hip", "t3_lno.hip"]
 CPP_SOURCES = ["t3_driver.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -82,15 +82,18 @@ struct EpiTriple {
   double fo;          // 1, 1/2, 1/6
   const void* w0;     // V^3 cube
   void* G;            // V^3 (backward)
-  void* H;            // V^3 (backward)
+  void* H;            // V^3 (backward; unused in the LNO form, where H = G)
   double* part;       // [ncube][EPI_PSTRIDE]
   const double* cm;   // [d][CM/CN][V][V] pair matrices of the V term (vcm_build)
+  const void* s;      // LNO form: V^3 cube of the slot-projected W
+  void* GS;           // LNO form, backward: V^3 adjoint of s
 };
 
 struct EpiArgs {
   int nocc, nvir, V;
   int ntile, ncube;
   int backward;
+  int lno;                 // 1: the LNO energy form (the triples carry s, GS)
   double et_bar;
   const double* eps_occ;   // (o)
   const double* eps_vir;   // (V), padded entries large and finite

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -65,6 +66,21 @@ hipError_t t3_scale_f64(double* x, long long n, const double* scale, hipStream_t
 hipError_t t3_noalloc_scratch(void** ptr, size_t* bytes);
 size_t t3_noalloc_scratch_bytes(void);
 hipError_t t3_zero_f64(double* p, long long n, hipStream_t s);
+hipError_t t3_prep_projected(int tp_is_f64, void* ovvvT_P, void* t2T_Q, void* t2T_P0, void* t2H_P0, void* ovooT_Q,
+                             const double* ovvvP, int packed, const double* t2Q, const double* t2P0,
+                             const double* ovooQ, int o, int v, int Op, int V, hipStream_t s);
+hipError_t t3_finalize_projected(double* ovvvP_bar, int packed, const double* ovvvP_acc, double* t2P0_bar,
+                                 const double* t2P0hacc, double* ovooQ_bar, const double* ovooQ_acc, int o, int v,
+                                 int Op, int V, hipStream_t s);
+int t3_lno_max_nocc(void);
+int t3_lno_matbar_blocks(void);
+hipError_t t3_lno_transpose(double* out, const double* in, long long rows, long long cols, int o, int swap_r,
+                            int swap_c, hipStream_t s);
+hipError_t t3_lno_fock(double* out, const double* in, int o, int v, int to_fock, hipStream_t s);
+hipError_t t3_lno_project(double* out, const double* in, const double* mat, long long A, int o, long long B,
+                          int transK, int accumulate, hipStream_t s);
+hipError_t t3_lno_matbar(double* C, double* part, const double* X, const double* Y, long long A, int o, long long B,
+                         int accumulate, hipStream_t s);
 hipError_t t3_finalize(double* t1_bar, double* t2_bar, double* fock_bar, double* mo_energy_bar, double* ovoo_bar,
                        double* ovvv_bar, int packed, const double* ovvv_acc, const double* t2hacc,
                        const double* ovoo_acc, const double* t1_acc, const double* fvo_acc, const double* eps_bar,
@@ -131,6 +147,9 @@ struct Dims {
   // density-fitted mode (naux > 0): ovvv slabs are built on the device from
   // the DF factors into a cache of `dfslots` block slots of `dfblk` slabs
   int naux = 0, NP = 0, dfblk = 0, dfslots = 0;
+  // LNO form (occupied projector): a second cube S per triple from the
+  // slot-projected operands, and their cotangents
+  int lno = 0;
 };
 
 Dims make_dims(int o, int v, int prec) {
@@ -171,9 +190,10 @@ void set_df(Dims& d, int naux, bool backward, int blk_override = 0) {
   d.dfslots = std::min(3, nblk);
 }
 
-Dims dims_for(int o, int v, int prec, int naux, bool backward, int dfblk = 0) {
+Dims dims_for(int o, int v, int prec, int naux, bool backward, int dfblk = 0, int lno = 0) {
   Dims d = make_dims(o, v, prec);
   if (naux > 0) set_df(d, naux, backward, dfblk);
+  d.lno = lno;
   return d;
 }
 
@@ -204,6 +224,18 @@ struct Regions {
   size_t slot_w0, slot_H, slot_part, slot_vp;
   size_t gslots, gslot_bytes;   // G cubes: B x S of them (S batches per ovvv-adjoint group)
   int S;
+  // LNO form.  Device layouts of the slot-projected operands (P: slot p of
+  // ovvv; Q: slots r and q of t2, p and q of ovoo; P0: slot r of t2):
+  size_t ovvvT_P, t2T_Q, t2T_P0, t2H_P0, ovooT_Q;
+  // reference-layout FP64 work arrays: the canonical t1, t2, fock formed from
+  // t1T, t2T, fvo, and the projected inputs
+  size_t lt1, lt2, lfock, lovvvP, lt2Q, lt2P0, lovooQ;
+  // backward: FP64 accumulators of the projected operands (cleared with the
+  // others), reference-layout cotangents, the mat cotangent's partials
+  size_t ovvvP_acc, ovooQ_acc, t2P0hacc, lt2Q_bar, lt2P0_bar;
+  size_t lt1_bar, lt2_bar, lfock_bar, lovvvP_bar, lovooQ_bar, matpart;
+  size_t slot_s;              // S cube of a batch slot
+  size_t gsslots;             // adjoints of S, laid out like the G cubes
   size_t split, split_bytes;
   size_t desc, desc_bytes;
   size_t total;
@@ -263,6 +295,21 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
     r.LovP = L.take(o * V * d.NP * 8);
     r.LvvF = L.take(V * V * d.NP * 8);
   }
+  const size_t v = d.v, n = d.n;
+  if (d.lno) {
+    r.ovvvT_P = L.take(o * V * V * V * d.elt);
+    r.t2T_Q = L.take(Op * Op * V * V * d.elt);
+    r.t2T_P0 = L.take(Op * Op * V * V * d.elt);
+    r.t2H_P0 = L.take(Op * Op * V * V * d.elt);
+    r.ovooT_Q = L.take(o * o * V * Op * d.elt);
+    r.lt1 = L.take(o * v * 8);
+    r.lt2 = L.take(o * o * v * v * 8);
+    r.lfock = L.take(n * n * 8);
+    r.lovvvP = L.take(o * v * v * v * 8);   // room for the dense form
+    r.lt2Q = L.take(o * o * v * v * 8);
+    r.lt2P0 = L.take(o * o * v * v * 8);
+    r.lovooQ = L.take(o * v * o * o * 8);
+  }
   if (backward) {
     r.ovvv_acc = L.take(nslab * V * V * V * 8);
     r.t2hacc = L.take(Op * Op * V * V * 8);
@@ -274,20 +321,40 @@ Regions plan_regions(const Dims& d, int B, bool backward) {
       r.LovBar = L.take(o * V * d.NP * 8);
       r.LvvBar = L.take(V * V * d.NP * 8);
     }
+    if (d.lno) {
+      r.ovvvP_acc = L.take(o * V * V * V * 8);
+      r.ovooQ_acc = L.take(o * o * V * Op * 8);
+      r.t2P0hacc = L.take(Op * Op * V * V * 8);
+      r.lt2Q_bar = L.take(o * o * v * v * 8);
+      r.lt2P0_bar = L.take(o * o * v * v * 8);
+    }
     r.acc_end = L.off;
+    if (d.lno) {
+      r.lt1_bar = L.take(o * v * 8);
+      r.lt2_bar = L.take(o * o * v * v * 8);
+      r.lfock_bar = L.take(n * n * 8);
+      r.lovvvP_bar = L.take(o * v * v * v * 8);
+      r.lovooQ_bar = L.take(o * v * o * o * 8);
+      r.matpart = L.take((size_t)t3_lno_matbar_blocks() * o * o * 8);
+    }
   }
   Layout S;
   r.slot_w0 = S.take(V * V * V * d.elt);
-  if (backward) r.slot_H = S.take(V * V * V * d.elt);
+  if (d.lno) r.slot_s = S.take(V * V * V * d.elt);
+  if (backward && !d.lno) r.slot_H = S.take(V * V * V * d.elt);   // the LNO form's H is its G
   r.slot_vp = S.take((size_t)VP_TOTAL * V * V * 8);
   r.slot_part = S.take((size_t)d.ncube * EPI_PSTRIDE * 8);
   r.slot_bytes = S.off;
   r.slots = L.take(r.slot_bytes * B);
   r.gslot_bytes = backward ? (size_t)round_up((long long)(V * V * V * d.elt), 256) : 0;
   r.gslots = L.take(r.gslot_bytes * B * r.S);
-  // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6 per triple
+  if (d.lno) r.gsslots = L.take(r.gslot_bytes * B * r.S);
+  // split-K scratch: B2 (V x V outputs) and B3 (V x Op) problems, at most 6
+  // per triple (12 in the LNO form, whose projected operands have outputs of
+  // their own)
   const size_t ks = (size_t)std::max(MAX_KSPLIT, acc_ksplit(d));
-  r.split_bytes = backward ? (size_t)6 * B * ks * V * std::max(V, Op) * d.elt + (size_t)6 * B * 256 : 0;
+  const size_t npt = d.lno ? 12 : 6;
+  r.split_bytes = backward ? npt * B * ks * V * std::max(V, Op) * d.elt + npt * B * 256 : 0;
   r.split = L.take(r.split_bytes);
   r.dfsplit_bytes = (backward && d.naux > 0) ? (size_t)d.dfblk * DF_KSPLIT * V * d.NP * 8 + (size_t)d.dfblk * 256 : 0;
   r.dfsplit = L.take(r.dfsplit_bytes);
@@ -393,6 +460,8 @@ struct Ctx {
   int packed = 1;
   // cotangent outputs accumulated in place by the plan (device, reference layouts)
   double *t2_out = nullptr, *ovov_out = nullptr;
+  // LNO form: the reference-layout cotangents of the projected t2 operands
+  double *t2Q_out = nullptr, *t2P0_out = nullptr;
   char* outp(double* base, size_t elems) const { return reinterpret_cast<char*>(base) + elems * 8; }
   Blob blob;
   std::vector<Launch> launches;
@@ -419,6 +488,11 @@ struct Ctx {
   char* slot(int s, size_t field) const { return ws + r.slots + (size_t)s * r.slot_bytes + field; }
   int gbase = 0;   // first G cube of the current batch within its ovvv-adjoint group
   char* gslot(int s) const { return ws + r.gslots + (size_t)(gbase + s) * r.gslot_bytes; }
+  char* gsslot(int s) const { return ws + r.gsslots + (size_t)(gbase + s) * r.gslot_bytes; }
+  // adjoint cube read by a backward contraction: G, or (LNO form) the adjoint of S
+  char* gcube(int which, int s) const { return which ? gsslot(s) : gslot(s); }
+  // the H cube of a batch slot (the LNO form's H is its G)
+  char* hcube(int s) const { return d.lno ? gslot(s) : slot(s, r.slot_H); }
   long long str(int pos) const {
     return pos == 0 ? (long long)d.V * d.V : (pos == 1 ? (long long)d.V : 1LL);
   }
@@ -758,12 +832,14 @@ struct Ctx {
   }
 
   // ---- W-build segments (output layout: batch position ib, row position im, col position 2) ----
-  GemmSeg particle_seg(const Triple& tr, int rho, int ib, int im) const {
+  // ovvv_reg / t2_reg: the operand regions (the plain ones, or for the LNO
+  // form's S their slot-projected forms)
+  GemmSeg particle_seg(const Triple& tr, int rho, int ib, int im, size_t ovvv_reg, size_t t2_reg) const {
     const int* p = PERM[rho];
     const long long V = d.V, Op = d.Op;
     GemmSeg s{};
-    const char* L = el(r.ovvvT, slab_pos(tr.t[p[0]]) * V * V * V);
-    const char* R = el(r.t2T, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
+    const char* L = el(ovvv_reg, slab_pos(tr.t[p[0]]) * V * V * V);
+    const char* R = el(t2_reg, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
     if (p[2] == 2) {
       s.A = L; s.a_sb = (p[0] == ib) ? V * V : V; s.a_sm = (p[0] == im) ? V * V : V; s.a_sk2 = 1;
       s.B = R; s.b_sb = 0; s.b_sn = V; s.b_sk2 = 1;
@@ -774,13 +850,13 @@ struct Ctx {
     s.K1 = 1; s.K2 = d.V; s.scale = 1.0f;
     return s;
   }
-  GemmSeg hole_seg(const Triple& tr, int rho, int ib, int im) const {
+  GemmSeg hole_seg(const Triple& tr, int rho, int ib, int im, size_t ovoo_reg, size_t t2_reg, size_t t2h_reg) const {
     const int* p = PERM[rho];
     const long long V = d.V, Op = d.Op;
     GemmSeg s{};
-    const char* Lp = el(r.ovooT, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
-    const char* T2 = el(r.t2T, (long long)tr.t[p[2]] * Op * V * V);   // [m][s_p2][s_p1]
-    const char* T2H = el(r.t2H, (long long)tr.t[p[2]] * Op * V * V);  // [m][s_p1][s_p2]
+    const char* Lp = el(ovoo_reg, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
+    const char* T2 = el(t2_reg, (long long)tr.t[p[2]] * Op * V * V);   // [m][s_p2][s_p1]
+    const char* T2H = el(t2h_reg, (long long)tr.t[p[2]] * Op * V * V);  // [m][s_p1][s_p2]
     if (p[0] == 2) {
       s.B = Lp; s.b_sb = 0; s.b_sn = Op; s.b_sk2 = 1;
       s.A = (p[1] == im) ? T2 : T2H;
@@ -794,7 +870,15 @@ struct Ctx {
     return s;
   }
 
-  void emit_wbuild(const std::vector<Triple>& bt) {
+  // Operand sets of a cube build: particle terms (ovvv, t2) and hole terms
+  // (ovoo, t2, t2H), as region offsets; every set contributes the W terms of
+  // ccsd_t.c:137-142 with its operands.
+  struct CubeTerms {
+    std::vector<std::array<size_t, 2>> part;
+    std::vector<std::array<size_t, 3>> hole;
+  };
+
+  void emit_cube_build(const std::vector<Triple>& bt, size_t field, const CubeTerms& ct) {
     const long long V = d.V;
     static const int P1[4] = {0, 2, 3, 5}, H1[4] = {0, 1, 4, 5};   // batch b (pos 1), rows a
     static const int P2[2] = {1, 4}, H2[2] = {2, 3};               // batch a (pos 0), rows b
@@ -803,19 +887,37 @@ struct Ctx {
       for (size_t s = 0; s < bt.size(); s++) {
         GemmProb P{};
         P.nb = d.V; P.M = d.V; P.N = d.V;
-        P.C = slot((int)s, r.slot_w0);
+        P.C = slot((int)s, field);
         P.c_sn = 1; P.alpha = 1.0;
         if (pass == 0) { P.c_sb = V; P.c_sm = V * V; }
         else { P.c_sb = V * V; P.c_sm = V; }
         const int p = gb.problem((long long)s, P);
         const int ib = pass == 0 ? 1 : 0, im = pass == 0 ? 0 : 1;
         const int np = pass == 0 ? 4 : 2;
-        for (int q = 0; q < np; q++) gb.add(p, particle_seg(bt[s], pass == 0 ? P1[q] : P2[q], ib, im));
-        for (int q = 0; q < np; q++) gb.add(p, hole_seg(bt[s], pass == 0 ? H1[q] : H2[q], ib, im));
+        for (const auto& t : ct.part)
+          for (int q = 0; q < np; q++) gb.add(p, particle_seg(bt[s], pass == 0 ? P1[q] : P2[q], ib, im, t[0], t[1]));
+        for (const auto& t : ct.hole)
+          for (int q = 0; q < np; q++)
+            gb.add(p, hole_seg(bt[s], pass == 0 ? H1[q] : H2[q], ib, im, t[0], t[1], t[2]));
       }
       emit_gemm(gb, cfg_for(FAM_GEMM_W, pick_cfg(gb)), pass == 0 ? GEMM_STORE : GEMM_ACCUM, d.tp_is_f64, false,
                 FAM_GEMM_W);
     }
+  }
+
+  void emit_wbuild(const std::vector<Triple>& bt) {
+    emit_cube_build(bt, r.slot_w0, CubeTerms{{{r.ovvvT, r.t2T}}, {{r.ovooT, r.t2T, r.t2H}}});
+  }
+
+  // LNO form: S = (P1 + P2 + P3) w0, each occupied slot of every term in turn
+  // projected by mat.  A particle term sum_f ovvv[p,x,f,y] t2[r,q,z,f] has
+  // slot p on ovvv and q, r on t2; a hole term sum_m ovoo[p,x,m,q] t2[r,m,z,y]
+  // has p, q on ovoo and r on t2.  So S is the W build over the operand sets
+  // (P ovvv, t2) + (ovvv, Q t2) and (Q ovoo, t2) + (ovoo, P0 t2).
+  void emit_sbuild(const std::vector<Triple>& bt) {
+    emit_cube_build(bt, r.slot_s,
+                    CubeTerms{{{r.ovvvT_P, r.t2T}, {r.ovvvT, r.t2T_Q}},
+                              {{r.ovooT_Q, r.t2T, r.t2H}, {r.ovooT, r.t2T_P0, r.t2H_P0}}});
   }
 
   void emit_epilogue(const std::vector<Triple>& bt, bool bw) {
@@ -826,7 +928,9 @@ struct Ctx {
       e.fo = bt[s].fo;
       e.w0 = slot((int)s, r.slot_w0);
       e.G = bw ? gslot((int)s) : nullptr;
-      e.H = bw ? slot((int)s, r.slot_H) : nullptr;
+      e.H = (bw && !d.lno) ? slot((int)s, r.slot_H) : nullptr;
+      e.s = d.lno ? slot((int)s, r.slot_s) : nullptr;
+      e.GS = (bw && d.lno) ? gsslot((int)s) : nullptr;
       e.part = reinterpret_cast<double*>(slot((int)s, r.slot_part));
       e.cm = reinterpret_cast<const double*>(slot((int)s, r.slot_vp)) + (size_t)VP_CM * d.V * d.V;
       et.push_back(e);
@@ -836,7 +940,7 @@ struct Ctx {
     for (size_t s = 0; s < bt.size(); s++) {
       VpTriple t{};
       t.i = bt[s].t[0]; t.j = bt[s].t[1]; t.k = bt[s].t[2];
-      t.H = bw ? slot((int)s, r.slot_H) : nullptr;
+      t.H = bw ? hcube((int)s) : nullptr;
       t.out = reinterpret_cast<double*>(slot((int)s, r.slot_vp));
       vt.push_back(t);
     }
@@ -848,7 +952,7 @@ struct Ctx {
     launches.push_back(M);
     EpiArgs a{};
     a.nocc = d.o; a.nvir = d.v; a.V = d.V; a.ntile = d.ntile; a.ncube = d.ncube;
-    a.backward = bw; a.et_bar = et_bar;
+    a.backward = bw; a.lno = d.lno; a.et_bar = et_bar;
     a.eps_occ = dl(r.eps_occ, 0); a.eps_vir = dl(r.eps_vir, 0);
     a.t1 = t1; a.fvoP = dl(r.fvoP, 0);
     epi_args.push_back(a);
@@ -873,31 +977,78 @@ struct Ctx {
   }
 
   // ---- backward contractions over G ----
+  // Each family runs over terms (adjoint cube, operand regions, output): the
+  // canonical term reads G with the plain operands.  In the LNO form two
+  // terms per family read the adjoint of S: one with the other factor's
+  // projected operand into the plain output (same problems, more K
+  // segments), one with the plain operand into the projected operand's own
+  // output (more problems, keyed apart).
+  struct B1Term { int g; size_t t2; size_t out; long long key; };
+  struct B2Term { int g; size_t ovvv; double* out; long long key; };
+  struct B3Term { int g; size_t t2, t2h, out; long long key; };
+  struct B4Term { int g; size_t ovoo; double* out; size_t hacc; long long key; };
+  static constexpr long long KEY_PROJ = 1LL << 40;
+
+  std::vector<B1Term> b1_terms() const {
+    std::vector<B1Term> t{{0, r.t2T, r.ovvv_acc, 0}};
+    if (d.lno) {
+      t.push_back({1, r.t2T_Q, r.ovvv_acc, 0});
+      t.push_back({1, r.t2T, r.ovvvP_acc, KEY_PROJ});
+    }
+    return t;
+  }
+  std::vector<B2Term> b2_terms() const {
+    std::vector<B2Term> t{{0, r.ovvvT, t2_out, 0}};
+    if (d.lno) {
+      t.push_back({1, r.ovvvT_P, t2_out, 0});
+      t.push_back({1, r.ovvvT, t2Q_out, KEY_PROJ});
+    }
+    return t;
+  }
+  std::vector<B3Term> b3_terms() const {
+    std::vector<B3Term> t{{0, r.t2T, r.t2H, r.ovoo_acc, 0}};
+    if (d.lno) {
+      t.push_back({1, r.t2T_P0, r.t2H_P0, r.ovoo_acc, 0});
+      t.push_back({1, r.t2T, r.t2H, r.ovooQ_acc, KEY_PROJ});
+    }
+    return t;
+  }
+  std::vector<B4Term> b4_terms() const {
+    std::vector<B4Term> t{{0, r.ovooT, t2_out, r.t2hacc, 0}};
+    if (d.lno) {
+      t.push_back({1, r.ovooT_Q, t2_out, r.t2hacc, 0});
+      t.push_back({1, r.ovooT, t2P0_out, r.t2P0hacc, KEY_PROJ});
+    }
+    return t;
+  }
+
   void emit_b1(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
+    const std::vector<B1Term> terms = b1_terms();
     for (int orient = 0; orient < 2; orient++) {
       GemmBuild gb(d.prec);
-      for (size_t s = 0; s < bt.size(); s++)
-        for (int rho = 0; rho < 6; rho++) {
-          const int* p = PERM[rho];
-          if ((p[0] == 2) != (orient == 1)) continue;
-          const Triple& tr = bt[s];
-          GemmProb P{};
-          P.nb = d.V; P.M = d.V; P.N = d.V;
-          P.C = ws ? ws + r.ovvv_acc + (size_t)slab_pos(tr.t[p[0]]) * V * V * V * 8 : nullptr;
-          P.c_sn = 1; P.alpha = 1.0;
-          if (orient == 0) { P.c_sb = V * V; P.c_sm = V; } else { P.c_sb = V; P.c_sm = V * V; }
-          const int q = gb.problem(tr.t[p[0]], P);
-          GemmSeg sg{};
-          sg.A = gslot((int)s);
-          if (orient == 0) { sg.a_sb = str(p[0]); sg.a_sm = str(p[1]); }
-          else { sg.a_sb = str(p[1]); sg.a_sm = str(p[0]); }
-          sg.a_sk2 = str(p[2]);
-          sg.B = el(r.t2T, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
-          sg.b_sb = 0; sg.b_sk2 = V; sg.b_sn = 1;
-          sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
-          gb.add(q, sg);
-        }
+      for (const B1Term& T : terms)
+        for (size_t s = 0; s < bt.size(); s++)
+          for (int rho = 0; rho < 6; rho++) {
+            const int* p = PERM[rho];
+            if ((p[0] == 2) != (orient == 1)) continue;
+            const Triple& tr = bt[s];
+            GemmProb P{};
+            P.nb = d.V; P.M = d.V; P.N = d.V;
+            P.C = ws ? ws + T.out + (size_t)slab_pos(tr.t[p[0]]) * V * V * V * 8 : nullptr;
+            P.c_sn = 1; P.alpha = 1.0;
+            if (orient == 0) { P.c_sb = V * V; P.c_sm = V; } else { P.c_sb = V; P.c_sm = V * V; }
+            const int q = gb.problem(T.key + tr.t[p[0]], P);
+            GemmSeg sg{};
+            sg.A = gcube(T.g, (int)s);
+            if (orient == 0) { sg.a_sb = str(p[0]); sg.a_sm = str(p[1]); }
+            else { sg.a_sb = str(p[1]); sg.a_sm = str(p[0]); }
+            sg.a_sk2 = str(p[2]);
+            sg.B = el(T.t2, ((long long)tr.t[p[2]] * Op + tr.t[p[1]]) * V * V);
+            sg.b_sb = 0; sg.b_sk2 = V; sg.b_sn = 1;
+            sg.K1 = 1; sg.K2 = d.V; sg.scale = 1.0f;
+            gb.add(q, sg);
+          }
       emit_gemm(gb, cfg_for(FAM_GEMM_B1, pick_cfg(gb)), GEMM_ACCUM, 1, false, FAM_GEMM_B1);
     }
   }
@@ -905,29 +1056,30 @@ struct Ctx {
   void emit_b2(const std::vector<Triple>& bt) {
     const long long V = d.V;
     GemmBuild gb(d.prec);
-    for (size_t s = 0; s < bt.size(); s++)
-      for (int rho = 0; rho < 6; rho++) {
-        const int* p = PERM[rho];
-        const Triple& tr = bt[s];
-        // t2_bar[t_p2][t_p1][m][n], accumulated in place (rows and columns past v not written)
-        GemmProb P{};
-        P.nb = 1; P.M = d.V; P.N = d.V; P.Mv = d.v; P.Nv = d.v;
-        P.C = outp(t2_out, ((size_t)tr.t[p[2]] * d.o + tr.t[p[1]]) * d.v * d.v);
-        P.c_sm = d.v; P.c_sn = 1; P.alpha = 1.0;
-        const int q = gb.problem((long long)tr.t[p[2]] * 4096 + tr.t[p[1]], P);
-        GemmSeg sg{};
-        sg.A = gslot((int)s);
-        sg.a_sm = str(p[2]);
-        sg.B = el(r.ovvvT, slab_pos(tr.t[p[0]]) * V * V * V);
-        sg.b_sn = 1;
-        if (p[0] != 2) {
-          sg.a_sk1 = str(p[0]); sg.a_sk2 = str(p[1]); sg.b_sk1 = V * V; sg.b_sk2 = V;
-        } else {
-          sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[0]); sg.b_sk1 = V; sg.b_sk2 = V * V;
+    for (const B2Term& T : b2_terms())
+      for (size_t s = 0; s < bt.size(); s++)
+        for (int rho = 0; rho < 6; rho++) {
+          const int* p = PERM[rho];
+          const Triple& tr = bt[s];
+          // t2_bar[t_p2][t_p1][m][n], accumulated in place (rows and columns past v not written)
+          GemmProb P{};
+          P.nb = 1; P.M = d.V; P.N = d.V; P.Mv = d.v; P.Nv = d.v;
+          P.C = outp(T.out, ((size_t)tr.t[p[2]] * d.o + tr.t[p[1]]) * d.v * d.v);
+          P.c_sm = d.v; P.c_sn = 1; P.alpha = 1.0;
+          const int q = gb.problem(T.key + (long long)tr.t[p[2]] * 4096 + tr.t[p[1]], P);
+          GemmSeg sg{};
+          sg.A = gcube(T.g, (int)s);
+          sg.a_sm = str(p[2]);
+          sg.B = el(T.ovvv, slab_pos(tr.t[p[0]]) * V * V * V);
+          sg.b_sn = 1;
+          if (p[0] != 2) {
+            sg.a_sk1 = str(p[0]); sg.a_sk2 = str(p[1]); sg.b_sk1 = V * V; sg.b_sk2 = V;
+          } else {
+            sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[0]); sg.b_sk1 = V; sg.b_sk2 = V * V;
+          }
+          sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
+          gb.add(q, sg);
         }
-        sg.K1 = d.V; sg.K2 = d.V; sg.scale = 1.0f;
-        gb.add(q, sg);
-      }
     // FP64 partials of 64 splits cost more in the reduction than the extra blocks gain
     emit_gemm(gb, cfg_for(FAM_GEMM_B2, pick_cfg(gb)), GEMM_ACCUM, 1, true, FAM_GEMM_B2,
               d.prec == PREC_F64 ? 16 : MAX_KSPLIT);
@@ -936,32 +1088,33 @@ struct Ctx {
   void emit_b3(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
     GemmBuild gb(d.prec);
-    for (size_t s = 0; s < bt.size(); s++)
-      for (int rho = 0; rho < 6; rho++) {
-        const int* p = PERM[rho];
-        const Triple& tr = bt[s];
-        GemmProb P{};
-        P.nb = 1; P.M = d.V; P.N = d.Op;
-        P.C = ws ? ws + r.ovoo_acc + ((size_t)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op * 8 : nullptr;
-        P.c_sm = Op; P.c_sn = 1; P.alpha = 1.0;
-        const int q = gb.problem((long long)tr.t[p[0]] * 4096 + tr.t[p[1]], P);
-        GemmSeg sg{};
-        sg.A = gslot((int)s);
-        sg.a_sm = str(p[0]);
-        sg.b_sn = V * V;
-        // k runs over the two contracted cube positions in increasing order,
-        // so the two permutations with the same free position read G alike
-        if (p[1] < p[2]) {   // k = (s_p1, s_p2), B from t2H[t_p2][m][s_p1][s_p2]
-          sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[2]);
-          sg.B = el(r.t2H, (long long)tr.t[p[2]] * Op * V * V);
-        } else {             // k = (s_p2, s_p1), B from t2T[t_p2][m][s_p2][s_p1]
-          sg.a_sk1 = str(p[2]); sg.a_sk2 = str(p[1]);
-          sg.B = el(r.t2T, (long long)tr.t[p[2]] * Op * V * V);
+    for (const B3Term& T : b3_terms())
+      for (size_t s = 0; s < bt.size(); s++)
+        for (int rho = 0; rho < 6; rho++) {
+          const int* p = PERM[rho];
+          const Triple& tr = bt[s];
+          GemmProb P{};
+          P.nb = 1; P.M = d.V; P.N = d.Op;
+          P.C = ws ? ws + T.out + ((size_t)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op * 8 : nullptr;
+          P.c_sm = Op; P.c_sn = 1; P.alpha = 1.0;
+          const int q = gb.problem(T.key + (long long)tr.t[p[0]] * 4096 + tr.t[p[1]], P);
+          GemmSeg sg{};
+          sg.A = gcube(T.g, (int)s);
+          sg.a_sm = str(p[0]);
+          sg.b_sn = V * V;
+          // k runs over the two contracted cube positions in increasing order,
+          // so the two permutations with the same free position read G alike
+          if (p[1] < p[2]) {   // k = (s_p1, s_p2), B from t2H[t_p2][m][s_p1][s_p2]
+            sg.a_sk1 = str(p[1]); sg.a_sk2 = str(p[2]);
+            sg.B = el(T.t2h, (long long)tr.t[p[2]] * Op * V * V);
+          } else {             // k = (s_p2, s_p1), B from t2T[t_p2][m][s_p2][s_p1]
+            sg.a_sk1 = str(p[2]); sg.a_sk2 = str(p[1]);
+            sg.B = el(T.t2, (long long)tr.t[p[2]] * Op * V * V);
+          }
+          sg.b_sk1 = V; sg.b_sk2 = 1;
+          sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
+          gb.add(q, sg);
         }
-        sg.b_sk1 = V; sg.b_sk2 = 1;
-        sg.K1 = d.V; sg.K2 = d.V; sg.scale = -1.0f;
-        gb.add(q, sg);
-      }
     // default: 32-wide tiles on the occupied side unless it is a multiple of 64
     emit_gemm(gb, cfg_for(FAM_GEMM_B3, d.Op % 64 ? 3 : 0), GEMM_ACCUM, 1, true, FAM_GEMM_B3);
   }
@@ -969,39 +1122,40 @@ struct Ctx {
   void emit_b4(const std::vector<Triple>& bt) {
     const long long V = d.V, Op = d.Op;
     GemmBuild gb(d.prec);
-    for (size_t s = 0; s < bt.size(); s++)
-      for (int rho = 0; rho < 6; rho++) {
-        const int* p = PERM[rho];
-        const Triple& tr = bt[s];
-        const bool t2layout = (p[1] == 2);
-        // t2 layout (the cube's c is t_p1's partner): t2_bar[t_p2][m][b][n],
-        // accumulated in place (batches and columns: the v that exist, rows:
-        // the o that exist); otherwise the [i][j][b][a] accumulator t2hacc,
-        // added to t2_bar transposed at the end (writing t2_bar directly
-        // would take uncoalesced stores: the cube's contiguous index is not
-        // t2_bar's)
-        GemmProb P{};
-        if (t2layout) {
-          P.nb = d.v; P.M = d.Op; P.N = d.V; P.Mv = d.o; P.Nv = d.v;
-          P.C = outp(t2_out, (size_t)tr.t[p[2]] * d.o * d.v * d.v);
-          P.c_sb = d.v; P.c_sm = (long long)d.v * d.v; P.c_sn = 1;
-        } else {
-          P.nb = d.V; P.M = d.Op; P.N = d.V;
-          P.C = ws ? ws + r.t2hacc + (size_t)tr.t[p[2]] * Op * V * V * 8 : nullptr;
-          P.c_sb = V; P.c_sm = V * V; P.c_sn = 1;
+    for (const B4Term& T : b4_terms())
+      for (size_t s = 0; s < bt.size(); s++)
+        for (int rho = 0; rho < 6; rho++) {
+          const int* p = PERM[rho];
+          const Triple& tr = bt[s];
+          const bool t2layout = (p[1] == 2);
+          // t2 layout (the cube's c is t_p1's partner): t2_bar[t_p2][m][b][n],
+          // accumulated in place (batches and columns: the v that exist, rows:
+          // the o that exist); otherwise the [i][j][b][a] accumulator t2hacc,
+          // added to t2_bar transposed at the end (writing t2_bar directly
+          // would take uncoalesced stores: the cube's contiguous index is not
+          // t2_bar's)
+          GemmProb P{};
+          if (t2layout) {
+            P.nb = d.v; P.M = d.Op; P.N = d.V; P.Mv = d.o; P.Nv = d.v;
+            P.C = outp(T.out, (size_t)tr.t[p[2]] * d.o * d.v * d.v);
+            P.c_sb = d.v; P.c_sm = (long long)d.v * d.v; P.c_sn = 1;
+          } else {
+            P.nb = d.V; P.M = d.Op; P.N = d.V;
+            P.C = ws ? ws + T.hacc + (size_t)tr.t[p[2]] * Op * V * V * 8 : nullptr;
+            P.c_sb = V; P.c_sm = V * V; P.c_sn = 1;
+          }
+          P.alpha = 1.0;
+          const int q = gb.problem(T.key + (t2layout ? 0 : 1) * 4096LL + tr.t[p[2]], P);
+          GemmSeg sg{};
+          sg.A = el(T.ovoo, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
+          sg.a_sb = 0; sg.a_sm = 1; sg.a_sk2 = Op;
+          sg.B = gcube(T.g, (int)s);
+          sg.b_sk2 = str(p[0]);
+          if (t2layout) { sg.b_sb = str(p[2]); sg.b_sn = str(p[1]); }
+          else { sg.b_sb = str(p[1]); sg.b_sn = str(p[2]); }
+          sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
+          gb.add(q, sg);
         }
-        P.alpha = 1.0;
-        const int q = gb.problem((t2layout ? 0 : 1) * 4096LL + tr.t[p[2]], P);
-        GemmSeg sg{};
-        sg.A = el(r.ovooT, ((long long)tr.t[p[0]] * d.o + tr.t[p[1]]) * V * Op);
-        sg.a_sb = 0; sg.a_sm = 1; sg.a_sk2 = Op;
-        sg.B = gslot((int)s);
-        sg.b_sk2 = str(p[0]);
-        if (t2layout) { sg.b_sb = str(p[2]); sg.b_sn = str(p[1]); }
-        else { sg.b_sb = str(p[1]); sg.b_sn = str(p[2]); }
-        sg.K1 = 1; sg.K2 = d.V; sg.scale = -1.0f;
-        gb.add(q, sg);
-      }
     emit_gemm(gb, cfg_for(FAM_GEMM_B4, d.Op % 64 ? 4 : 0), GEMM_ACCUM, 1, false, FAM_GEMM_B4);
   }
 
@@ -1012,7 +1166,7 @@ struct Ctx {
     for (size_t s = 0; s < bt.size(); s++) {
       VpTriple t{};
       t.i = bt[s].t[0]; t.j = bt[s].t[1]; t.k = bt[s].t[2];
-      t.H = slot((int)s, r.slot_H);
+      t.H = hcube((int)s);
       t.out = reinterpret_cast<double*>(slot((int)s, r.slot_vp));
       trs.push_back(t);
       for (int rho = 0; rho < 6; rho++) {
@@ -1093,6 +1247,7 @@ struct Ctx {
         std::vector<Triple> bt(all.begin() + b0, all.begin() + std::min(g1, b0 + B));
         gbase = (int)(b0 - g0);
         emit_wbuild(bt);
+        if (d.lno) emit_sbuild(bt);
         emit_epilogue(bt, backward);
         if (backward) {
           emit_b2(bt);
@@ -1197,9 +1352,10 @@ int check_args(int nocc, int nvir, int precision) {
 
 // Size the descriptor blob by building the plan against a null workspace.
 size_t plan_bytes(int nocc, int nvir, int precision, bool backward, Regions* out_r, int* out_B,
-                  long long t_begin = 0, long long t_end = -1, int batch = -1, int naux = 0, int dfblk = 0) {
+                  long long t_begin = 0, long long t_end = -1, int batch = -1, int naux = 0, int dfblk = 0,
+                  int lno = 0) {
   Ctx c;
-  c.d = dims_for(nocc, nvir, precision, naux, backward, dfblk);
+  c.d = dims_for(nocc, nvir, precision, naux, backward, dfblk, lno);
   c.B = batch > 0 ? batch : choose_batch(c.d);
   c.backward = backward;
   c.t_begin = t_begin;
@@ -1226,11 +1382,28 @@ struct Problem {
   int naux = 0;
   const double *Lov = nullptr, *Lvv = nullptr;
   double *Lov_bar = nullptr, *Lvv_bar = nullptr;
+  // LNO form (pyscfad/lno/ccsd_t.py:70): mat (o,o), t1T (v,o), t2T (v,v,o,o)
+  // and fvo (v,o) replace t1, t2 and fock, and the cotangents follow them
+  bool lno = false;
+  const double *mat = nullptr, *t1T = nullptr, *t2T = nullptr, *fvo = nullptr;
+  double *mat_bar = nullptr, *t1T_bar = nullptr, *t2T_bar = nullptr, *fvo_bar = nullptr;
 };
 
 int check_problem(const Problem& p, int precision) {
   int rc = check_args(p.nocc, p.nvir, precision);
   if (rc) return rc;
+  if (p.lno) {
+    if (p.nocc > t3_lno_max_nocc())
+      return fail(PYSCFAD_AMD_ERR_ARGS, "the LNO form takes nocc <= " + std::to_string(t3_lno_max_nocc()));
+    if (p.packed != 0 && p.packed != 1) return fail(PYSCFAD_AMD_ERR_ARGS, "ovvv_packed must be 0 or 1");
+    if (!p.mat || !p.t1T || !p.t2T || !p.mo_energy || !p.fvo || !p.ovoo || !p.ovov || !p.ovvv)
+      return fail(PYSCFAD_AMD_ERR_ARGS, "null input pointer");
+    if (p.backward && (!p.mat_bar || !p.t1T_bar || !p.t2T_bar || !p.mo_energy_bar || !p.fvo_bar || !p.ovoo_bar ||
+                       !p.ovov_bar || !p.ovvv_bar))
+      return fail(PYSCFAD_AMD_ERR_ARGS, "null output pointer");
+    if (!p.backward && !p.e_out) return fail(PYSCFAD_AMD_ERR_ARGS, "null e_out");
+    return PYSCFAD_AMD_OK;
+  }
   const bool df = p.naux > 0;
   if (p.naux < 0 || p.naux > 1000000) return fail(PYSCFAD_AMD_ERR_ARGS, "naux must be in 1..1000000");
   if (!p.mo_energy || !p.t1 || !p.t2 || !p.fock || !p.ovoo || !p.ovov || (df ? (!p.Lov || !p.Lvv) : !p.ovvv))
@@ -1299,15 +1472,106 @@ hipError_t enqueue_finish(Ctx& c, const Problem& p, hipStream_t st) {
   return hipSuccess;
 }
 
+// LNO form: the canonical problem the plan runs on.  Its t1, t2, fock (and
+// their cotangents) are workspace arrays formed from (and folded back into)
+// the caller's t1T, t2T, fvo; the other inputs and outputs are the caller's.
+Problem lno_inner(const Ctx& c, const Problem& p) {
+  Problem q = p;
+  q.lno = false;
+  q.t1 = c.at<double>(c.r.lt1);
+  q.t2 = c.at<double>(c.r.lt2);
+  q.fock = c.at<double>(c.r.lfock);
+  if (p.backward) {
+    q.t1_bar = c.at<double>(c.r.lt1_bar);
+    q.t2_bar = c.at<double>(c.r.lt2_bar);
+    q.fock_bar = c.at<double>(c.r.lfock_bar);
+  }
+  return q;
+}
+
+// Sizes of the projected tensors viewed as [A][o][B] around the slot.
+struct LnoViews {
+  long long ovvv_B;                 // ovvv (o, v, pairs): slot 0
+  long long t2_B0, t2_A1, t2_B1;    // t2 (o, o, v, v): slot 0, slot 1
+  long long ovoo_B0, ovoo_A3;       // ovoo (o, v, o, o): slot 0, slot 3
+};
+LnoViews lno_views(const Problem& p) {
+  const long long o = p.nocc, v = p.nvir;
+  return LnoViews{v * (p.packed ? v * (v + 1) / 2 : v * v), o * v * v, o, v * v, v * o * o, o * v * o};
+}
+
+// Before the canonical setup: t1, t2, fock from t1T, t2T, fvo, then the
+// slot-projected inputs (mat's column on the slot, lno_ccsd_t.c:16-18) in the
+// reference layouts, then their device layouts.
+hipError_t lno_setup(Ctx& c, const Problem& p, hipStream_t st) {
+  const int o = p.nocc, v = p.nvir;
+  const long long vv = (long long)v * v, oo = (long long)o * o;
+  const LnoViews w = lno_views(p);
+  double *t1 = c.at<double>(c.r.lt1), *t2 = c.at<double>(c.r.lt2);
+  double *ovvvP = c.at<double>(c.r.lovvvP), *t2Q = c.at<double>(c.r.lt2Q), *t2P0 = c.at<double>(c.r.lt2P0);
+  double* ovooQ = c.at<double>(c.r.lovooQ);
+  hipError_t e;
+  if ((e = t3_lno_transpose(t1, p.t1T, v, o, o, 0, 0, st)) != hipSuccess) return e;        // t1[i,a] = t1T[a,i]
+  if ((e = t3_lno_transpose(t2, p.t2T, vv, oo, o, 0, 1, st)) != hipSuccess) return e;      // t2[j,i,a,b] = t2T[a,b,i,j]
+  if ((e = t3_lno_fock(c.at<double>(c.r.lfock), p.fvo, o, v, 1, st)) != hipSuccess) return e;
+  if ((e = t3_lno_project(ovvvP, p.ovvv, p.mat, 1, o, w.ovvv_B, 0, 0, st)) != hipSuccess) return e;
+  if ((e = t3_lno_project(t2Q, t2, p.mat, 1, o, w.t2_B0, 0, 0, st)) != hipSuccess) return e;
+  if ((e = t3_lno_project(t2Q, t2, p.mat, w.t2_A1, o, w.t2_B1, 0, 1, st)) != hipSuccess) return e;
+  if ((e = t3_lno_project(t2P0, t2, p.mat, 1, o, w.t2_B0, 0, 0, st)) != hipSuccess) return e;
+  if ((e = t3_lno_project(ovooQ, p.ovoo, p.mat, 1, o, w.ovoo_B0, 0, 0, st)) != hipSuccess) return e;
+  if ((e = t3_lno_project(ovooQ, p.ovoo, p.mat, w.ovoo_A3, o, 1, 0, 1, st)) != hipSuccess) return e;
+  return t3_prep_projected(c.d.tp_is_f64, c.ws + c.r.ovvvT_P, c.ws + c.r.t2T_Q, c.ws + c.r.t2T_P0, c.ws + c.r.t2H_P0,
+                           c.ws + c.r.ovooT_Q, ovvvP, p.packed, t2Q, t2P0, ovooQ, o, v, c.d.Op, c.d.V, st);
+}
+
+// After the canonical write-back (backward): the projected operands'
+// cotangents in the reference layouts; mat_bar[l][i] = sum over each
+// projected slot of input[.., l, ..] * cotangent[.., i, ..]
+// (lno_ccsd_t_vjp.c:104-106 for the cube, here through the operands); the
+// back-projections onto the inputs' own cotangents; t1T_bar, t2T_bar, fvo_bar.
+hipError_t lno_finish(Ctx& c, const Problem& p, const Problem& q, hipStream_t st) {
+  const int o = p.nocc, v = p.nvir;
+  const long long vv = (long long)v * v, oo = (long long)o * o;
+  const LnoViews w = lno_views(p);
+  double *ovvvPb = c.at<double>(c.r.lovvvP_bar), *t2Qb = c.at<double>(c.r.lt2Q_bar);
+  double *t2P0b = c.at<double>(c.r.lt2P0_bar), *ovooQb = c.at<double>(c.r.lovooQ_bar);
+  double* part = c.at<double>(c.r.matpart);
+  const double* t2 = q.t2;
+  hipError_t e;
+  if ((e = t3_finalize_projected(ovvvPb, p.packed, c.at<double>(c.r.ovvvP_acc), t2P0b, c.at<double>(c.r.t2P0hacc),
+                                 ovooQb, c.at<double>(c.r.ovooQ_acc), o, v, c.d.Op, c.d.V, st)) != hipSuccess)
+    return e;
+  struct Term { const double* x; const double* y; double* xbar; long long A, B; };
+  const Term terms[6] = {
+      {p.ovvv, ovvvPb, p.ovvv_bar, 1, w.ovvv_B},
+      {p.ovoo, ovooQb, p.ovoo_bar, 1, w.ovoo_B0},
+      {p.ovoo, ovooQb, p.ovoo_bar, w.ovoo_A3, 1},
+      {t2, t2Qb, q.t2_bar, 1, w.t2_B0},
+      {t2, t2Qb, q.t2_bar, w.t2_A1, w.t2_B1},
+      {t2, t2P0b, q.t2_bar, 1, w.t2_B0},
+  };
+  for (int k = 0; k < 6; k++) {
+    const Term& T = terms[k];
+    if ((e = t3_lno_matbar(p.mat_bar, part, T.x, T.y, T.A, o, T.B, k > 0, st)) != hipSuccess) return e;
+    if ((e = t3_lno_project(T.xbar, T.y, p.mat, T.A, o, T.B, 1, 1, st)) != hipSuccess) return e;
+  }
+  if ((e = t3_lno_transpose(p.t1T_bar, q.t1_bar, o, v, o, 0, 0, st)) != hipSuccess) return e;
+  if ((e = t3_lno_transpose(p.t2T_bar, q.t2_bar, oo, vv, o, 1, 0, st)) != hipSuccess) return e;
+  return t3_lno_fock(p.fvo_bar, q.fock_bar, o, v, 0, st);
+}
+
 hipError_t enqueue(Ctx& c, const Problem& p, hipStream_t st, hipEvent_t* sev) {
   hipError_t e;
   auto mark = [&](int i) { return sev ? hipEventRecord(sev[i], st) : hipSuccess; };
+  const Problem q = p.lno ? lno_inner(c, p) : p;
   if ((e = mark(0)) != hipSuccess) return e;
-  if ((e = enqueue_setup(c, p, st)) != hipSuccess) return e;
+  if (p.lno && (e = lno_setup(c, p, st)) != hipSuccess) return e;
+  if ((e = enqueue_setup(c, q, st)) != hipSuccess) return e;
   if ((e = mark(1)) != hipSuccess) return e;
   if ((e = c.execute(st)) != hipSuccess) return e;
   if ((e = mark(2)) != hipSuccess) return e;
-  if ((e = enqueue_finish(c, p, st)) != hipSuccess) return e;
+  if ((e = enqueue_finish(c, q, st)) != hipSuccess) return e;
+  if (p.lno && p.backward && (e = lno_finish(c, p, q, st)) != hipSuccess) return e;
   return mark(3);
 }
 
@@ -1363,11 +1627,12 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
   }
 
   Ctx c;
-  c.d = dims_for(p.nocc, p.nvir, precision, p.naux, p.backward);
+  c.d = dims_for(p.nocc, p.nvir, precision, p.naux, p.backward, 0, p.lno ? 1 : 0);
   c.t_begin = t_begin;
   c.t_end = t_end;
   bind(c, p);
-  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end, batch, p.naux);
+  const size_t need = plan_bytes(p.nocc, p.nvir, precision, p.backward, &c.r, &c.B, t_begin, t_end, batch, p.naux,
+                                 0, p.lno ? 1 : 0);
   bool own = false;
   if (workspace) {
     if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
@@ -1381,6 +1646,11 @@ int run(const Problem& p, int precision, void* workspace, size_t workspace_bytes
       return fail(PYSCFAD_AMD_ERR_OOM, "hipMalloc of " + std::to_string(need) + " bytes failed");
     }
     own = true;
+  }
+  if (p.lno) {   // the plan reads the workspace copies of t1, t2 and writes the projected t2 cotangents there
+    bind(c, lno_inner(c, p));
+    c.t2Q_out = p.backward ? c.at<double>(c.r.lt2Q_bar) : nullptr;
+    c.t2P0_out = p.backward ? c.at<double>(c.r.lt2P0_bar) : nullptr;
   }
   c.build_plan();
   if (!c.plan_error.empty()) {
@@ -2100,6 +2370,40 @@ int pyscfad_amd_ccsd_t_df_energy_vjp(int nocc, int nvir, int naux, const double*
   p.Lov_bar = Lov_bar;
   p.Lvv_bar = Lvv_bar;
   return run(p, precision, workspace, workspace_bytes, stream, triple_begin, triple_end);
+}
+
+size_t pyscfad_amd_lno_ccsd_t_workspace_bytes(int nocc, int nvir, int precision, int with_vjp) {
+  if (check_args(nocc, nvir, precision) || nocc > t3_lno_max_nocc()) return 0;
+  return plan_bytes(nocc, nvir, precision, with_vjp != 0, nullptr, nullptr, 0, -1, -1, 0, 0, 1);
+}
+
+int pyscfad_amd_lno_ccsd_t_energy(int nocc, int nvir, const double* mat, const double* t1T, const double* t2T,
+                                  const double* mo_energy, const double* fvo, const double* ovoo, const double* ovov,
+                                  const double* ovvv, int ovvv_packed, int precision, double* e_out, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  Problem p;
+  p.lno = true;
+  p.nocc = nocc; p.nvir = nvir; p.packed = ovvv_packed; p.backward = false;
+  p.mat = mat; p.t1T = t1T; p.t2T = t2T; p.mo_energy = mo_energy; p.fvo = fvo;
+  p.ovoo = ovoo; p.ovov = ovov; p.ovvv = ovvv; p.e_out = e_out;
+  return run(p, precision, workspace, workspace_bytes, stream);
+}
+
+int pyscfad_amd_lno_ccsd_t_energy_vjp(int nocc, int nvir, const double* mat, const double* t1T, const double* t2T,
+                                      const double* mo_energy, const double* fvo, const double* ovoo,
+                                      const double* ovov, const double* ovvv, int ovvv_packed, double et_bar,
+                                      int precision, double* mat_bar, double* t1T_bar, double* t2T_bar,
+                                      double* mo_energy_bar, double* fvo_bar, double* ovoo_bar, double* ovov_bar,
+                                      double* ovvv_bar, double* e_out, void* workspace, size_t workspace_bytes,
+                                      void* stream) {
+  Problem p;
+  p.lno = true;
+  p.nocc = nocc; p.nvir = nvir; p.packed = ovvv_packed; p.backward = true; p.et_bar = et_bar;
+  p.mat = mat; p.t1T = t1T; p.t2T = t2T; p.mo_energy = mo_energy; p.fvo = fvo;
+  p.ovoo = ovoo; p.ovov = ovov; p.ovvv = ovvv; p.e_out = e_out;
+  p.mat_bar = mat_bar; p.t1T_bar = t1T_bar; p.t2T_bar = t2T_bar; p.mo_energy_bar = mo_energy_bar;
+  p.fvo_bar = fvo_bar; p.ovoo_bar = ovoo_bar; p.ovov_bar = ovov_bar; p.ovvv_bar = ovvv_bar;
+  return run(p, precision, workspace, workspace_bytes, stream);
 }
 
 const char* pyscfad_amd_last_error(void) { return g_last_error.c_str(); }

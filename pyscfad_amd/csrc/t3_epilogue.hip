@@ -30,6 +30,17 @@
 //             adjoint (ccsd_t_vjp.c:98-117).
 // Per-block partial sums go to a [triple][cube][32] FP64 table reduced later
 // in a fixed order; nothing is accumulated with atomics.
+//
+// LNO form (the occupied projector of lno_ccsd_t.c:6-29): the triple also
+// carries S = (P1 + P2 + P3) w0, w0 with each occupied slot in turn projected
+// by mat (built by the GEMMs from projected operands).  Its energy is
+//     E = 2/3 sum_{i>=j>=k} f_ijk sum_{abc} S Z / D,
+// which for mat = I (S = 3 w0) is E(T).  The summand pairs S with Z at the
+// same orbit points, so the block computes Q_ijk = 1/3 sum f_abc sum_pi S Z / D
+// and, backward, with s = 2/3 f_ijk et_bar,
+//     GS = s Z / D (the adjoint of S),  G = H = s (C S) / D
+// (the adjoints of w0 and v0 through Z only); H is not written, its
+// consumers read G.
 #include "t3_common.h"
 
 namespace t3 {
@@ -95,7 +106,7 @@ __device__ __forceinline__ VT xor_perm(VT v, int k) {
   }
 }
 
-template <class TP, bool BW>
+template <class TP, bool BW, bool LNO>
 __global__ void __launch_bounds__(256)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   constexpr int T = EPI_T, T3 = T * T * T;
@@ -121,6 +132,8 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
   __shared__ __attribute__((aligned(16))) TP sc[6][T3];   // w0 sub-cubes, later G
+  // LNO form: the S sub-cubes
+  __shared__ __attribute__((aligned(16))) TP ss[LNO ? 6 : 1][LNO ? T3 : 1];
   __shared__ __attribute__((aligned(16))) double cmh[NU]; // CM/CN tile-pair blocks [d][pair][M/N][8x8], later H
   __shared__ double vec[2][3][3][T];                      // [t1/fvo][d][tile pos][8]
   __shared__ double wred[4][20];                          // per-wave partial sums
@@ -139,8 +152,13 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       case 4: q0 = TC; q1 = TA; q2 = TB; break;
       default: q0 = TC; q1 = TB; q2 = TA; break;
     }
-    const tpv x = *reinterpret_cast<const tpv*>(&w0[(((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2)]);
+    const long long off = (((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2);
+    const tpv x = *reinterpret_cast<const tpv*>(&w0[off]);
     *reinterpret_cast<tpv*>(&sc[p][sub_pos(x0, x1, x2 ^ (x0 & (VW - 1)))]) = xor_perm<tpv, VW>(x, x0 & (VW - 1));
+    if constexpr (LNO) {
+      const tpv y = *reinterpret_cast<const tpv*>(&reinterpret_cast<const TP*>(tr.s)[off]);
+      *reinterpret_cast<tpv*>(&ss[p][sub_pos(x0, x1, x2 ^ (x0 & (VW - 1)))]) = xor_perm<tpv, VW>(y, x0 & (VW - 1));
+    }
   }
   // --- CM_d / CN_d on the ordered tile pairs: block [d][pair mu][which][8x8] ---
   {
@@ -168,7 +186,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   __syncthreads();
 
   const double eijk = args.eps_occ[tr.i] + args.eps_occ[tr.j] + args.eps_occ[tr.k];
-  const double s_bw = 2.0 * tr.fo * args.et_bar;
+  const double s_bw = (LNO ? 2.0 / 3.0 : 2.0) * tr.fo * args.et_bar;
   const bool eqAB = (TA == TB), eqBC = (TB == TC);
 
   // two points per thread: pt = tid and tid + 256, i.e. al0 = wave (+4), al1 = lane bits 3-5,
@@ -186,11 +204,12 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     const int s0 = TA * T + al[0], s1 = TB * T + al[1], s2 = TC * T + al[2];
     const double fp = (s0 == s2) ? (1.0 / 6.0) : ((s0 == s1 || s1 == s2) ? 0.5 : 1.0);
     const double D = eijk - args.eps_vir[s0] - args.eps_vir[s1] - args.eps_vir[s2];
-    double W[6], U[6];
+    double W[6], U[6], Sv[6];
 #pragma unroll
     for (int p = 0; p < 6; p++) {
       const int x0 = al[c_perm[p][0]], x1 = al[c_perm[p][1]], x2 = al[c_perm[p][2]];
       W[p] = (double)sc[p][sub_pos(x0, x1, x2)];
+      if constexpr (LNO) Sv[p] = (double)ss[p][sub_pos(x0, x1, x2)];
       // v0 at the orbit point pi(s): position m of the point holds s_{pi_m}
       double vv = 0.0;
 #pragma unroll
@@ -201,23 +220,33 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       }
       U[p] = W[p] + 0.5 * vv;
     }
-    double sU[2] = {0.0, 0.0}, sW[2] = {0.0, 0.0};
+    // X: the first factor of the summand (w0, or S in the LNO form)
+    double sU[2] = {0.0, 0.0}, sX[2] = {0.0, 0.0};
 #pragma unroll
-    for (int p = 0; p < 6; p++) { sU[c_parity[p]] += U[p]; sW[c_parity[p]] += W[p]; }
+    for (int p = 0; p < 6; p++) {
+      sU[c_parity[p]] += U[p];
+      sX[c_parity[p]] += LNO ? Sv[p] : W[p];
+    }
     double wz = 0.0;
     const double invD = 1.0 / D;
 #pragma unroll
     for (int p = 0; p < 6; p++) {
       const int par = c_parity[p];
+      const double X = LNO ? Sv[p] : W[p];
       const double Z = 3.0 * U[p] + sU[par] - 2.0 * sU[par ^ 1];
-      const double CW = 3.0 * W[p] + sW[par] - 2.0 * sW[par ^ 1];
-      wz += W[p] * Z;
+      const double CX = 3.0 * X + sX[par] - 2.0 * sX[par ^ 1];
+      wz += X * Z;
       if constexpr (BW) {
-        gval[h][p] = s_bw * (Z + CW) * invD;
-        hval[h][p] = s_bw * CW * invD;
+        if constexpr (LNO) {
+          gval[h][p] = s_bw * CX * invD;   // G (= H)
+          hval[h][p] = s_bw * Z * invD;    // GS
+        } else {
+          gval[h][p] = s_bw * (Z + CX) * invD;
+          hval[h][p] = s_bw * CX * invD;
+        }
       }
     }
-    qpt[h] = fp * wz * invD;                    // Q contribution
+    qpt[h] = (LNO ? 1.0 / 3.0 : 1.0) * fp * wz * invD;   // Q contribution
     rpt[h] = s_bw * fp * wz * invD * invD;      // mo_energy adjoint weight
   }
 
@@ -274,7 +303,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   // canonical point and no lower-numbered sub-cube covering the same global
   // block also holds it (one writer per element).
   TP* G = reinterpret_cast<TP*>(tr.G);
-  TP* H = reinterpret_cast<TP*>(tr.H);
+  TP* H = reinterpret_cast<TP*>(LNO ? tr.GS : tr.H);   // the second LDS set: H, or GS in the LNO form
   const int tt[3] = {TA, TB, TC};
   const bool distinct = !eqAB && !eqBC;
 #pragma unroll
@@ -422,13 +451,23 @@ extern "C" hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, con
                                          int ntriple, hipStream_t stream) {
   if (ntriple <= 0) return hipSuccess;
   dim3 grid(args->ncube, ntriple), block(256);
-  if (args->backward) {
-    if (tp_is_f64) hipLaunchKernelGGL((orbit_epilogue<double, true>), grid, block, 0, stream, *args, triples);
-    else hipLaunchKernelGGL((orbit_epilogue<float, true>), grid, block, 0, stream, *args, triples);
+#define T3_EPI(TP, BW, LNO) hipLaunchKernelGGL((orbit_epilogue<TP, BW, LNO>), grid, block, 0, stream, *args, triples)
+  if (args->lno) {
+    if (args->backward) {
+      if (tp_is_f64) T3_EPI(double, true, true);
+      else T3_EPI(float, true, true);
+    } else {
+      if (tp_is_f64) T3_EPI(double, false, true);
+      else T3_EPI(float, false, true);
+    }
+  } else if (args->backward) {
+    if (tp_is_f64) T3_EPI(double, true, false);
+    else T3_EPI(float, true, false);
   } else {
-    if (tp_is_f64) hipLaunchKernelGGL((orbit_epilogue<double, false>), grid, block, 0, stream, *args, triples);
-    else hipLaunchKernelGGL((orbit_epilogue<float, false>), grid, block, 0, stream, *args, triples);
+    if (tp_is_f64) T3_EPI(double, false, false);
+    else T3_EPI(float, false, false);
   }
+#undef T3_EPI
   return hipGetLastError();
 }
 

@@ -527,7 +527,8 @@ __global__ void __launch_bounds__(256) fin_t2(double* __restrict__ out, const do
 }
 
 // ovoo_bar[p,x,m,q] = acc[p][q][x][m]; t1_bar, fock_bar and mo_energy_bar from
-// their accumulators (ovov_bar is accumulated in place)
+// their accumulators (ovov_bar is accumulated in place).  With t1_bar null,
+// ovoo_bar only.
 __global__ void fin_small(double* __restrict__ ovoo_bar, double* __restrict__ t1_bar,
                           double* __restrict__ fock_bar, double* __restrict__ mo_energy_bar,
                           const double* __restrict__ ovoo_acc, const double* __restrict__ t1_acc,
@@ -535,7 +536,7 @@ __global__ void fin_small(double* __restrict__ ovoo_bar, double* __restrict__ t1
                           int v, int Op, int V) {
   const int n = o + v;
   const long long n1 = (long long)o * v * o * o, n3 = (long long)o * v, n4 = (long long)n * n, n5 = n;
-  const long long total = n1 + n3 + n4 + n5;
+  const long long total = t1_bar ? n1 + n3 + n4 + n5 : n1;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (long long)gridDim.x * blockDim.x) {
     long long r = e;
@@ -597,6 +598,52 @@ extern "C" hipError_t t3_prep_inputs(int tp_is_f64, void* ovvvT, void* t2T, void
   }
   const long long ne = (long long)o * V > V ? (long long)o * V : V;
   hipLaunchKernelGGL(prep_eps, grid_for(ne), dim3(256), 0, s, eps_occ, eps_vir, fvoP, mo_energy, fock, o, v, V);
+  return hipGetLastError();
+}
+
+// Device layouts of the slot-projected operands of the LNO form, from their
+// reference-layout FP64 forms (any of them may be null):
+//   ovvvT_P <- ovvvP;  t2T_Q <- t2Q;  t2T_P0, t2H_P0 <- t2P0;  ovooT_Q <- ovooQ.
+// prep_t2 writes both t2 layouts: t2Q's t2H goes to t2H_P0 first and is
+// overwritten by t2P0's (stream order), so no buffer is needed for it.
+extern "C" hipError_t t3_prep_projected(int tp_is_f64, void* ovvvT_P, void* t2T_Q, void* t2T_P0, void* t2H_P0,
+                                        void* ovooT_Q, const double* ovvvP, int packed, const double* t2Q,
+                                        const double* t2P0, const double* ovooQ, int o, int v, int Op, int V,
+                                        hipStream_t s) {
+  const long long n_ovoo = (long long)o * o * V * Op;
+  const dim3 g_ovvv = tile_grid((long long)(V / TR_T) * (V / TR_T), (long long)o * V);
+  const dim3 g_t2 = tile_grid((long long)(V / TR_T) * (V / TR_T), (long long)Op * Op);
+  if (tp_is_f64) {
+    if (ovvvP) hipLaunchKernelGGL(prep_ovvv<double>, g_ovvv, dim3(256), 0, s, (double*)ovvvT_P, ovvvP, o, v, V, packed);
+    if (t2Q) hipLaunchKernelGGL(prep_t2<double>, g_t2, dim3(256), 0, s, (double*)t2T_Q, (double*)t2H_P0, t2Q, o, v, Op, V);
+    if (t2P0) hipLaunchKernelGGL(prep_t2<double>, g_t2, dim3(256), 0, s, (double*)t2T_P0, (double*)t2H_P0, t2P0, o, v, Op, V);
+    if (ovooQ) hipLaunchKernelGGL(prep_ovoo<double>, grid_for(n_ovoo), dim3(256), 0, s, (double*)ovooT_Q, ovooQ, o, v, V, Op);
+  } else {
+    if (ovvvP) hipLaunchKernelGGL(prep_ovvv<float>, g_ovvv, dim3(256), 0, s, (float*)ovvvT_P, ovvvP, o, v, V, packed);
+    if (t2Q) hipLaunchKernelGGL(prep_t2<float>, g_t2, dim3(256), 0, s, (float*)t2T_Q, (float*)t2H_P0, t2Q, o, v, Op, V);
+    if (t2P0) hipLaunchKernelGGL(prep_t2<float>, g_t2, dim3(256), 0, s, (float*)t2T_P0, (float*)t2H_P0, t2P0, o, v, Op, V);
+    if (ovooQ) hipLaunchKernelGGL(prep_ovoo<float>, grid_for(n_ovoo), dim3(256), 0, s, (float*)ovooT_Q, ovooQ, o, v, V, Op);
+  }
+  return hipGetLastError();
+}
+
+// Reference-layout FP64 cotangents of the slot-projected LNO operands from
+// their device accumulators (any output may be null):
+//   ovvvP_bar <- ovvvP_acc (packed like the input);  t2P0_bar += t2P0hacc^T
+//   (the rest of t2P0_bar accumulated in place);  ovooQ_bar <- ovooQ_acc.
+extern "C" hipError_t t3_finalize_projected(double* ovvvP_bar, int packed, const double* ovvvP_acc,
+                                            double* t2P0_bar, const double* t2P0hacc, double* ovooQ_bar,
+                                            const double* ovooQ_acc, int o, int v, int Op, int V, hipStream_t s) {
+  const long long ntv = (v + TR_T - 1) / TR_T;
+  if (ovvvP_bar)
+    hipLaunchKernelGGL(fin_ovvv, tile_grid(packed ? ntv * (ntv + 1) / 2 : ntv * ntv, (long long)o * v), dim3(256), 0, s,
+                       ovvvP_bar, ovvvP_acc, o, v, V, packed);
+  if (t2P0_bar)
+    hipLaunchKernelGGL(fin_t2, tile_grid(ntv * ntv, (long long)o * o), dim3(256), 0, s, t2P0_bar, t2P0hacc, o, v, Op, V);
+  if (ovooQ_bar)
+    hipLaunchKernelGGL(fin_small, grid_for((long long)o * v * o * o), dim3(256), 0, s, ovooQ_bar, (double*)nullptr,
+                       (double*)nullptr, (double*)nullptr, ovooQ_acc, (const double*)nullptr, (const double*)nullptr,
+                       (const double*)nullptr, o, v, Op, V);
   return hipGetLastError();
 }
 

@@ -156,3 +156,36 @@ def flops_energy(nocc, nvir):
     v^3 W evaluations of a (o^2 x v)(v x o) and an (o x o)(o x o^2) product."""
     o, v = float(nocc), float(nvir)
     return 2.0 * o ** 3 * v ** 3 * (v + o)
+
+
+def make_lno_instance(nocc, nvir, seed, mat="gram", fvo_scale=0.02, packed=True, integral_scale=1.0,
+                      nlo=None):
+    """The inputs of the LNO (T) (pyscfad/lno/ccsd_t.py:70) in the forms that
+    function takes, from the instance ``make_instance`` builds: a dict with
+    mat (o,o), t1T (v,o), t2T = t2.transpose(2,3,1,0) (v,v,o,o), mo_energy,
+    fvo = fock[o:, :o], ovoo, ovov, ovvv, plus "canonical": the make_instance
+    dict.  ``mat``: "gram" (ulo^T ulo with ulo of shape (nlo, o), entries
+    N(0, 1/o), nlo = o//2 + 1 by default, as lno/ccsd_t.py:42 forms it),
+    "identity", or "random" (I + N(0, 0.3^2), not symmetric).  The projector
+    draws come from ``seed + 2000``, after the instance's own."""
+    d = make_instance(nocc, nvir, seed, fvo_scale=fvo_scale, packed=packed, integral_scale=integral_scale)
+    rng = np.random.default_rng(seed + 2000)
+    if mat == "identity":
+        m = np.eye(nocc)
+    elif mat == "random":
+        m = np.eye(nocc) + 0.3 * rng.normal(size=(nocc, nocc))
+    elif mat == "gram":
+        k = nlo if nlo is not None else nocc // 2 + 1
+        ulo = rng.normal(0.0, 1.0 / np.sqrt(nocc), size=(k, nocc))
+        m = ulo.T @ ulo
+    else:
+        raise ValueError(f"unknown mat kind {mat!r}")
+    return {
+        "mat": m,
+        "t1T": np.ascontiguousarray(d["t1"].T),
+        "t2T": np.ascontiguousarray(np.asarray(d["t2"]).transpose(2, 3, 1, 0)),
+        "mo_energy": d["mo_energy"],
+        "fvo": np.ascontiguousarray(np.asarray(d["fock"])[nocc:, :nocc]),
+        "ovoo": d["ovoo"], "ovov": d["ovov"], "ovvv": d["ovvv"],
+        "canonical": d,
+    }

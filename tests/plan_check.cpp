@@ -10,25 +10,33 @@
 // that interleaved launches have one tile grid and that no split is empty,
 // and counts the interleaved launches and the problems with their own split
 // count.
-// Usage: plan_check nocc nvir precision [naux [plant]]
+// Usage: plan_check nocc nvir precision [naux [plant | lno]]
 //   plant: moves the split-K scratch to the end of the workspace, so that a
 //   correct checker must report out-of-bounds partials.
+//   lno: the plan of the LNO form (second cube S from the slot-projected
+//   operands, the adjoint families' extra terms, cotangents of t2 and of the
+//   projected t2 operands in the workspace).
 #include DRIVER
 #include <cstdio>
 static const uintptr_t OUT_T2 = 1ull << 44, OUT_OVOV = 2ull << 44;
 int main(int argc, char** argv) {
   int o = atoi(argv[1]), v = atoi(argv[2]), prec = atoi(argv[3]);
   int naux = argc > 4 ? atoi(argv[4]) : 0;
+  const bool plant = argc > 5 && std::string(argv[5]) == "plant";
+  const bool lno = argc > 5 && std::string(argv[5]) == "lno";
   Ctx c;
-  c.d = dims_for(o, v, prec, naux, true);
+  c.d = dims_for(o, v, prec, naux, true, 0, lno ? 1 : 0);
   c.B = choose_batch(c.d);
   c.backward = true;
   c.r = plan_regions(c.d, c.B, true);
-  const bool plant = argc > 5 && std::string(argv[5]) == "plant";
   if (plant) c.r.split = c.r.desc - 4096;
-  c.t2_out = reinterpret_cast<double*>(OUT_T2);
-  c.ovov_out = reinterpret_cast<double*>(OUT_OVOV);
   c.ws = reinterpret_cast<char*>(1ull << 40);   // fake workspace base (never dereferenced)
+  c.t2_out = lno ? c.at<double>(c.r.lt2_bar) : reinterpret_cast<double*>(OUT_T2);
+  c.ovov_out = reinterpret_cast<double*>(OUT_OVOV);
+  if (lno) {
+    c.t2Q_out = c.at<double>(c.r.lt2Q_bar);
+    c.t2P0_out = c.at<double>(c.r.lt2P0_bar);
+  }
   c.build_plan();
   const uintptr_t WS = 1ull << 40;
   const size_t ws_size = c.r.desc;   // regions end where descriptors start

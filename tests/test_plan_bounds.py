@@ -2,7 +2,8 @@
 
 tests/plan_check.cpp builds the plan of a backward call with the library's
 own planning code and checks every GEMM and split-K reduction access against
-the workspace and the caller's in-place outputs (see its header).  It is
+the workspace and the caller's in-place outputs (see its header), also for
+the LNO form's plan.  It is
 compiled here with hipcc against the current sources and the kernel objects
 of the last build.  Covers the dense and the density-fitted plans, every
 precision, uneven sizes, batch merging (many triples per output slab) and
@@ -16,7 +17,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "pyscfad_amd", "_build")
-OBJS = [os.path.join(BUILD, f) for f in ("t3_gemm.hip.o", "t3_epilogue.hip.o", "t3_prep.hip.o", "t3_noalloc.hip.o")]
+OBJS = [os.path.join(BUILD, f) for f in ("t3_gemm.hip.o", "t3_epilogue.hip.o", "t3_prep.hip.o", "t3_noalloc.hip.o",
+                                         "t3_lno.hip.o")]
 HIPCC = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
 
 
@@ -74,6 +76,16 @@ def test_plan_interleave_and_split_counts(checker):
     m = re.search(r"interleaved=(\d+), split problems=(\d+) \((\d+) with", r.stdout)
     assert m, r.stdout
     assert int(m.group(1)) > 0 and int(m.group(2)) > 0 and int(m.group(3)) > 0, r.stdout
+
+
+@pytest.mark.parametrize("o,v,prec", [(5, 19, 0), (6, 40, 1), (12, 100, 0), (12, 100, 1), (7, 33, 2), (9, 64, 3),
+                                      (32, 256, 1), (2, 3, 0)])
+def test_lno_plan_in_bounds(checker, o, v, prec):
+    """The LNO form's plan: the S build from the projected operands and the
+    adjoint families' extra terms and outputs."""
+    r = _run(checker, o, v, prec, 0, "lno")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout and "plan_error=''" in r.stdout, r.stdout
 
 
 def test_checker_reports_planted_overflow(checker):
