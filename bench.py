@@ -70,6 +70,9 @@ def parse():
                          "backward in HBM; --triples then counts in the blocked order ('auto': the first "
                          "block triple of three distinct blocks)")
     ap.add_argument("--df-block", type=int, default=4, help="occupied indices per block in --df mode")
+    ap.add_argument("--lno", action="store_true",
+                    help="the LNO (T) with an occupied projector (a Gram mat of nocc//2+1 local orbitals), "
+                         "whole solves of --nocc x --nvir at molecular integral scale")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of this many MiB (reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -262,6 +265,8 @@ def main():
         sys.exit(2)
     if args.batched:
         return main_batched(args)
+    if args.lno:
+        return main_lno(args)
     import torch
     from pyscfad_amd import _lib
     from pyscfad_amd.cc import ccsd_t
@@ -602,6 +607,157 @@ def main_batched(args):
                      "gemm_ms_per_step": gemm_ms, "tuned_classes": tuned,
                      "family_ms": {k: val[0] for k, val in fam.items() if not k.startswith("gemm_")}},
         "cpu_baseline": None,
+        "accuracy_vs_f64": accuracy,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline_lno(x, nocc, nvir, target_s):
+    """LNO oracle energy + VJP on a bounded run of the LNO job list (every a,
+    then b >= c), timed on the host cores.  Every job costs the same
+    12 o^3 (v + o) flops of W forward (plus the o^4 projector products), so
+    the rate of a run stands for the list.  A job is credited three times its
+    forward (forward and adjoint): the list has about 3 F_E / (12 o^3 (v+o))
+    jobs, so this is the 9 F_E per solve the GPU path is credited with."""
+    from oracle import ccsd_t_oracle as orc
+    prep = orc.LnoPrepared(*[x[k] for k in ("mat", "t1T", "t2T", "mo_energy", "fvo", "ovoo", "ovov", "ovvv")])
+    flops_per_job = 12.0 * nocc ** 3 * (nvir + nocc)
+    nj = prep.njobs()
+
+    def run(j0, j1):
+        t0 = time.perf_counter()
+        prep.energy(j0, j1)
+        prep.energy_vjp(j0, j1)
+        return time.perf_counter() - t0
+
+    with orc.blas():
+        j0 = nj // 4
+        probe = max(2, min(64, nj // 8))
+        t_probe = run(j0, j0 + probe)
+        n = int(max(probe, min(nj - j0, probe * target_s / max(t_probe, 1e-3))))
+        t = run(j0, j0 + n)
+    rate = 3.0 * flops_per_job * n / t / 1e12
+    return {"value": rate, "unit": "TFLOP/s", "cores": orc.num_threads(), "kind": "port",
+            "sample": f"LNO energy+VJP of jobs [{j0}, {j0 + n}) of the {nj} (a, b >= c) of the nocc={nocc}, "
+                      f"nvir={nvir} instance in {t:.1f} s, FP64 C/OpenMP with scipy's BLAS (one BLAS thread per "
+                      f"OpenMP thread), credited 3 x 12 o^3 (v+o) per job"}
+
+
+def main_lno(args):
+    """LNO (T) with an occupied projector: whole energy+VJP solves.
+
+    Credited flops: 9 F_E per energy+VJP, F_E = 2 o^3 v^3 (v + o).  The
+    reference's LNO job list (all a, b >= c) is about 3x the canonical
+    a >= b >= c list, so its forward W costs about 3 F_E and the adjoint of W
+    twice that; the GPU engine forms the same 3 F_E forward as W (F_E) plus
+    the slot-projected S (2 F_E), and 6 F_E of adjoint contractions (2 over
+    G, 4 over the adjoint of S).  Its backward also recomputes W and S
+    (3 F_E executed, not credited)."""
+    import math
+    import torch
+    from pyscfad_amd import _lib
+    from pyscfad_amd.lno import ccsd_t as lno
+    from pyscfad_amd.synthetic import make_lno_instance, flops_energy
+
+    o, v = args.nocc, args.nvir
+    log(f"[bench] building the LNO instance nocc={o} nvir={v} seed={args.seed}")
+    x = make_lno_instance(o, v, args.seed, mat="gram", integral_scale=1.0 / math.sqrt(o + v))
+    keys = ("mat", "t1T", "t2T", "mo_energy", "fvo", "ovoo", "ovov", "ovvv")
+    a = [torch.as_tensor(x[k]).cuda().contiguous() for k in keys]
+    torch.cuda.synchronize()
+    W, K = max(args.warmup, 1), args.steps
+    fe = flops_energy(o, v)
+    credit = 9.0
+    wsp = {}
+
+    def workspace(prec):
+        if prec not in wsp:
+            wsp.clear()
+            torch.cuda.empty_cache()
+            wsp[prec] = torch.empty(lno.workspace_bytes(o, v, prec, with_vjp=True), dtype=torch.uint8, device="cuda")
+        return wsp[prec]
+
+    def step(prec, timing=False):
+        _lib.set_timing(timing)
+        e = lno.energy(*a, precision=prec, workspace=workspace(prec))
+        tf = _lib.last_timing() if timing else None
+        bars = lno.energy_vjp(*a, et_bar=1.0, precision=prec, workspace=workspace(prec))
+        tb = _lib.last_timing() if timing else None
+        _lib.set_timing(False)
+        fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf} if timing else None
+        return e, bars, fam
+
+    def timed(prec, n):
+        log(f"[bench] timing {n} LNO step(s) ({prec})")
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        e = bars = None
+        for _ in range(n):
+            bars = None
+            e, bars, _ = step(prec)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - c0) / n, e, bars
+
+    tuned = None
+    if not args.no_autotune:
+        # the tuning table is keyed by size and shared with the canonical
+        # path, whose contraction families the LNO form runs
+        log(f"[bench] autotuning tile shapes ({args.precision})")
+        from pyscfad_amd.cc import ccsd_t
+        tuned = ccsd_t.autotune(o, v, args.precision)
+        if args.precision != "f64" and not args.no_compare:
+            ccsd_t.autotune(o, v, "f64")
+    torch.cuda.reset_peak_memory_stats()
+    fam = None
+    for w in range(W):
+        log(f"[bench] LNO warm-up step {w} ({args.precision})" + (", kernel timing on" if w == 0 else ""))
+        _, _, f = step(args.precision, timing=(w == 0))
+        fam = fam or f
+    dt, e, bars = timed(args.precision, K)
+    peak_hbm = torch.cuda.max_memory_allocated() / 1e9
+    gemm_ms = fam["gemm"][0]
+    achieved = credit * fe / (gemm_ms / 1e3) / 1e12
+    peak = MI355X_PEAK_TFLOPS[args.precision]
+    f64 = accuracy = None
+    if args.precision != "f64" and not args.no_compare:
+        step("f64")
+        dt64, e64, bars64 = timed("f64", K)
+        f64 = {"tflops": credit * fe / dt64 / 1e12, "s_per_solve": dt64, "speedup_over_f64": dt64 / dt}
+        rel = {}
+        for n, b, r in zip(keys, bars, bars64):
+            rel[n] = max_rel(b, r)[0]
+        accuracy = {"dE_Eh": abs(float(e) - float(e64)), "E_f64": float(e64), "max_rel": rel}
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] LNO CPU baseline sample (~{args.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline_lno(x, o, v, args.cpu_seconds)
+    line = {
+        "metric": "LNO (T) energy+VJP TFLOP/s (9*F_E credited) & s/solve",
+        "value": credit * fe / dt / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": dt * 1e3,
+        "s_per_solve": dt,
+        "peak_hbm_GB": peak_hbm,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"LNO (T) instance nocc={o} nvir={v} seed={args.seed}, integral_scale "
+                               f"1/sqrt(n), Gram projector of {o // 2 + 1} local orbitals; whole energy+VJP "
+                               f"solve per step", "precision": args.precision, "E_last_step": float(e)},
+        "f64": f64,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": "gemm_grouped",
+                     "credited": "9 F_E of the first warm-up solve over its gemm_grouped device time (the "
+                                 "backward's W and S recomputes run in the same kernel and are not credited)",
+                     "gemm_ms_per_step": gemm_ms,
+                     "tiles": tuned,
+                     "family_ms": {k: val[0] for k, val in fam.items()}},
+        "cpu_baseline": cpu,
         "accuracy_vs_f64": accuracy,
     }
     print(json.dumps(line))

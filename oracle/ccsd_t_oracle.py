@@ -294,3 +294,34 @@ def lno_energy_vjp(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, et_bar=1.0, 
            np.ascontiguousarray(vvop_b[:, :, :, :nocc].transpose(2, 0, 3, 1)),
            pack_ovvv_bar(ovvv_full_b) if np.ndim(ovvv) == 3 else np.ascontiguousarray(ovvv_full_b))
     return (out, e.value) if return_energy else out
+
+
+class LnoPrepared:
+    """Host arrays of one LNO instance (built once) for timed calls on slices
+    of the LNO job list -- the CPU baseline times only the kernel."""
+
+    def __init__(self, mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv):
+        self.nvir, self.nocc = np.shape(t1T)
+        self.mat = np.ascontiguousarray(mat, dtype=np.float64)
+        (self.t1T, self.t2T, self.me, self.fvo, self.vooo,
+         self.vvop) = lno_host_arrays(t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv)
+        self.bars = None
+
+    def njobs(self):
+        return lno_njobs(self.nvir)
+
+    def energy(self, j0, j1):
+        return _load().oracle_lno_ccsd_t_energy_range(
+            self.nocc, self.nvir, _ptr(self.mat), _ptr(self.me), _ptr(self.t1T), _ptr(self.t2T),
+            _ptr(self.vooo), _ptr(self.fvo), _ptr(self.vvop), j0, j1)
+
+    def energy_vjp(self, j0, j1, et_bar=1.0):
+        if self.bars is None:
+            self.bars = [np.zeros_like(x) for x in (self.mat, self.me, self.t1T, self.t2T, self.vooo,
+                                                    self.fvo, self.vvop)]
+        e = ctypes.c_double(0.0)
+        _load().oracle_lno_ccsd_t_energy_vjp_range(
+            self.nocc, self.nvir, _ptr(self.mat), _ptr(self.me), _ptr(self.t1T), _ptr(self.t2T),
+            _ptr(self.vooo), _ptr(self.fvo), _ptr(self.vvop), float(et_bar), *[_ptr(b) for b in self.bars],
+            j0, j1, ctypes.byref(e))
+        return e.value
