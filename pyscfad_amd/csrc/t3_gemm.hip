@@ -27,7 +27,9 @@
 // LDS layout is always [row][k] with a 16-byte row stagger; row-contiguous
 // tiles are transposed in registers (each thread holds a VEC-row block).  Two
 // LDS buffers and two register stages, one barrier per chunk: the global
-// loads of chunk c+2 are in flight while chunk c is multiplied.
+// loads of chunk c+2 are in flight while chunk c is multiplied.  (bf16x6
+// tiles with a 128-row side keep one LDS buffer and two barriers per chunk,
+// for occupancy; see the bf16x6 branch.)
 #include <type_traits>
 
 #include "t3_common.h"
@@ -41,6 +43,9 @@ template <> struct PrecTraits<PREC_F64> { using T = double; using Acc = double; 
 template <> struct PrecTraits<PREC_F32> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 template <> struct PrecTraits<PREC_BF16X3> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
 template <> struct PrecTraits<PREC_BF16X6> { using T = float; using Acc = float; using V = f4v; static constexpr int BK = 32; static constexpr int VEC = 4; };
+
+// instruction classes of __builtin_amdgcn_sched_group_barrier
+constexpr int SG_VALU = 0x002, SG_MFMA = 0x008;
 
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
@@ -248,6 +253,64 @@ struct Stage {
     if (neg) store_split3_as<LDK, true>(hi, mid, lo);
     else store_split3_as<LDK, false>(hi, mid, lo);
   }
+  // The same split without branches (128- and 64-row tiles, NV = 4 or 2):
+  // the sign of the segment is an XOR of the sign bits, the load mode a
+  // per-element select between the row-contiguous (transposed) and the
+  // k-contiguous element sets and offsets.  Branch-free, the store shares the
+  // basic block of the MFMAs it is meant to be scheduled between.  With
+  // NV = 2 every mode writes four pairs of k per plane (4-byte stores).
+  template <int LDK>
+  __device__ __forceinline__ void store_split3_flat(__bf16* __restrict__ hi, __bf16* __restrict__ mid,
+                                                    __bf16* __restrict__ lo) const {
+    static_assert((NV == 4 || NV == 2) && VEC == 4, "the branch-free split store takes 2 or 4 4-vectors per thread");
+    const int t = threadIdx.x;
+    const bool rc = mode == LOAD_ROWCONT;
+    const uint32_t sg = neg ? 0x80000000u : 0u;
+    auto part = [](float& x0, float& x1) {
+      const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{x0, x1}, bf2v));
+      x0 -= __uint_as_float(h << 16);
+      x1 -= __uint_as_float(h & 0xffff0000u);
+      return h;
+    };
+    auto split2 = [&](float x0, float x1, uint32_t& h, uint32_t& m, uint32_t& l) {
+      x0 = __uint_as_float(__float_as_uint(x0) ^ sg);
+      x1 = __uint_as_float(__float_as_uint(x1) ^ sg);
+      h = part(x0, x1);
+      m = part(x0, x1);
+      l = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2v{x0, x1}, bf2v));
+    };
+    const int rb = t % RB, kb = t / RB;        // row-contiguous: row block, k block
+    const int kv = t % KV, row0 = t / KV;      // k-contiguous: k vector, first row
+    if constexpr (NV == 2) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        // row-contiguous: k pair (kb*2, +1) of row rb*4 + q; k-contiguous:
+        // k pair kv*4 + 2(q&1) of row row0 + (q>>1) RSTEP
+        const float x0 = rc ? r[0][q] : r[q >> 1][2 * (q & 1)];
+        const float x1 = rc ? r[1][q] : r[q >> 1][2 * (q & 1) + 1];
+        const int off = rc ? lds_off<LDK, 8, SWZ_BF16>(rb * VEC + q, kb * NV)
+                           : lds_off<LDK, 8, SWZ_BF16>(row0 + (q >> 1) * RSTEP, kv * VEC + 2 * (q & 1));
+        uint32_t h, m, l;
+        split2(x0, x1, h, m, l);
+        *reinterpret_cast<uint32_t*>(hi + off) = h;
+        *reinterpret_cast<uint32_t*>(mid + off) = m;
+        *reinterpret_cast<uint32_t*>(lo + off) = l;
+      }
+      return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const f4v x = rc ? f4v{r[0][q], r[1][q], r[2][q], r[3][q]} : r[q];
+      const int off = rc ? lds_off<LDK, 8, SWZ_BF16>(rb * VEC + q, kb * NV)
+                         : lds_off<LDK, 8, SWZ_BF16>(row0 + q * RSTEP, kv * VEC);
+      uint32_t h0, m0, l0, h1, m1, l1;
+      split2(x[0], x[1], h0, m0, l0);
+      split2(x[2], x[3], h1, m1, l1);
+      *reinterpret_cast<u2v*>(hi + off) = u2v{h0, h1};
+      *reinterpret_cast<u2v*>(mid + off) = u2v{m0, m1};
+      *reinterpret_cast<u2v*>(lo + off) = u2v{l0, l1};
+    }
+  }
   template <int LDK, bool NEG>
   __device__ __forceinline__ void store_split3_as(__bf16* __restrict__ hi, __bf16* __restrict__ mid,
                                                   __bf16* __restrict__ lo) const {
@@ -378,10 +441,10 @@ __device__ __forceinline__ void write_out(const GemmProb& P, const TileCoord& tc
 
 // BM x BN tile, 4 waves arranged WGM x (4/WGM).  Register budget: four waves
 // per SIMD for tiles up to 64x64 (four blocks per CU fit in LDS), two above;
-// bf16x6, whose three-part LDS tiles allow two blocks per CU at 64x64 and one
-// above, gets two and one.
+// bf16x6, whose three-part LDS tiles (double-buffered up to 64x64, single
+// above) fit two blocks per CU at every shape, gets two.
 template <int PREC, int BM, int BN> struct MinBlocks {
-  static constexpr int value = PREC == PREC_BF16X6 ? ((BM * BN <= 64 * 64) ? 2 : 1) : ((BM * BN <= 64 * 64) ? 4 : 2);
+  static constexpr int value = PREC == PREC_BF16X6 ? 2 : ((BM * BN <= 64 * 64) ? 4 : 2);
 };
 
 template <int PREC, int BM, int BN, int WGM>
@@ -431,8 +494,8 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
   // half of them past the closing barrier, overlapping them with the next
   // step's reads.
   // bf16x6 keeps two stages (chunk c+2 loaded while c is multiplied, c+1
-  // split and written after it): its three-way split is long VALU work that
-  // measured faster behind the MFMAs than in front of them.
+  // split and written between its MFMAs): its three-way split is long VALU
+  // work, which the scheduler interleaves with the MFMAs.
   constexpr int NSTAGE = PREC == PREC_BF16X6 ? 2 : 1;
   using SA = Stage<T, V, BM, BK, VEC>;
   using SB = Stage<T, V, BN, BK, VEC>;
@@ -566,8 +629,14 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     if (it < nch) step(it, std::integral_constant<int, 0>{});
   } else if constexpr (PREC == PREC_BF16X6) {
     constexpr int LDK = BK + 8;   // bf16 elements; 80-byte rows keep 16-byte reads aligned
-    __shared__ __attribute__((aligned(16))) __bf16 la[3][2][BM * LDK];   // [hi/mid/lo][buffer]
-    __shared__ __attribute__((aligned(16))) __bf16 lb[3][2][BN * LDK];
+    // Tiles whose double-buffered three-part planes would exceed 80 KB (a
+    // 128-row side: 92-123 KB, one block per CU) keep one LDS buffer, so that
+    // two or three blocks share a CU and hide each other's barrier and load
+    // waits; a step then reads its fragments, waits until every wave has, and
+    // writes the next chunk into the same buffer.
+    constexpr int NBUF = 3 * 2 * (BM + BN) * LDK * 2 > 80 * 1024 ? 1 : 2;
+    __shared__ __attribute__((aligned(16))) __bf16 la[3][NBUF][BM * LDK];   // [hi/mid/lo][buffer]
+    __shared__ __attribute__((aligned(16))) __bf16 lb[3][NBUF][BN * LDK];
     if (nch > 0) {
       load_next(sa[0], sb[0]);
       load_next(sa[NSTAGE - 1], sb[NSTAGE - 1]);
@@ -576,24 +645,33 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
     }
     __syncthreads();
     const FragOff<LDK, 8, SWZ_BF16> fa(wm0 + l16, g * 8), fb(wn0 + l16, g * 8);
+    // 128- and 64-row operand tiles take the branch-free split store
+    auto store3 = [&](auto& st, __bf16* h, __bf16* m, __bf16* l) {
+      constexpr int nv = std::remove_reference_t<decltype(st)>::NV;
+      if constexpr (nv == 4 || nv == 2) st.template store_split3_flat<LDK>(h, m, l);
+      else st.template store_split3<LDK>(h, m, l);
+    };
     auto step = [&](int it, auto par) {
       constexpr int p = decltype(par)::value;
+      constexpr int rbuf = NBUF == 2 ? p : 0, wbuf = NBUF == 2 ? p ^ 1 : 0;   // buffer read / written
       load_next(sa[p % NSTAGE], sb[p % NSTAGE]);
       bf8v ah[FM], am[FM], al[FM], bh[FN], bm[FN], bl[FN];
 #pragma unroll
       for (int i = 0; i < FM; i++) {
         const int o = fa(i);
-        ah[i] = *reinterpret_cast<const bf8v*>(&la[0][p][o]);
-        am[i] = *reinterpret_cast<const bf8v*>(&la[1][p][o]);
-        al[i] = *reinterpret_cast<const bf8v*>(&la[2][p][o]);
+        ah[i] = *reinterpret_cast<const bf8v*>(&la[0][rbuf][o]);
+        am[i] = *reinterpret_cast<const bf8v*>(&la[1][rbuf][o]);
+        al[i] = *reinterpret_cast<const bf8v*>(&la[2][rbuf][o]);
       }
 #pragma unroll
       for (int j = 0; j < FN; j++) {
         const int o = fb(j);
-        bh[j] = *reinterpret_cast<const bf8v*>(&lb[0][p][o]);
-        bm[j] = *reinterpret_cast<const bf8v*>(&lb[1][p][o]);
-        bl[j] = *reinterpret_cast<const bf8v*>(&lb[2][p][o]);
+        bh[j] = *reinterpret_cast<const bf8v*>(&lb[0][rbuf][o]);
+        bm[j] = *reinterpret_cast<const bf8v*>(&lb[1][rbuf][o]);
+        bl[j] = *reinterpret_cast<const bf8v*>(&lb[2][rbuf][o]);
       }
+      // one buffer: every wave's fragments are in registers before it is rewritten
+      if constexpr (NBUF == 1) __syncthreads();
       // smallest products first
 #pragma unroll
       for (int i = 0; i < FM; i++)
@@ -606,9 +684,17 @@ gemm_grouped(const GemmProb* __restrict__ probs, int nprob, const GemmSeg* __res
           acc[i][j] = mfma_bf16(ah[i], bm[j], acc[i][j]);
           acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         }
-      if (it + 1 < nch) {
-        sa[(p ^ 1) % NSTAGE].template store_split3<LDK>(la[0][p ^ 1], la[1][p ^ 1], la[2][p ^ 1]);
-        sb[(p ^ 1) % NSTAGE].template store_split3<LDK>(lb[0][p ^ 1], lb[1][p ^ 1], lb[2][p ^ 1]);
+      // The split of the next chunk is written unconditionally (the last
+      // step writes the buffer no later step reads; the walker reloads its
+      // last chunk), so it shares the MFMAs' basic block, and its VALU work
+      // is issued between the MFMAs, two instructions after each: they run
+      // in the shadow of the 16-cycle bf16 MFMAs instead of after them.
+      store3(sa[(p ^ 1) % NSTAGE], la[0][wbuf], la[1][wbuf], la[2][wbuf]);
+      store3(sb[(p ^ 1) % NSTAGE], lb[0][wbuf], lb[1][wbuf], lb[2][wbuf]);
+#pragma unroll
+      for (int q = 0; q < FM * FN * 6; q++) {
+        __builtin_amdgcn_sched_group_barrier(SG_MFMA, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(SG_VALU, 2, 0);
       }
       __syncthreads();
     };

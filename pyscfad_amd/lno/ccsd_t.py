@@ -108,22 +108,31 @@ def _ccsd_t_energy_bwd(max_memory, res, et_bar, precision=DEFAULT_PRECISION):
 
 
 def _make_ccsd_t_energy():
+    """_ccsd_t_energy(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv,
+    max_memory=None, precision='f64'): a jax.custom_vjp whose non-differentiable
+    arguments are max_memory (as in lno/ccsd_t.py:70) and precision, or, without
+    JAX, the energy as a float."""
     try:
         from functools import partial
         from jax import custom_vjp
     except ImportError:
         def _ccsd_t_energy(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory=None,
                            precision=DEFAULT_PRECISION):
-            """The LNO (T) energy as a float (JAX is not importable)."""
             return _ccsd_t_energy_fwd(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory, precision)[0]
         return _ccsd_t_energy
 
-    @partial(custom_vjp, nondiff_argnums=(8,))
-    def _ccsd_t_energy(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory=None):
-        return _ccsd_t_energy_fwd(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory)[0]
+    @partial(custom_vjp, nondiff_argnums=(8, 9))
+    def _ccsd_t_energy(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory=None,
+                       precision=DEFAULT_PRECISION):
+        return _ccsd_t_energy_fwd(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory, precision)[0]
 
-    _ccsd_t_energy.defvjp(lambda *a: _ccsd_t_energy_fwd(*a[:8], a[8] if len(a) > 8 else None),
-                          lambda max_memory, res, et_bar: _ccsd_t_energy_bwd(max_memory, res, et_bar))
+    def _fwd(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory, precision):
+        return _ccsd_t_energy_fwd(mat, t1T, t2T, mo_energy, fvo, ovoo, ovov, ovvv, max_memory, precision)
+
+    def _bwd(max_memory, precision, res, et_bar):
+        return _ccsd_t_energy_bwd(max_memory, res, et_bar, precision)
+
+    _ccsd_t_energy.defvjp(_fwd, _bwd)
     return _ccsd_t_energy
 
 
@@ -145,11 +154,8 @@ def kernel(mycc, eris, ulo, t1=None, t2=None, verbose=None, precision=DEFAULT_PR
     t1T = t1.T
     t2T = t2.transpose(2, 3, 1, 0) if not isinstance(t2, torch.Tensor) else t2.permute(2, 3, 1, 0)
     fvo = eris.fock[nocc:, :nocc]
-    max_memory = getattr(mycc, "max_memory", None)
-    if precision != DEFAULT_PRECISION:
-        return float(energy(mat, t1T, t2T, eris.mo_energy, fvo, eris.ovoo, eris.ovov, eris.ovvv,
-                            precision=precision))
-    return _ccsd_t_energy(mat, t1T, t2T, eris.mo_energy, fvo, eris.ovoo, eris.ovov, eris.ovvv, max_memory)
+    return _ccsd_t_energy(mat, t1T, t2T, eris.mo_energy, fvo, eris.ovoo, eris.ovov, eris.ovvv,
+                          getattr(mycc, "max_memory", None), precision)
 
 
 class LNOCCSDTEnergy(torch.autograd.Function):
