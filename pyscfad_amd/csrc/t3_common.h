@@ -99,6 +99,7 @@ struct EpiArgs {
   const double* eps_vir;   // (V), padded entries large and finite
   const double* t1;        // (o, v) reference layout
   const double* fvoP;      // (o, V): fvoP[r][z] = fock[o+z, r], zero padded
+  const int* cube_tab;     // (ncube): tile triple A | B << 10 | C << 20 of each cube id
 };
 
 // Per-batch reduction of the epilogue partials (one block, fixed order):

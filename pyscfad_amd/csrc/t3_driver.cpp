@@ -466,6 +466,7 @@ struct Ctx {
   Blob blob;
   std::vector<Launch> launches;
   std::vector<EpiArgs> epi_args;
+  size_t cube_tab = (size_t)-1;   // blob offset of the epilogue's cube table (one per plan)
   std::vector<EpiReduceArgs> epired_args;
   std::string plan_error;   // set when a contraction does not fit the GEMM's operand modes
   int force_cfg[FAM_COUNT] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};   // autotuner trials
@@ -950,6 +951,15 @@ struct Ctx {
     M.n_a = (int)vt.size();
     M.aux = bw ? 1 : 0;
     launches.push_back(M);
+    if (cube_tab == (size_t)-1) {
+      // tile triple of every cube id, in the order cube = A(A+1)(A+2)/6 + B(B+1)/2 + C, A >= B >= C
+      std::vector<int> tab;
+      tab.reserve(d.ncube);
+      for (int A = 0; A < d.ntile; A++)
+        for (int B = 0; B <= A; B++)
+          for (int C = 0; C <= B; C++) tab.push_back(A | B << 10 | C << 20);
+      cube_tab = blob.put(tab);
+    }
     EpiArgs a{};
     a.nocc = d.o; a.nvir = d.v; a.V = d.V; a.ntile = d.ntile; a.ncube = d.ncube;
     a.backward = bw; a.lno = d.lno; a.et_bar = et_bar;
@@ -1300,10 +1310,12 @@ struct Ctx {
         case L_ZERO:
           e = t3_zero(ws + L.off_a, (size_t)L.work, st);
           break;
-        case L_EPI:
-          e = t3_launch_epilogue(d.tp_is_f64, &epi_args[L.aux], reinterpret_cast<const EpiTriple*>(db + L.off_a),
-                                 L.n_a, st);
+        case L_EPI: {
+          EpiArgs a = epi_args[L.aux];
+          a.cube_tab = reinterpret_cast<const int*>(db + cube_tab);
+          e = t3_launch_epilogue(d.tp_is_f64, &a, reinterpret_cast<const EpiTriple*>(db + L.off_a), L.n_a, st);
           break;
+        }
         case L_EPIRED: {
           EpiReduceArgs a = epired_args[L.aux];
           a.triples = reinterpret_cast<const EpiTriple*>(db + L.off_a);

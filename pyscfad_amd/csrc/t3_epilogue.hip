@@ -59,15 +59,6 @@ __host__ __device__ constexpr int perm_index(int p0, int p1) {
   return 2 * p0 + (p1 > 3 - p0 - p1 ? 1 : 0);
 }
 
-__device__ __forceinline__ void decode_cube(int m, int& A, int& B, int& C) {
-  int a = 0;
-  while ((a + 1) * (a + 2) * (a + 3) / 6 <= m) a++;
-  m -= a * (a + 1) * (a + 2) / 6;
-  int b = 0;
-  while ((b + 1) * (b + 2) / 2 <= m) b++;
-  m -= b * (b + 1) / 2;
-  A = a; B = b; C = m;
-}
 
 // Sum over the 64 lanes of a wave with a fixed butterfly (every lane gets the
 // same, schedule-independent result).
@@ -126,8 +117,10 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     const int nb = gridDim.x, q = nb / 8;
     if (cube < 8 * q) cube = (cube % 8) * q + cube / 8;
   }
-  int TA, TB, TC;
-  decode_cube(cube, TA, TB, TC);
+  // tile triple of the cube from the host-built table (a scalar load; the
+  // closed-form decode is a serial scalar search)
+  const int tcode = args.cube_tab[cube];
+  const int TA = tcode & 1023, TB = (tcode >> 10) & 1023, TC = tcode >> 20;
   const int V = args.V, v = args.nvir;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
@@ -140,18 +133,16 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
 
   // --- the six permuted sub-cubes of w0, 16-byte loads along the rows ---
   const TP* w0 = reinterpret_cast<const TP*>(tr.w0);
-  for (int e = tid; e < 6 * T3 / VW; e += 256) {
+  static_assert((6 * T3 / VW) % 256 == 0, "whole load rounds");
+#pragma unroll
+  for (int it = 0; it < 6 * T3 / VW / 256; it++) {
+    const int e = tid + 256 * it;
     const int p = e / (T3 / VW), r = (e % (T3 / VW)) * VW;
     const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
-    int q0, q1, q2;
-    switch (p) {
-      case 0: q0 = TA; q1 = TB; q2 = TC; break;
-      case 1: q0 = TA; q1 = TC; q2 = TB; break;
-      case 2: q0 = TB; q1 = TA; q2 = TC; break;
-      case 3: q0 = TB; q1 = TC; q2 = TA; break;
-      case 4: q0 = TC; q1 = TA; q2 = TB; break;
-      default: q0 = TC; q1 = TB; q2 = TA; break;
-    }
+    // tiles of sub-cube p (c_perm order), as selects: p is a lane value
+    const int q0 = p < 2 ? TA : (p < 4 ? TB : TC);
+    const int q1 = (p == 0 || p == 5) ? TB : ((p == 1 || p == 3) ? TC : TA);
+    const int q2 = (p == 1 || p == 4) ? TB : ((p == 3 || p == 5) ? TA : TC);
     const long long off = (((long long)q0 * T + x0) * V + (q1 * T + x1)) * V + (q2 * T + x2);
     const tpv x = *reinterpret_cast<const tpv*>(&w0[off]);
     *reinterpret_cast<tpv*>(&sc[p][sub_pos(x0, x1, x2 ^ (x0 & (VW - 1)))]) = xor_perm<tpv, VW>(x, x0 & (VW - 1));
@@ -163,18 +154,26 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   // --- CM_d / CN_d on the ordered tile pairs: block [d][pair mu][which][8x8] ---
   {
     const long long VV = (long long)V * V;
-    for (int e2 = tid; e2 < NCM / 2; e2 += 256) {
+#pragma unroll
+    for (int it = 0; it < (NCM / 2 + 255) / 256; it++) {
+      const int e2 = tid + 256 * it;
+      if (e2 >= NCM / 2) break;
       const int e = 2 * e2;
       const int xy = e % (T * T);
       const int which = (e / (T * T)) % 2;
       const int mu = (e / (2 * T * T)) % 6;
       const int d = e / (12 * T * T);
-      const int x = sel3(c_perm[mu][0], TA, TB, TC) * T + xy / T;
-      const int y = sel3(c_perm[mu][1], TA, TB, TC) * T + xy % T;
+      // first two tiles of the ordered pair mu (c_perm order), as selects
+      const int u0 = mu < 2 ? TA : (mu < 4 ? TB : TC);
+      const int u1 = (mu == 0 || mu == 5) ? TB : ((mu == 1 || mu == 3) ? TC : TA);
+      const int x = u0 * T + xy / T;
+      const int y = u1 * T + xy % T;
       *reinterpret_cast<dv2*>(&cmh[e]) = *reinterpret_cast<const dv2*>(&tr.cm[(2 * d + which) * VV + (long long)x * V + y]);
     }
   }
-  for (int e = tid; e < 2 * 3 * 3 * T; e += 256) {
+  static_assert(2 * 3 * 3 * T <= 256, "one round");
+  if (tid < 2 * 3 * 3 * T) {
+    const int e = tid;
     const int which = e / (9 * T), r = e % (9 * T);
     const int d = r / (3 * T), tp = (r / T) % 3, z = sel3(tp, TA, TB, TC) * T + r % T;
     const int td = sel3(d, tr.i, tr.j, tr.k);
@@ -311,7 +310,9 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     const long long b0 = (long long)tt[c_perm[p][0]] * T, b1 = (long long)tt[c_perm[p][1]] * T,
                     b2 = (long long)tt[c_perm[p][2]] * T;
     if (distinct) {   // every sub-cube is a different global block and all its points are canonical
-      for (int e = tid; e < T3 / VW; e += 256) {
+      static_assert(T3 / VW <= 256, "one round");
+      if (tid < T3 / VW) {
+        const int e = tid;
         const int r = e * VW;
         const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
         const long long off = ((b0 + x0) * V + (b1 + x1)) * V + (b2 + x2);
@@ -321,7 +322,10 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       }
       continue;
     }
-    for (int r = tid; r < T3; r += 256) {
+    static_assert(T3 % 256 == 0, "whole rounds");
+#pragma unroll
+    for (int rr = 0; rr < T3 / 256; rr++) {
+      const int r = tid + 256 * rr;
       const int x[3] = {r / (T * T), (r / T) % T, r % T};
       bool write = false, owned = false;
 #pragma unroll
