@@ -16,6 +16,10 @@ metric   : TFLOP/s of energy+VJP credited as 3 * F_E of the triples processed,
 value    : that rate for the --precision path (default f32 MFMA)
 f64      : the FP64 path timed on the same batches in the same process, and
            the ratio of the two rates
+bf16x6   : (f32 runs) the bf16x6 path on the same batches: FP32 operands
+           split into three bf16 parts, six products per K-step on the bf16
+           matrix cores, FP32 accumulation; its rate, GEMM roofline against
+           the bf16 peak over six products, speedup over FP64 and accuracy
 roofline : the grouped MFMA GEMM kernel, against the MI355X dense matrix peak
            of the precision used; achieved = the credited 3 F_E over the
            kernel's device time (hipEvents around every launch on the call's
@@ -52,6 +56,9 @@ def parse():
                     help="first triple of the timed batches (the warm-up batches precede it)")
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "bf16x3", "bf16x6"])
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 accuracy reference run")
+    ap.add_argument("--no-bf16x6", action="store_true",
+                    help="skip the bf16x6 leg (FP32-accurate products on the bf16 matrix cores) that an "
+                         "f32 run times on the same batches")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--triples", default=None,
@@ -425,6 +432,38 @@ def main():
     traffic, traffic_src = measured_traffic(args.precision, o, v, None if whole else timed[0][1] - timed[0][0])
     full = flops_energy(o, v)
 
+    # bf16x6 on the same batches: three bf16 parts per FP32 operand, six
+    # products, FP32 accumulation (FP32-level accuracy on the bf16 matrix cores)
+    b6 = None
+    bars6 = None
+    if args.precision == "f32" and not args.no_bf16x6 and not args.df:
+        if bars is not None and o * v ** 3 * 8 > 8e9:
+            bars = [b.cpu() for b in bars]
+        wsp.clear()
+        torch.cuda.empty_cache()
+        n6 = len(timed) if not whole else min(len(timed), 2)
+        if not args.no_autotune:
+            log("[bench] autotuning tile shapes (bf16x6)")
+            ccsd_t.autotune(o, v, "bf16x6")
+        log("[bench] bf16x6 warm-up step")
+        step("bf16x6", warm[0] if warm else timed[0])
+        _, bars_w, tf6, tb6 = step("bf16x6", warm[-1] if warm else timed[0], timing=True)
+        del bars_w
+        dt6, per6, e6, bars6 = timed_run("bf16x6", timed[-n6:], "bf16x6")
+        if bars6 is not None and o * v ** 3 * 8 > 8e9:
+            bars6 = [b.cpu() for b in bars6]
+        fe6_w = fe_of(warm[-1] if warm else timed[0])
+        gemm6 = tf6["gemm"][0] + tb6["gemm"][0]
+        ach6 = credit * fe6_w / (gemm6 / 1e3) / 1e12
+        b6 = {"tflops": credit * sum(fe_of(sl) for sl in timed[-n6:]) / sum(per6) / 1e12,
+              "ms_per_step": dt6 * 1e3, "steps": n6, "batches": f"the last {n6} of the timed batches",
+              "gemm_ms_per_step": gemm6,
+              "roofline": {"achieved": ach6, "peak": MI355X_PEAK_TFLOPS["bf16x6"], "unit": "TFLOP/s",
+                           "frac": ach6 / MI355X_PEAK_TFLOPS["bf16x6"],
+                           "peak_note": "the dense bf16 matrix peak over the six products"}}
+        wsp.clear()
+        torch.cuda.empty_cache()
+
     f64 = None
     accuracy = None
     if compare:
@@ -441,6 +480,10 @@ def main():
         dt64, per64, e64, bars64 = timed_run("f64", timed[-n64:], "FP64")
         t32_same = sum(per32[-n64:])
         t64_same = sum(per64)
+        if b6 is not None:
+            b6["speedup_over_f64"] = sum(per64[-b6["steps"]:]) / sum(per6)
+            b6["accuracy_vs_f64"] = {"batch": list(timed[-1]), "dE_Eh": abs(float(e6) - float(e64)),
+                                     "dE_rel": abs(float(e6) - float(e64)) / max(abs(float(e64)), 1e-300)}
         f64 = {"tflops": credit * sum(fe_of(sl) for sl in timed[-n64:]) / t64_same / 1e12,
                "ms_per_step": dt64 * 1e3, "steps": n64,
                "batches": f"the last {n64} of the timed batches",
@@ -455,6 +498,10 @@ def main():
                 rel[n], elem[n] = max_rel_chunked(b, r)
             accuracy["max_rel"] = rel
             accuracy["max_rel_elementwise"] = elem
+            if b6 is not None and bars6 is not None:
+                b6["accuracy_vs_f64"]["max_rel"] = {n: max_rel_chunked(b, r)[0]
+                                                    for n, b, r in zip(names, bars6, bars64)}
+                bars6 = None
         del last
 
     cpu = None
@@ -494,6 +541,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl, "precision": args.precision, "E_T_last_step": float(e)},
         "f64": f64,
+        "bf16x6": b6,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic, "traffic_unit": "bytes per launch",
                      "traffic_source": traffic_src,
@@ -750,6 +798,7 @@ def main_lno(args):
                                f"1/sqrt(n), Gram projector of {o // 2 + 1} local orbitals; whole energy+VJP "
                                f"solve per step", "precision": args.precision, "E_last_step": float(e)},
         "f64": f64,
+        "bf16x6": b6,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "kernel": "gemm_grouped",
                      "credited": "9 F_E of the first warm-up solve over its gemm_grouped device time (the "
