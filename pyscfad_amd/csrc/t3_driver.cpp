@@ -269,9 +269,18 @@ int choose_batch(const Dims& d) {
 // consecutive batches runs as one launch, whose problems merge the triples
 // that share an output slab, so the FP64 slab is read and written once per
 // group instead of once per batch.  Their G cubes stay resident meanwhile;
-// the extra cubes are held to about 512 MB.
+// the extra cubes are held to about 512 MB, so that they stay in the
+// last-level cache.  A batch whose cubes alone exceed that is not cache
+// resident anyway: it takes groups of 4 (at o=64/v=512 FP32 the ovvv adjoint
+// went from 475 to 446 ms per 64 triples).  PYSCFAD_AMD_B1_GROUP (1..8)
+// overrides it, for experiments.
 int b1_group(const Dims& d, int B) {
+  if (const char* env = std::getenv("PYSCFAD_AMD_B1_GROUP")) {   // experiments: 1..8
+    const long long x = std::atoll(env);
+    if (x >= 1 && x <= 8) return (int)x;
+  }
   const double cube = (double)d.V * d.V * d.V * (double)d.elt;
+  if (cube * B > 512.0e6) return 4;
   const long long g = (long long)(512.0e6 / (cube * B));
   return (int)std::max<long long>(1, std::min<long long>(g, 4));
 }
