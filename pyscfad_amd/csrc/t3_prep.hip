@@ -469,20 +469,21 @@ __global__ void __launch_bounds__(256) fin_ovvv(double* __restrict__ out, const 
     const int x = (int)(slab % v);
     const long long p = slab / v;
     const double* a = acc + (p * V + x) * V * V;
-    // tile[yi][fi] = acc[p][x][Y0+yi][F0+fi]
+    // tile[yi][fi] = acc[p][x][Y0+yi][F0+fi]; packed: the other triangle's
+    // element of each output pair is read in the same round (one round trip)
+    double lower[4];
     for (int q = 0; q < 4; q++) {
       const int y = Y0 + ty + 8 * q, f = F0 + tx;
       tile[ty + 8 * q][tx] = (y < v && f < v) ? a[(long long)y * V + f] : 0.0;
+      const int fo = F0 + ty + 8 * q, yo = Y0 + tx;
+      lower[q] = (packed && fo < v && yo < fo) ? a[(long long)fo * V + yo] : 0.0;
     }
     __syncthreads();
     if (packed) {
       double* dst = out + slab * npair;
       for (int q = 0; q < 4; q++) {
         const int f = F0 + ty + 8 * q, y = Y0 + tx;
-        if (f < v && y <= f) {
-          const double t = tile[tx][ty + 8 * q];
-          dst[(long long)f * (f + 1) / 2 + y] = f == y ? t : t + a[(long long)f * V + y];
-        }
+        if (f < v && y <= f) dst[(long long)f * (f + 1) / 2 + y] = tile[tx][ty + 8 * q] + lower[q];
       }
     } else {
       double* dst = out + slab * v * v;
