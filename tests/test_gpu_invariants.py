@@ -70,3 +70,23 @@ def test_triple_slices_partition(gpu_lib):
     assert abs(e_sum - float(e_full)) <= 1e-12 * abs(float(e_full))
     for x, y in zip(acc, full):
         assert torch.allclose(x, y, rtol=1e-11, atol=1e-14 * float(torch.max(torch.abs(y))))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_ovvv_adjoint_grouping_changes_only_summation_order(gpu_lib, monkeypatch, prec, tol):
+    """The ovvv adjoint of several consecutive batches runs as one launch that
+    merges the triples sharing an output slab (PYSCFAD_AMD_B1_GROUP sets how
+    many): every grouping gives the same E(T) and cotangents up to summation
+    order."""
+    d = make_instance(12, 100, 21, fvo_scale=0.02)
+    monkeypatch.setenv("PYSCFAD_AMD_BATCH", "2")
+    res = {}
+    for g in ("1", "4", "8"):
+        monkeypatch.setenv("PYSCFAD_AMD_B1_GROUP", g)
+        res[g] = _run(d, prec)
+    e1, b1 = res["1"]
+    for g in ("4", "8"):
+        e, b = res[g]
+        assert abs(e - e1) <= 1e-12 * abs(e1), (g, e, e1)
+        for x, y in zip(b, b1):
+            assert np.abs(x - y).max() <= tol * np.abs(y).max(), (g, prec)

@@ -38,12 +38,15 @@ def checker(tmp_path_factory):
     shutil.rmtree(d, ignore_errors=True)
 
 
-def _run(exe, *args, block=None, tile=None, batch=None):
+def _run(exe, *args, block=None, tile=None, batch=None, b1_group=None):
     env = dict(os.environ)
     env.pop("PYSCFAD_AMD_BATCH", None)
     env.pop("PYSCFAD_AMD_TILE", None)
+    env.pop("PYSCFAD_AMD_B1_GROUP", None)
     if batch is not None:
         env["PYSCFAD_AMD_BATCH"] = str(batch)
+    if b1_group is not None:
+        env["PYSCFAD_AMD_B1_GROUP"] = str(b1_group)
     if block is not None:
         env["PYSCFAD_AMD_DF_BLOCK"] = str(block)
     if tile is not None:
@@ -103,3 +106,13 @@ def test_plan_in_bounds_forced_tile(checker, prec, tile):
     assert "bad=0" in r.stdout and "plan_error=''" in r.stdout, r.stdout
     if not (prec == 0 and tile == 1):
         assert f"cfgs={tile},\n" in r.stdout, r.stdout   # every GEMM launch on the forced tile
+
+
+@pytest.mark.parametrize("o,v,prec,batch,group", [(12, 100, 1, 2, 8), (12, 100, 0, 3, 5), (7, 33, 3, 4, 8),
+                                                  (64, 512, 1, 2, 4)])
+def test_plan_in_bounds_ovvv_adjoint_groups(checker, o, v, prec, batch, group):
+    """Several batches per ovvv-adjoint launch: the G cubes of the whole
+    group stay resident, and the merged problems' accesses stay in bounds."""
+    r = _run(checker, o, v, prec, 0, batch=batch, b1_group=group)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout and "plan_error=''" in r.stdout, r.stdout
