@@ -65,6 +65,12 @@ def parse():
                     help="BEGIN:END slice of the unique occupied triples (a bounded sample of a large "
                          "instance, or a profiling run); default: all")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--progress", action="store_true",
+                    help="wait for each timed step and log it (long steps, e.g. a whole solve in slices); "
+                         "the host's per-call planning is then no longer hidden behind the previous step")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the extra untimed step that carries the per-kernel timing when there is no "
+                         "warm-up step (the roofline fields are then left empty)")
     ap.add_argument("--energy-only", action="store_true",
                     help="time the forward E(T) alone (F_E credited), for instances whose VJP "
                          "workspace does not fit")
@@ -360,6 +366,8 @@ def main():
         _lib.set_timing(False)
         return e, bars, t_fwd, t_bwd
 
+    e_sums = {}   # E(T) summed over the timed steps of each leg (disjoint batches: their total)
+
     def timed_run(prec, sls, label):
         """Time the steps over the batches sls: (seconds per step, per-step
         seconds, last energy, last cotangents)."""
@@ -369,13 +377,20 @@ def main():
         c0 = time.perf_counter()
         evs[0].record()
         e = bars = None
+        es = []
         for x, sl in enumerate(sls):
             bars = None
             e, bars, _, _ = step(prec, sl)
+            es.append(e)
             evs[x + 1].record()
+            if args.progress:
+                evs[x + 1].synchronize()
+                log(f"[bench] {label} step {x + 1}/{len(sls)} (triples [{sl[0]},{sl[1]})) done at "
+                    f"{time.perf_counter() - c0:.1f} s")
         torch.cuda.synchronize()
         dt = (time.perf_counter() - c0) / len(sls)
         per = [evs[x].elapsed_time(evs[x + 1]) / 1e3 for x in range(len(sls))]
+        e_sums[label] = sum(float(x) for x in es)
         return dt, per, e, bars
 
     # tile shapes from the autotuner (outside the timed region)
@@ -401,7 +416,7 @@ def main():
     peak_hbm = torch.cuda.max_memory_allocated() / 1e9
     credit = 1.0 if args.energy_only else 3.0    # forward only: F_E; energy + VJP: 3 F_E
     tflops = credit * fe / (dt * K) / 1e12
-    if fe_w is None:
+    if fe_w is None and not args.no_kernel_timing:
         # no warm-up step carried the kernel timing: one more (untimed) step
         # does, after the last timed step's cotangents leave the device
         if bars is not None:
@@ -411,13 +426,16 @@ def main():
         del bars_t
         fam = {k: (tf[k][0] + tb[k][0], tf[k][1] + tb[k][1]) for k in tf}
         fe_w = fe_of(timed[0])
+    kernel_timed = fe_w is not None
+    if not kernel_timed:   # --no-kernel-timing without a warm-up step: no roofline fields
+        fam, fe_w = {"gemm": (0.0, 0)}, 0.0
     gemm_ms = fam["gemm"][0]
     peak = MI355X_PEAK_TFLOPS[args.precision]
     # density-fitted mode: the FP64 slab builds and folds run through the same
     # GEMM kernel but are not part of the credited contractions; their family
     # time (which also holds their clears and reductions) is taken out
     df_gemm_ms = fam.get("df", (0.0, 0))[0]
-    achieved = credit * fe_w / ((gemm_ms - df_gemm_ms) / 1e3) / 1e12
+    achieved = credit * fe_w / ((gemm_ms - df_gemm_ms) / 1e3) / 1e12 if kernel_timed else None
     # algorithmic flops of each contraction of the timed batch: W build
     # (forward + recompute) 2 F_E, of which F_E is credited; particle-term
     # adjoints (ovvv, t2) F_E v/(v+o) each; hole-term adjoints (ovoo, t2)
@@ -539,11 +557,13 @@ def main():
                   "bf16x3": "bf16x3 (FP32 accumulate, FP64 denominators and sums)",
                   "bf16x6": "bf16x6 (three-part bf16 split, FP32 accumulate, FP64 denominators and sums)"}[args.precision],
         "data": "synthetic",
-        "config": {"workload": wl, "precision": args.precision, "E_T_last_step": float(e)},
+        "config": {"workload": wl, "precision": args.precision, "E_T_last_step": float(e),
+                   "E_T_sum_timed": e_sums.get("timed")},
         "f64": f64,
         "bf16x6": b6,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic, "traffic_unit": "bytes per launch",
+                     "frac": achieved / peak if achieved is not None else None,
+                     "traffic": traffic, "traffic_unit": "bytes per launch",
                      "traffic_source": traffic_src,
                      "kernel": "gemm_grouped",
                      "credited": f"{credit:.0f} F_E of the first warm-up batch over its gemm_grouped device time "
