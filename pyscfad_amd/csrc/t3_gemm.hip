@@ -21,15 +21,15 @@
 //
 // Staging: a K-chunk of each operand tile is read from HBM with 16-byte
 // global loads into registers (rows past the edge are clamped to the last
-// valid row, so the loads carry no branches), and written to LDS after the
-// MFMAs of the previous chunk, negated for segments of scale -1 (the clamped
-// rows are left as loaded: they reach only outputs that are not written).  The
-// LDS layout is always [row][k] with a 16-byte row stagger; row-contiguous
-// tiles are transposed in registers (each thread holds a VEC-row block).  Two
-// LDS buffers and two register stages, one barrier per chunk: the global
-// loads of chunk c+2 are in flight while chunk c is multiplied.  (bf16x6
-// tiles with a 128-row side keep one LDS buffer and two barriers per chunk,
-// for occupancy; see the bf16x6 branch.)
+// valid row, so the loads carry no branches), and written to LDS during the
+// multiply phase of the previous chunk, negated for segments of scale -1 (the
+// clamped rows are left as loaded: they reach only outputs that are not
+// written).  The LDS layout is always [row][k] in 16-byte units permuted per
+// row (lds_off); row-contiguous tiles are transposed in registers (each thread
+// holds a VEC-row block).  Two LDS buffers and one barrier per chunk; the
+// global loads of chunk c+2 are in flight while chunk c is multiplied (one
+// register stage; bf16x6 keeps two, and its tiles with a 128-row side keep one
+// LDS buffer and two barriers per chunk, for occupancy).
 #include <type_traits>
 
 #include "t3_common.h"
