@@ -271,16 +271,17 @@ int choose_batch(const Dims& d) {
 // group instead of once per batch.  Their G cubes stay resident meanwhile;
 // the extra cubes are held to about 512 MB, so that they stay in the
 // last-level cache.  A batch whose cubes alone exceed that is not cache
-// resident anyway: it takes groups of 4 (at o=64/v=512 FP32 the ovvv adjoint
-// went from 475 to 446 ms per 64 triples).  PYSCFAD_AMD_B1_GROUP (1..8)
-// overrides it, for experiments.
+// resident anyway: it takes groups of up to 4, with the group's cubes held to
+// about 4.3 GB (at o=64/v=512 FP32, four batches of two triples: the ovvv
+// adjoint went from 475 to 446 ms per 64 triples; FP64 takes two).
+// PYSCFAD_AMD_B1_GROUP (1..8) overrides it, for experiments.
 int b1_group(const Dims& d, int B) {
   if (const char* env = std::getenv("PYSCFAD_AMD_B1_GROUP")) {   // experiments: 1..8
     const long long x = std::atoll(env);
     if (x >= 1 && x <= 8) return (int)x;
   }
   const double cube = (double)d.V * d.V * d.V * (double)d.elt;
-  if (cube * B > 512.0e6) return 4;
+  if (cube * B > 512.0e6) return (int)std::max<long long>(1, std::min<long long>(4, (long long)(4.3e9 / (cube * B))));
   const long long g = (long long)(512.0e6 / (cube * B));
   return (int)std::max<long long>(1, std::min<long long>(g, 4));
 }

@@ -97,11 +97,16 @@ __device__ __forceinline__ VT xor_perm(VT v, int k) {
   }
 }
 
+// 512 threads per block, one point of the 8^3 tile per thread: the kernel is
+// bound by the latency of its operand loads, and with the LDS of five blocks
+// per CU this keeps 32 waves per CU in flight instead of 20.
+constexpr int EPI_THREADS = 512, EPI_WAVES = EPI_THREADS / 64;
+
 template <class TP, bool BW, bool LNO>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(EPI_THREADS)
 orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
-  constexpr int T = EPI_T, T3 = T * T * T;
-  static_assert(T == 8 && T3 == 512, "two points per thread, lane bits = point bits");
+  constexpr int T = EPI_T, T3 = T * T * T, NT = EPI_THREADS, NW = EPI_WAVES;
+  static_assert(T == 8 && T3 == NT, "one point per thread, wave = first index, lane bits = the other two");
   constexpr int NCM = 3 * 6 * 2 * T * T;                   // CM/CN blocks
   constexpr int NH = (6 * T3 * (int)sizeof(TP) + 7) / 8;   // H sub-cubes, in doubles
   constexpr int NU = NCM > NH ? NCM : NH;
@@ -129,14 +134,14 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   __shared__ __attribute__((aligned(16))) TP ss[LNO ? 6 : 1][LNO ? T3 : 1];
   __shared__ __attribute__((aligned(16))) double cmh[NU]; // CM/CN tile-pair blocks [d][pair][M/N][8x8], later H
   __shared__ double vec[2][3][3][T];                      // [t1/fvo][d][tile pos][8]
-  __shared__ double wred[4][20];                          // per-wave partial sums
+  __shared__ double wred[NW][20];                         // per-wave partial sums
 
   // --- the six permuted sub-cubes of w0, 16-byte loads along the rows ---
   const TP* w0 = reinterpret_cast<const TP*>(tr.w0);
-  static_assert((6 * T3 / VW) % 256 == 0, "whole load rounds");
 #pragma unroll
-  for (int it = 0; it < 6 * T3 / VW / 256; it++) {
-    const int e = tid + 256 * it;
+  for (int it = 0; it < (6 * T3 / VW + NT - 1) / NT; it++) {
+    const int e = tid + NT * it;
+    if (e >= 6 * T3 / VW) break;
     const int p = e / (T3 / VW), r = (e % (T3 / VW)) * VW;
     const int x0 = r / (T * T), x1 = (r / T) % T, x2 = r % T;
     // tiles of sub-cube p (c_perm order), as selects: p is a lane value
@@ -155,8 +160,8 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   {
     const long long VV = (long long)V * V;
 #pragma unroll
-    for (int it = 0; it < (NCM / 2 + 255) / 256; it++) {
-      const int e2 = tid + 256 * it;
+    for (int it = 0; it < (NCM / 2 + NT - 1) / NT; it++) {
+      const int e2 = tid + NT * it;
       if (e2 >= NCM / 2) break;
       const int e = 2 * e2;
       const int xy = e % (T * T);
@@ -171,7 +176,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       *reinterpret_cast<dv2*>(&cmh[e]) = *reinterpret_cast<const dv2*>(&tr.cm[(2 * d + which) * VV + (long long)x * V + y]);
     }
   }
-  static_assert(2 * 3 * 3 * T <= 256, "one round");
+  static_assert(2 * 3 * 3 * T <= NT, "one round");
   if (tid < 2 * 3 * 3 * T) {
     const int e = tid;
     const int which = e / (9 * T), r = e % (9 * T);
@@ -188,12 +193,12 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   const double s_bw = (LNO ? 2.0 / 3.0 : 2.0) * tr.fo * args.et_bar;
   const bool eqAB = (TA == TB), eqBC = (TB == TC);
 
-  // two points per thread: pt = tid and tid + 256, i.e. al0 = wave (+4), al1 = lane bits 3-5,
-  // al2 = lane bits 0-2
-  double gval[2][6], hval[2][6], qpt[2], rpt[2];
+  // one point per thread: pt = tid, i.e. al0 = wave, al1 = lane bits 3-5, al2 = lane bits 0-2
+  constexpr int NPT = T3 / NT;
+  double gval[NPT][6], hval[NPT][6], qpt[NPT], rpt[NPT];
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int pt = tid + 256 * h;
+  for (int h = 0; h < NPT; h++) {
+    const int pt = tid + NT * h;
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
     qpt[h] = 0.0;
     rpt[h] = 0.0;
@@ -250,42 +255,46 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
   }
 
   // wave partials, fixed shuffle order:
-  //   [0] Q, [1] R, [2,3] vir pos 0 (x = wave, wave + 4), [4,12) pos 1, [12,20) pos 2
+  //   [0] Q, [1] R (= the vir pos-0 partial of x = wave), [4,12) pos 1, [12,20) pos 2
+  static_assert(NPT == 1 && NW == T, "one point per thread, one wave per value of the first index");
   {
-    const double q = wave_sum(qpt[0] + qpt[1]);
+    const double q = wave_sum(qpt[0]);
     if (lane == 0) wred[wv][0] = q;
   }
   if constexpr (BW) {
-    const double ra = wave_sum(rpt[0]), rb = wave_sum(rpt[1]);
-    const double r1 = lane_group_sum<7>(rpt[0] + rpt[1]);    // same al1 = lane bits 3-5
-    const double r2 = lane_group_sum<56>(rpt[0] + rpt[1]);   // same al2 = lane bits 0-2
-    if (lane == 0) {
-      wred[wv][1] = ra + rb;
-      wred[wv][2] = ra;
-      wred[wv][3] = rb;
-    }
+    const double ra = wave_sum(rpt[0]);
+    const double r1 = lane_group_sum<7>(rpt[0]);    // same al1 = lane bits 3-5
+    const double r2 = lane_group_sum<56>(rpt[0]);   // same al2 = lane bits 0-2
+    if (lane == 0) wred[wv][1] = ra;
     if ((lane & 7) == 0) wred[wv][4 + (lane >> 3)] = r1;
     if (lane < 8) wred[wv][12 + lane] = r2;
   }
   __syncthreads();   // every read of sc / cmh is done; wave partials are in place
 
+  // sums over the waves in wave order
+  auto wsum = [&](int k) {
+    double x = wred[0][k];
+#pragma unroll
+    for (int w = 1; w < NW; w++) x += wred[w][k];
+    return x;
+  };
   double* part = tr.part + (long long)cube * EPI_PSTRIDE;
-  if (tid == 0) part[0] = ((wred[0][0] + wred[1][0]) + wred[2][0]) + wred[3][0];
+  if (tid == 0) part[0] = wsum(0);
   if constexpr (!BW) return;
-  if (tid == 1) part[1] = ((wred[0][1] + wred[1][1]) + wred[2][1]) + wred[3][1];
+  if (tid == 1) part[1] = wsum(1);
   if (tid >= 2 && tid < 2 + T) {
-    const int x = tid - 2;
-    part[2 + x] = wred[x % 4][2 + x / 4];
+    const int x = tid - 2;   // pos 0: x is the wave
+    part[2 + x] = wred[x][1];
   } else if (tid >= 2 + T && tid < 2 + 3 * T) {
     const int k = tid - 2 - T;   // 0..15: pos 1 then pos 2
-    part[2 + T + k] = ((wred[0][4 + k] + wred[1][4 + k]) + wred[2][4 + k]) + wred[3][4 + k];
+    part[2 + T + k] = wsum(4 + k);
   }
 
   // G and H into LDS, over the w0 sub-cubes and the CM/CN area
   TP* hl = reinterpret_cast<TP*>(cmh);
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int pt = tid + 256 * h;
+  for (int h = 0; h < NPT; h++) {
+    const int pt = tid + NT * h;
     const int al[3] = {pt / (T * T), (pt / T) % T, pt % T};
     if ((eqAB && al[0] < al[1]) || (eqBC && al[1] < al[2])) continue;
 #pragma unroll
@@ -310,7 +319,7 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
     const long long b0 = (long long)tt[c_perm[p][0]] * T, b1 = (long long)tt[c_perm[p][1]] * T,
                     b2 = (long long)tt[c_perm[p][2]] * T;
     if (distinct) {   // every sub-cube is a different global block and all its points are canonical
-      static_assert(T3 / VW <= 256, "one round");
+      static_assert(T3 / VW <= NT, "one round");
       if (tid < T3 / VW) {
         const int e = tid;
         const int r = e * VW;
@@ -322,10 +331,10 @@ orbit_epilogue(EpiArgs args, const EpiTriple* __restrict__ triples) {
       }
       continue;
     }
-    static_assert(T3 % 256 == 0, "whole rounds");
+    static_assert(T3 % NT == 0, "whole rounds");
 #pragma unroll
-    for (int rr = 0; rr < T3 / 256; rr++) {
-      const int r = tid + 256 * rr;
+    for (int rr = 0; rr < T3 / NT; rr++) {
+      const int r = tid + NT * rr;
       const int x[3] = {r / (T * T), (r / T) % T, r % T};
       bool write = false, owned = false;
 #pragma unroll
@@ -454,7 +463,7 @@ using namespace t3;
 extern "C" hipError_t t3_launch_epilogue(int tp_is_f64, const EpiArgs* args, const EpiTriple* triples,
                                          int ntriple, hipStream_t stream) {
   if (ntriple <= 0) return hipSuccess;
-  dim3 grid(args->ncube, ntriple), block(256);
+  dim3 grid(args->ncube, ntriple), block(EPI_THREADS);
 #define T3_EPI(TP, BW, LNO) hipLaunchKernelGGL((orbit_epilogue<TP, BW, LNO>), grid, block, 0, stream, *args, triples)
   if (args->lno) {
     if (args->backward) {
