@@ -3,7 +3,7 @@
 These paths contract in reduced precision (FP32 accumulation) while the
 denominators, the E(T) sum and all cotangent accumulation stay FP64.
 Tolerances written here: FP32 and bf16x6 |dE| <= 1e-6 |E| and max relative
-cotangent error (||gpu - oracle||_inf / ||oracle||_inf) <= 1e-6; bf16x3
+cotangent error (||gpu - oracle||_inf / ||oracle||_inf) <= 1e-6 (the target bound for FP32 work); bf16x3
 |dE| <= 1e-4 |E| and <= 1e-3 per cotangent.  The bench reports the errors
 actually reached at the benchmark size.
 """
