@@ -136,13 +136,17 @@ int pyscfad_amd_ccsd_t_autotune(int nocc, int nvir, int precision, int *cfg_out,
 size_t pyscfad_amd_ccsd_t_batched_workspace_bytes(int n, const PyscfadAmdInstance *instances,
                                                   int precision, int with_vjp);
 
-/* Runs every instance, in order, on `stream` as captured hipGraphs of at most
- * 2048 kernel nodes each, launched back to back (one graph for small
- * batches); the graphs are cached and replayed when the same call (same
- * instance array contents, precision and workspace) comes again, so new
- * values in the same buffers cost only the graph launches.  workspace is required
- * (256-byte aligned, at least the size above) and must not be used by
- * anything else while the call runs. */
+/* Runs every instance as captured hipGraphs of at most 2048 kernel nodes
+ * each.  The instances are dealt to up to four groups (largest F_E first, to
+ * the group with the least work so far); a group's instances run one after
+ * another on a library-created stream of its own (default priority), which is
+ * forked from `stream` after the descriptor upload and joined back into it,
+ * so the call is ordered on `stream` as a whole.  The graphs are cached and
+ * replayed when the same call (same instance array contents, precision and
+ * workspace) comes again, so new values in the same buffers cost only the
+ * upload and the graph launches.  workspace is required (256-byte aligned, at
+ * least the size above) and must not be used by anything else while the call
+ * runs. */
 int pyscfad_amd_ccsd_t_batched(int n, const PyscfadAmdInstance *instances, int precision,
                                int with_vjp, void *workspace, size_t workspace_bytes, void *stream);
 

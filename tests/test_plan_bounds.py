@@ -6,8 +6,9 @@ the workspace and the caller's in-place outputs (see its header), also for
 the LNO form's plan.  It is
 compiled here with hipcc against the current sources and the kernel objects
 of the last build.  Covers the dense and the density-fitted plans, every
-precision, uneven sizes, batch merging (many triples per output slab) and
-small occupied blocks (cache evictions)."""
+precision, uneven sizes, batch merging (many triples per output slab),
+small occupied blocks (cache evictions) and the workspace layout of the
+batched call (region parts per stream group, descriptor blobs after them)."""
 import os
 import re
 import shutil
@@ -116,3 +117,36 @@ def test_plan_in_bounds_ovvv_adjoint_groups(checker, o, v, prec, batch, group):
     r = _run(checker, o, v, prec, 0, batch=batch, b1_group=group)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "bad=0" in r.stdout and "plan_error=''" in r.stdout, r.stdout
+
+
+def _batched_args():
+    from pyscfad_amd.synthetic import batched_shapes
+    return [x for o, v, _ in batched_shapes(64, 4) for x in (o, v)]
+
+
+@pytest.mark.parametrize("prec", [0, 1, 3])
+def test_batched_plan_in_bounds(checker, prec):
+    """The workspace layout of a batched backward call over the 64 uneven
+    instances (nocc 4-24, nvir 40-200, seed 4): every instance's plan stays in
+    the region part of its stream group and in its own outputs, and each
+    descriptor blob lies after the parts, where its launches read it, apart
+    from the others and inside the size the entry point reports."""
+    r = _run(checker, "batched", prec, *_batched_args())
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "groups=4" in r.stdout and "bad=0" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("batch", [None, 3, 64])
+def test_batched_plan_in_bounds_small_instances(checker, batch):
+    """Tiny instances (as the GPU batched tests use) with the default, a
+    small and the largest batch of triples."""
+    r = _run(checker, "batched", 1, 3, 17, 5, 24, 4, 40, 2, 9, 1, 5, batch=batch)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout, r.stdout
+
+
+def test_batched_checker_reports_overlapping_parts(checker):
+    """Region parts planted 256 bytes too small: the largest instances no
+    longer fit their part, and the checker must say so."""
+    r = _run(checker, "batched_plant", 1, *_batched_args())
+    assert r.returncode != 0 and "exceed the part" in r.stdout, r.stdout
