@@ -224,8 +224,10 @@ def batched_workspace_bytes(problems, precision=DEFAULT_PRECISION, with_vjp=True
 
 def batched(problems, precision=DEFAULT_PRECISION, with_vjp=True, stream=None, workspace=None, outputs=None):
     """E(T) (and, with with_vjp, the cotangents) of many independent instances
-    of uneven size in one call: the instances run in order as one captured
-    hipGraph, replayed when the same call comes again.  ``problems`` is a list
+    of uneven size in one call: the instances, dealt to up to four stream
+    groups (largest first), run as captured hipGraphs on the groups' streams,
+    forked from and joined back into ``stream``, and the graphs are replayed
+    when the same call comes again.  ``problems`` is a list
     of dicts with t1, t2, mo_energy, fock, ovoo, ovov, ovvv (and optionally
     et_bar).  Returns a list of (E, bars) per instance, bars in the order of
     energy_vjp (None without with_vjp).  The arrays are used in place when they
