@@ -962,7 +962,9 @@ struct Ctx {
     M.aux = bw ? 1 : 0;
     launches.push_back(M);
     if (cube_tab == (size_t)-1) {
-      // tile triple of every cube id, in the order cube = A(A+1)(A+2)/6 + B(B+1)/2 + C, A >= B >= C
+      // tile triple of every cube id, in the order cube = A(A+1)(A+2)/6 + B(B+1)/2 + C, A >= B >= C;
+      // 10 bits a tile (nvir < 4096 keeps ntile <= 512)
+      if (d.ntile > 1023) plan_error = "cube table: more than 1023 tiles per virtual edge";
       std::vector<int> tab;
       tab.reserve(d.ncube);
       for (int A = 0; A < d.ntile; A++)
