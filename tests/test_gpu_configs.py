@@ -22,9 +22,11 @@ Tolerances, written per test:
 * FP32 / bf16x6 against FP64: max|d| / max|ref| <= 1e-6 per cotangent and
   |dE| <= 1e-8 |E| (1e-6 Eh absolute on the rescaled instance).  The 64
   uneven instances are held to 2e-6 per cotangent: FP32 operands (t2 and the
-  G cube rounded to 2^-24) leave 1.3e-6 on ovvv_bar of the nocc=5, nvir=189
-  instance, where the contraction over nvir has few occupied triples to
-  average over; at nocc=64, nvir=512 the same path measures 6.5e-7.
+  G cube rounded to 2^-24) leave the largest error on ovvv_bar of instances
+  with few occupied triples to average over in the contraction over nvir
+  (1.3e-6 at nocc=5, nvir=189 with one split count per launch, 9.1e-7 over
+  the draw with split counts sized per problem, the tile choice moving it by
+  the summation order); at nocc=64, nvir=512 the same path measures 6.5e-7.
 """
 import functools
 import gc
