@@ -48,3 +48,23 @@ def test_product_path_fails_loudly_without_gpu():
     with pytest.raises(_lib.T3Error):
         ccsd_t.energy(np.zeros((1, 1)), np.zeros((1, 1, 1, 1)), np.zeros(2), np.zeros((2, 2)),
                       np.zeros((1, 1, 1, 1)), np.zeros((1, 1, 1, 1)), np.zeros((1, 1, 1)))
+
+
+def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):
+    """With the HIP library absent, the Python boundary raises instead of
+    computing anything on the host: the canonical energy, the workspace query
+    and the LNO energy."""
+    from pyscfad_amd import _lib
+    from pyscfad_amd.cc import ccsd_t
+    from pyscfad_amd.lno import ccsd_t as lno_ccsd_t
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libpyscfad_amd.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    z = np.zeros
+    with pytest.raises(_lib.T3Error, match="missing"):
+        ccsd_t.energy(z((1, 1)), z((1, 1, 1, 1)), z(2), z((2, 2)), z((1, 1, 1, 1)), z((1, 1, 1, 1)),
+                      z((1, 1, 1)))
+    with pytest.raises(_lib.T3Error, match="missing"):
+        ccsd_t.workspace_bytes(4, 8)
+    with pytest.raises(_lib.T3Error, match="missing"):
+        lno_ccsd_t.energy(z((1, 1)), z((1, 1)), z((1, 1, 1, 1)), z(2), z((1, 1)), z((1, 1, 1, 1)),
+                          z((1, 1, 1, 1)), z((1, 1, 1)))
