@@ -8,9 +8,12 @@
  *     ovvv (o,v,v(v+1)/2) lower-triangle packed over the last pair
  *          (ovvv_packed = 1), or (o,v,v,v) (ovvv_packed = 0)
  * with o = nocc, v = nvir, n = o + v.  Work is enqueued on `stream` (a
- * hipStream_t passed as void*; NULL is the default stream) and the calls
- * return once it is enqueued and the workspace released (a call that
- * allocates its own workspace synchronises the stream before freeing it).
+ * hipStream_t passed as void*; NULL is the default stream).  A call first
+ * uploads its launch descriptors from host memory on `stream` and waits for
+ * that upload (so for the work already on `stream`), then enqueues its
+ * kernels and returns; a call that allocates its own workspace also
+ * synchronises the stream before freeing it.  The batched call uploads from
+ * pinned memory that it keeps and does not wait.
  *
  * Return value: PYSCFAD_AMD_OK or an error code; pyscfad_amd_last_error()
  * describes the last failure of the calling thread.
@@ -283,8 +286,8 @@ int pyscfad_amd_last_timing(double *ms, long long *launches, int n);
  * applied on the stream).  With workspace_bytes > 0 the last buffer is a
  * caller scratch result of that size (at least
  * pyscfad_amd_ccsd_t_workspace_bytes, 256-byte aligned), so a call makes no
- * device allocation; with 0 the library allocates and frees its own, which
- * synchronises the stream.
+ * device allocation (it still waits for its descriptor upload); with 0 the
+ * library allocates and frees its own, which synchronises the stream.
  * The legacy ABI has no error channel: failures are printed on stderr.
  */
 typedef struct {
