@@ -245,8 +245,9 @@ int main(int argc, char** argv) {
   const size_t t2_size = (size_t)o * o * v * v * 8;
   Stats st;
   const int bad = check_plan(c, WS, ws_size, OUT_T2, OUT_OVOV, t2_size, st);
-  printf("o=%d v=%d prec=%d naux=%d B=%d launches=%d gemm, interleaved=%d, split problems=%d (%d with their own count), "
-         "bad=%d, plan_error='%s'", o, v, prec, naux, c.B, st.nl, st.ninter, st.nsplit, st.nvaried, bad,
+  printf("o=%d v=%d prec=%d naux=%d B=%d S=%d cube=%zu launches=%d gemm, interleaved=%d, split problems=%d "
+         "(%d with their own count), bad=%d, plan_error='%s'", o, v, prec, naux, c.B, c.r.S,
+         (size_t)c.d.V * c.d.V * c.d.V * c.d.elt, st.nl, st.ninter, st.nsplit, st.nvaried, bad,
          c.plan_error.c_str());
   print_cfgs(st);
   return bad ? 1 : 0;

@@ -150,3 +150,31 @@ def test_batched_checker_reports_overlapping_parts(checker):
     longer fit their part, and the checker must say so."""
     r = _run(checker, "batched_plant", 1, *_batched_args())
     assert r.returncode != 0 and "exceed the part" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("o,v,prec,naux,batch,expect", [
+    (64, 512, 1, 0, 2, 4),      # FP32, tuned two triples per batch: four batches
+    (64, 512, 0, 0, 2, 2),      # FP64: two, which keeps the whole solve clear of 288 GB
+    (64, 512, 0, 0, 1, 4),
+    (80, 640, 1, 2160, 1, 4),   # density-fitted o=80/v=640
+    (32, 256, 1, 0, None, None),
+    (12, 100, 0, 0, None, None),
+])
+def test_ovvv_adjoint_group_memory_bound(checker, o, v, prec, naux, batch, expect):
+    """The G cubes that one ovvv-adjoint group keeps resident stay within the
+    cache budget (512 MB) or, for batches whose cubes alone exceed it, within
+    about 4.3 GB and four batches: the bound that keeps an FP64 o=64/v=512
+    backward below the device memory."""
+    r = _run(checker, o, v, prec, naux, batch=batch)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"B=(\d+) S=(\d+) cube=(\d+)", r.stdout)
+    assert m, r.stdout
+    B, S, cube = (int(x) for x in m.groups())
+    held = S * B * cube
+    assert 1 <= S <= 4, r.stdout
+    if cube * B > 512e6:
+        assert held <= 4.3e9, r.stdout
+    else:
+        assert S == 1 or held <= 512e6, r.stdout
+    if expect is not None:
+        assert S == expect, r.stdout
